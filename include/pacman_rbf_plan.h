@@ -1,0 +1,71 @@
+/*
+ * Extended C ABI of the RBF-PUM transfer path: a persistent plan that splits
+ * setup (patch radius, patch centres, membership, weights) from apply
+ * (per-patch solve and partition-of-unity blend for F fields), with device-
+ * or host-resident arrays and explicit algorithm parameters.
+ *
+ * The reference has no counterpart: it re-runs the whole setup on every
+ * call and hard-codes the parameters (src/rbf_pum/interpolator.hxx:39-41,
+ * src/rbf_interface.cpp:62-63). pacman_rbf_interpolate_c is a one-shot plan
+ * with those defaults.
+ *
+ * Conventions: points are row-major [n * spaceDimension] doubles; values are
+ * field-major [nFields * nPoints]. `onDevice` = 1 means the pointers are HIP
+ * device pointers, 0 means host memory. `stream` is a hipStream_t (NULL for
+ * the default stream); calls are synchronous with respect to the host unless
+ * stated otherwise. Every function returns 0 on success and a negative code
+ * on error; pacman_rbf_last_error() gives the message (thread-local).
+ */
+#ifndef PACMAN_AMD_RBF_PLAN_H
+#define PACMAN_AMD_RBF_PLAN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pacman_rbf_plan_s *pacman_rbf_plan;
+
+/* Builds a plan. supportRadius <= 0 selects the patch radius as support. */
+int pacman_rbf_plan_create(pacman_rbf_plan *plan, int spaceDimension,
+                           int rbfFunction, int nodesPerCluster,
+                           double relativeOverlap, double supportRadius,
+                           const double *sourcePoints, long long nSourcePoints,
+                           const double *targetPoints, long long nTargetPoints,
+                           int onDevice, void *stream);
+
+/* targetValues[f * nTarget + i] = transfer of sourceValues[f * nSource + :].
+ * With onDevice = 1 the call is asynchronous on `stream`. */
+int pacman_rbf_plan_apply(pacman_rbf_plan plan, const double *sourceValues,
+                          int nFields, double *targetValues, int onDevice,
+                          void *stream);
+
+int pacman_rbf_plan_destroy(pacman_rbf_plan plan);
+
+/* Introspection (host outputs). */
+double pacman_rbf_plan_radius(pacman_rbf_plan plan);
+double pacman_rbf_plan_support(pacman_rbf_plan plan);
+long long pacman_rbf_plan_num_patches(pacman_rbf_plan plan);
+long long pacman_rbf_plan_num_pairs(pacman_rbf_plan plan);
+int pacman_rbf_plan_max_patch_size(pacman_rbf_plan plan);
+int pacman_rbf_plan_lattice(pacman_rbf_plan plan, long long *shape3,
+                            long long *nCandidates);
+/* centers[K * 3] (padded to 3 coordinates), candidate[K] lattice index. */
+int pacman_rbf_plan_centers(pacman_rbf_plan plan, double *centers,
+                            long long *candidate);
+/* which = 0: per patch, source indices within r; 1: per patch, target
+ * indices within r - 1e-14; 2: per target, patches within r. Lists are in
+ * ascending index order; offsets have (rows + 1) entries. */
+long long pacman_rbf_plan_csr_size(pacman_rbf_plan plan, int which);
+int pacman_rbf_plan_csr(pacman_rbf_plan plan, int which, long long *offsets,
+                        long long *indices);
+/* After an apply: active polynomial columns and pivot status per patch. */
+int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
+                               int *status);
+
+const char *pacman_rbf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PACMAN_AMD_RBF_PLAN_H */

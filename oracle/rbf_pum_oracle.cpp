@@ -1,0 +1,976 @@
+// Oracle: CPU FP64 restatement of PACMAN's RBF-PUM transfer.
+//
+// TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
+// calls this file; only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg use it, and only as the checker.
+//
+// It restates, step by step, the reference algorithm:
+//   FindRadius        src/rbf_pum/clustering/clusters_radius.hpp:22-125
+//   CreateClusters    src/rbf_pum/clustering/clustering.hpp:127-271
+//   callbacks         src/rbf_pum/callbacks.hpp:53-114
+//   Interpolate       src/rbf_pum/solver/solver.hpp:34-289
+//   Wendland kernels  src/rbf_pum/solver/rbf_functions.hpp:22-150
+//   per-patch ops     src/rbf_pum/solver/utils/serial_ops.hpp:28-185,
+//                     src/rbf_pum/solver/utils/serial_solve.hpp:46-110
+//   distances         src/rbf_pum/utils/operators.hpp:125-185
+//   constants         src/common/types.hpp:15-21,98
+// Deliberate differences (documented in the repository's design notes):
+//   * int64 offsets (the reference's int32 offset_t overflows at 4M points);
+//   * members of a patch are listed in ascending point index (the reference
+//     lists them in ArborX traversal order, which only changes rounding);
+//   * nearest-neighbour ties are broken towards the lowest point index;
+//   * the partition-of-unity blend sums each target's contributions in
+//     ascending patch order (the reference uses atomics in arbitrary order);
+//   * the rank-deficient branch refits Q on exactly the active columns
+//     (serial_ops.hpp:124-138 reads a stale column after an axis drop);
+//   * KokkosKernels' SerialQR / SerialLU / SerialSVD (third party, not in the
+//     reference tree, KokkosKernels >= 5.0) are restated from the textbook
+//     algorithms they implement: Householder QR, unpivoted Doolittle LU,
+//     and singular values by one-sided Jacobi.
+// Build with -ffp-contract=off so the geometric decisions (distances,
+// lattice positions) are evaluated exactly as written.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using i64 = int64_t;
+
+constexpr double kEps = 1.0e-14;        // src/common/types.hpp:98
+constexpr double kCondMax = 1.0e5;      // serial_ops.hpp:130
+constexpr double kPruneFactor = 0.4;    // clustering.hpp:228
+
+// ---------------------------------------------------------------------------
+// Distances (src/rbf_pum/utils/operators.hpp:125-185). Coordinates are padded
+// to 3 with zeros for dim < 3; the padding adds exact zeros to the sums.
+// ---------------------------------------------------------------------------
+inline double sqdiff(const double *lhs, const double *rhs) {
+  double acc = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const double d = rhs[i] - lhs[i];
+    acc += d * d;
+  }
+  return acc;
+}
+inline double dist(const double *lhs, const double *rhs) {
+  return std::sqrt(sqdiff(lhs, rhs));
+}
+
+// ---------------------------------------------------------------------------
+// Wendland kernels, src/rbf_pum/solver/rbf_functions.hpp:22-150.
+// kind: 0=C0, 1=C2, 2=C4, 3=C6, 4=C8. Not normalised: phi(0) = 1,1,3,1,15.
+// ---------------------------------------------------------------------------
+inline double wendland(int kind, double r, double rinv) {
+  const double p = r * rinv;
+  const double mask = (r >= 0.0 && p < 1.0 && p >= 0.0) ? 1.0 : 0.0;
+  const double q = 1.0 - p;
+  double v = 0.0;
+  switch (kind) {
+  case 0:
+    v = q * q;
+    break;
+  case 1:
+    v = std::pow(q, 4) * std::fma(4.0, p, 1.0);
+    break;
+  case 2:
+    v = std::pow(q, 6) * (35.0 * std::pow(p, 2) + std::fma(18.0, p, 3.0));
+    break;
+  case 3:
+    v = std::pow(q, 8) * (32.0 * std::pow(p, 3) + 25.0 * std::pow(p, 2) +
+                          std::fma(8.0, p, 1.0));
+    break;
+  case 4:
+    v = std::pow(q, 10) *
+        (1287.0 * std::pow(p, 4) + 1350.0 * std::pow(p, 3) +
+         630.0 * std::pow(p, 2) + std::fma(150.0, p, 15.0));
+    break;
+  default:
+    v = std::numeric_limits<double>::quiet_NaN();
+  }
+  return mask * v;
+}
+
+// ---------------------------------------------------------------------------
+// Uniform cell grid used only to accelerate exact sphere / nearest queries.
+// Every query visits all points whose cells overlap the query cube (padded),
+// then applies the exact predicate, so results equal brute force.
+// ---------------------------------------------------------------------------
+struct Grid {
+  double lo[3] = {0, 0, 0};
+  double h = 1.0;
+  i64 dims[3] = {1, 1, 1};
+  std::vector<i64> start;  // ncell + 1
+  std::vector<i64> items;  // point indices grouped by cell, ascending inside
+  const double *P = nullptr;
+  bool brute = false;
+  i64 n = 0;
+
+  void build(const double *pts, i64 np, double cell, bool bruteForce) {
+    P = pts;
+    n = np;
+    brute = bruteForce || np == 0;
+    if (brute)
+      return;
+    double hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::numeric_limits<double>::max();
+      hi[a] = -std::numeric_limits<double>::max();
+    }
+    for (i64 i = 0; i < np; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], pts[3 * i + a]);
+        hi[a] = std::max(hi[a], pts[3 * i + a]);
+      }
+    h = cell > 0 ? cell : 1.0;
+    for (;;) {
+      i64 tot = 1;
+      for (int a = 0; a < 3; ++a) {
+        dims[a] = (i64)std::floor((hi[a] - lo[a]) / h) + 1;
+        tot *= dims[a];
+      }
+      if (tot <= (i64)1 << 26)
+        break;
+      h *= 1.5;
+    }
+    const i64 nc = dims[0] * dims[1] * dims[2];
+    std::vector<i64> key(np);
+    start.assign(nc + 1, 0);
+    for (i64 i = 0; i < np; ++i) {
+      key[i] = cellOf(pts + 3 * i);
+      start[key[i] + 1]++;
+    }
+    for (i64 c = 0; c < nc; ++c)
+      start[c + 1] += start[c];
+    items.resize(np);
+    std::vector<i64> fill(start.begin(), start.end() - 1);
+    for (i64 i = 0; i < np; ++i)
+      items[fill[key[i]]++] = i;  // ascending i inside each cell
+  }
+  i64 axisCell(int a, double x) const {
+    double t = std::floor((x - lo[a]) / h);
+    if (t < 0)
+      t = 0;
+    if (t > (double)(dims[a] - 1))
+      t = (double)(dims[a] - 1);
+    return (i64)t;
+  }
+  i64 cellOf(const double *x) const {
+    return (axisCell(0, x[0]) * dims[1] + axisCell(1, x[1])) * dims[2] +
+           axisCell(2, x[2]);
+  }
+  // Calls f(i) for every candidate i in cells overlapping [c-rad, c+rad].
+  template <class F> void candidates(const double *c, double rad, F &&f) const {
+    if (brute) {
+      for (i64 i = 0; i < n; ++i)
+        f(i);
+      return;
+    }
+    const double pad = rad * 1e-9 + 1e-300;
+    i64 c0[3], c1[3];
+    for (int a = 0; a < 3; ++a) {
+      c0[a] = axisCell(a, c[a] - rad - pad);
+      c1[a] = axisCell(a, c[a] + rad + pad);
+    }
+    for (i64 x = c0[0]; x <= c1[0]; ++x)
+      for (i64 y = c0[1]; y <= c1[1]; ++y) {
+        const i64 b = (x * dims[1] + y) * dims[2];
+        for (i64 k = start[b + c0[2]]; k < start[b + c1[2] + 1]; ++k)
+          f(items[k]);
+      }
+  }
+};
+
+// Nearest point to c among P (by squared difference; ties -> lowest index),
+// restricted to candidates within rad when a grid is used. Returns -1 when no
+// point lies within rad (grid) or when P is empty.
+i64 nearestWithin(const Grid &g, const double *c, double rad) {
+  i64 best = -1;
+  double bd = std::numeric_limits<double>::infinity();
+  const double r2 = rad * rad;
+  g.candidates(c, rad, [&](i64 i) {
+    const double d2 = sqdiff(c, g.P + 3 * i);
+    if (!g.brute && d2 > r2)
+      return;
+    if (d2 < bd || (d2 == bd && i < best)) {
+      bd = d2;
+      best = i;
+    }
+  });
+  return best;
+}
+
+i64 nearestBrute(const double *P, i64 n, const double *c) {
+  i64 best = -1;
+  double bd = std::numeric_limits<double>::infinity();
+  for (i64 i = 0; i < n; ++i) {
+    const double d2 = sqdiff(c, P + 3 * i);
+    if (d2 < bd) {
+      bd = d2;
+      best = i;
+    }
+  }
+  return best;
+}
+
+// ---------------------------------------------------------------------------
+// Dense per-patch linear algebra (textbook restatements of what the
+// reference calls in KokkosKernels; row-major storage).
+// ---------------------------------------------------------------------------
+
+// Singular values of an n x p matrix (p <= 4) by one-sided Jacobi
+// (Hestenes). Returns them in decreasing order.
+void singularValues(const std::vector<double> &Q, int n, int p, double *S) {
+  std::vector<double> U(Q.begin(), Q.begin() + (size_t)n * p);
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int a = 0; a < p; ++a)
+      for (int b = a + 1; b < p; ++b) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (int i = 0; i < n; ++i) {
+          const double ua = U[(size_t)i * p + a], ub = U[(size_t)i * p + b];
+          alpha += ua * ua;
+          beta += ub * ub;
+          gamma += ua * ub;
+        }
+        if (gamma == 0.0)
+          continue;
+        const double rel = std::fabs(gamma) / std::sqrt(alpha * beta);
+        off = std::max(off, rel);
+        if (rel < 1e-17)
+          continue;
+        const double zeta = (beta - alpha) / (2.0 * gamma);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) /
+                         (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < n; ++i) {
+          const double ua = U[(size_t)i * p + a], ub = U[(size_t)i * p + b];
+          U[(size_t)i * p + a] = cs * ua - sn * ub;
+          U[(size_t)i * p + b] = sn * ua + cs * ub;
+        }
+      }
+    if (off < 1e-16)
+      break;
+  }
+  for (int a = 0; a < p; ++a) {
+    double s = 0;
+    for (int i = 0; i < n; ++i)
+      s += U[(size_t)i * p + a] * U[(size_t)i * p + a];
+    S[a] = std::sqrt(s);
+  }
+  std::sort(S, S + p, [](double x, double y) { return x > y; });
+}
+
+// Least squares min ||Q beta - b|| by Householder QR (n x p, p <= 4).
+// Mirrors SerialSolveQR: QR(Q), X = Q^T b, solve R beta = X[0:p]
+// (serial_solve.hpp:46-67).
+void householderLstsq(std::vector<double> Q, int n, int p, const double *b,
+                      double *beta) {
+  std::vector<double> x(b, b + n);
+  for (int j = 0; j < p && j < n; ++j) {
+    double nrm2 = 0.0;
+    for (int i = j; i < n; ++i)
+      nrm2 += Q[(size_t)i * p + j] * Q[(size_t)i * p + j];
+    const double nrm = std::sqrt(nrm2);
+    if (nrm == 0.0)
+      continue;
+    const double x0 = Q[(size_t)j * p + j];
+    const double alpha = (x0 >= 0 ? -nrm : nrm);
+    // v = column - alpha e_j ; H = I - 2 v v^T / (v^T v)
+    std::vector<double> v(n - j);
+    for (int i = j; i < n; ++i)
+      v[i - j] = Q[(size_t)i * p + j];
+    v[0] -= alpha;
+    double vv = 0.0;
+    for (double e : v)
+      vv += e * e;
+    if (vv == 0.0)
+      continue;
+    for (int c = j; c < p; ++c) {
+      double s = 0.0;
+      for (int i = j; i < n; ++i)
+        s += v[i - j] * Q[(size_t)i * p + c];
+      const double f = 2.0 * s / vv;
+      for (int i = j; i < n; ++i)
+        Q[(size_t)i * p + c] -= f * v[i - j];
+    }
+    double s = 0.0;
+    for (int i = j; i < n; ++i)
+      s += v[i - j] * x[i];
+    const double f = 2.0 * s / vv;
+    for (int i = j; i < n; ++i)
+      x[i] -= f * v[i - j];
+  }
+  // back substitution R beta = x[0:p] (rows beyond n do not exist: a patch
+  // with fewer points than columns keeps those coefficients at zero)
+  for (int j = p - 1; j >= n; --j)
+    beta[j] = 0.0;
+  for (int j = std::min(p, n) - 1; j >= 0; --j) {
+    double s = x[j];
+    for (int c = j + 1; c < p; ++c)
+      s -= Q[(size_t)j * p + c] * beta[c];
+    beta[j] = s / Q[(size_t)j * p + j];
+  }
+}
+
+// Unpivoted Doolittle LU in place + solve (SerialSolveLU,
+// serial_solve.hpp:76-84). A is n x n row-major, b overwritten by the
+// solution.
+void luSolveInPlace(std::vector<double> &A, int n, double *b) {
+  for (int k = 0; k < n; ++k) {
+    const double piv = A[(size_t)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      const double l = A[(size_t)i * n + k] / piv;
+      A[(size_t)i * n + k] = l;
+      if (l == 0.0)
+        continue;
+      double *ri = &A[(size_t)i * n];
+      const double *rk = &A[(size_t)k * n];
+      for (int j = k + 1; j < n; ++j)
+        ri[j] -= l * rk[j];
+    }
+  }
+  for (int i = 0; i < n; ++i) {  // L y = b (unit lower)
+    double s = b[i];
+    for (int j = 0; j < i; ++j)
+      s -= A[(size_t)i * n + j] * b[j];
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {  // U x = y
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j)
+      s -= A[(size_t)i * n + j] * b[j];
+    b[i] = s / A[(size_t)i * n + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pipeline state
+// ---------------------------------------------------------------------------
+struct Params {
+  int dim = 3;
+  int kind = 1;          // Wendland C0..C8 -> 0..4
+  int npc = 50;          // interpolator.hxx:39 default
+  double overlap = 0.15; // interpolator.hxx:40 default
+  double support = 0.1;  // interpolator.hxx:41 default; <= 0 means "= r"
+  int brute = 0;         // 1: brute-force queries (test cross-check)
+};
+
+struct Setup {
+  double lower[3] = {0, 0, 0}, upper[3] = {0, 0, 0};
+  double radius = 0.0;
+  double samples[7][3];       // projected FindRadius probes
+  double sampleMax[7];        // max squared distance per probe
+  i64 shape[3] = {1, 1, 1};   // lattice points per axis (after ++)
+  double spacing[3] = {0, 0, 0};
+  i64 nbCandidates = 0;
+  std::vector<double> centers;        // K x 3 (padded)
+  std::vector<i64> centerCandidate;   // lattice index of each kept centre
+  std::vector<i64> srcOff, srcIdx;    // CSR: sources within r of centre k
+  std::vector<i64> tgtOff, tgtIdx;    // CSR: targets within r - eps
+  std::vector<i64> wOff, wIdx;        // CSR per target: centres within r
+  std::vector<double> w;              // normalised PU weights
+  double minMargin = std::numeric_limits<double>::infinity();
+};
+
+std::vector<double> pad3(const double *p, i64 n, int dim) {
+  std::vector<double> out((size_t)n * 3, 0.0);
+  for (i64 i = 0; i < n; ++i)
+    for (int a = 0; a < dim; ++a)
+      out[3 * i + a] = p[(size_t)i * dim + a];
+  return out;
+}
+
+void bbox(const std::vector<double> &P, i64 n, int dim, double *lo,
+          double *hi) {
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = 0.0;
+    hi[a] = 0.0;
+  }
+  for (int a = 0; a < dim; ++a) {
+    lo[a] = std::numeric_limits<double>::max();
+    hi[a] = -std::numeric_limits<double>::max();
+    for (i64 i = 0; i < n; ++i) {
+      lo[a] = std::min(lo[a], P[3 * i + a]);
+      hi[a] = std::max(hi[a], P[3 * i + a]);
+    }
+  }
+}
+
+// FindRadius, clusters_radius.hpp:22-125.
+void findRadius(const Params &prm, const std::vector<double> &S, i64 ns,
+                Setup &st) {
+  const int D = prm.dim;
+  double center[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a)
+    center[a] = (st.lower[a] + st.upper[a]) / 2.;
+  const int ns7 = 2 * D + 1;
+  double smp[7][3];
+  for (int a = 0; a < 3; ++a)
+    smp[0][a] = center[a];
+  for (int ax = 0; ax < D; ++ax) {
+    double s[3] = {center[0], center[1], center[2]};
+    const double len = std::fabs(st.upper[ax] - st.lower[ax]);
+    s[ax] -= len * 0.25;
+    for (int a = 0; a < 3; ++a)
+      smp[ax + 1][a] = s[a];
+    s[ax] += len * 0.5;
+    for (int a = 0; a < 3; ++a)
+      smp[D + ax + 1][a] = s[a];
+  }
+  std::vector<double> d2(ns);
+  const i64 k = std::min<i64>(prm.npc, ns);
+  for (int q = 0; q < ns7; ++q) {
+    const i64 nn = nearestBrute(S.data(), ns, smp[q]);
+    for (int a = 0; a < 3; ++a)
+      st.samples[q][a] = S[3 * nn + a];
+    for (i64 i = 0; i < ns; ++i)
+      d2[i] = sqdiff(st.samples[q], &S[3 * i]);
+    std::nth_element(d2.begin(), d2.begin() + (k - 1), d2.end());
+    st.sampleMax[q] = d2[k - 1];
+  }
+  std::vector<double> m(st.sampleMax, st.sampleMax + ns7);
+  std::sort(m.begin(), m.end());
+  st.radius = std::sqrt(m[D]);
+}
+
+// Row-major lattice neighbour offsets of ZCurveNeighborOffsets<D>
+// (clustering.hpp:34-116): all 3^D - 1 unit offsets; only the set matters.
+std::vector<i64> neighbourOffsets(int D, const i64 *shape) {
+  std::vector<i64> out;
+  const int total = (D == 1 ? 3 : D == 2 ? 9 : 27);
+  for (int t = 0; t < total; ++t) {
+    int o[3] = {0, 0, 0}, tt = t;
+    bool zero = true;
+    for (int d = 0; d < D; ++d) {
+      o[d] = tt % 3 - 1;
+      tt /= 3;
+      if (o[d])
+        zero = false;
+    }
+    if (zero)
+      continue;
+    i64 idx = 0, stride = 1;
+    for (int d = D - 1; d >= 0; --d) {
+      idx += o[d] * stride;
+      stride *= shape[d];
+    }
+    out.push_back(idx);
+  }
+  return out;
+}
+
+// CreateClusters, clustering.hpp:127-271. `sequentialPrune` selects the
+// literal sequential loop (:230-254); otherwise the equivalent parallel rule.
+void createClusters(const Params &prm, const std::vector<double> &S, i64 ns,
+                    const std::vector<double> &T, i64 nt, const Grid &gs,
+                    const Grid &gt, bool sequentialPrune, Setup &st) {
+  const int D = prm.dim;
+  const double r = st.radius;
+  const double spacing = std::sqrt(4.0 / D) * r * (1.0 - prm.overlap);
+  double distances[3] = {0, 0, 0};
+  double minDistance = std::numeric_limits<double>::max();
+  i64 nb = 1;
+  for (int a = 0; a < D; ++a) {
+    double edge = st.upper[a] - st.lower[a];
+    edge = (edge < 0) ? -edge : edge;
+    const int shp = (int)std::ceil(std::max(1.0, edge / spacing));
+    distances[a] = edge / shp;
+    if (distances[a] < minDistance)
+      minDistance = distances[a];
+    st.shape[a] = shp + 1;
+    nb *= st.shape[a];
+  }
+  for (int a = 0; a < 3; ++a)
+    st.spacing[a] = distances[a];
+  st.nbCandidates = nb;
+  std::vector<double> cand((size_t)nb * 3, 0.0);
+  std::vector<char> alive(nb, 1);
+  const double r2 = r * r;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i64 L = 0; L < nb; ++L) {
+    i64 rem = L, coords[3] = {0, 0, 0};
+    for (int a = D - 1; a >= 0; --a) {
+      coords[a] = rem % st.shape[a];
+      rem /= st.shape[a];
+    }
+    double *c = &cand[3 * L];
+    for (int a = 0; a < D; ++a)
+      c[a] = st.lower[a] + distances[a] * (double)coords[a];
+    // 6. empty of sources -> dead (nearest source farther than r)
+    const i64 ns_ = nearestWithin(gs, c, r);
+    if (ns_ < 0 || sqdiff(c, &S[3 * ns_]) > r2) {
+      alive[L] = 0;
+      continue;
+    }
+    // 7. empty of targets -> dead
+    const i64 nt_ = nearestWithin(gt, c, r);
+    if (nt_ < 0 || sqdiff(c, &T[3 * nt_]) > r2) {
+      alive[L] = 0;
+      continue;
+    }
+    // 8. snap to nearest source
+    for (int a = 0; a < 3; ++a)
+      c[a] = S[3 * ns_ + a];
+  }
+  // 9. prune centres closer than 0.4 * min spacing
+  double thr = kPruneFactor * minDistance;
+  thr *= thr;
+  const std::vector<i64> offs = neighbourOffsets(D, st.shape);
+  if (sequentialPrune) {
+    for (i64 i = 0; i < nb; ++i) {
+      if (!alive[i])
+        continue;
+      for (i64 off : offs) {
+        const i64 j = i + off;
+        if (j >= 0 && j < nb && alive[j] &&
+            sqdiff(&cand[3 * i], &cand[3 * j]) < thr) {
+          alive[i] = 0;
+          break;
+        }
+      }
+    }
+  } else {
+    std::vector<char> before(alive);
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < nb; ++i) {
+      if (!before[i])
+        continue;
+      for (i64 off : offs) {
+        if (off <= 0)
+          continue;
+        const i64 j = i + off;
+        if (j < nb && before[j] && sqdiff(&cand[3 * i], &cand[3 * j]) < thr) {
+          alive[i] = 0;
+          break;
+        }
+      }
+    }
+  }
+  // 10. re-tag against targets from the snapped position
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i64 L = 0; L < nb; ++L) {
+    if (!alive[L])
+      continue;
+    const i64 nt_ = nearestWithin(gt, &cand[3 * L], r);
+    if (nt_ < 0 || sqdiff(&cand[3 * L], &T[3 * nt_]) > r2)
+      alive[L] = 0;
+  }
+  // 11. stable compaction
+  st.centers.clear();
+  st.centerCandidate.clear();
+  for (i64 L = 0; L < nb; ++L)
+    if (alive[L]) {
+      for (int a = 0; a < 3; ++a)
+        st.centers.push_back(cand[3 * L + a]);
+      st.centerCandidate.push_back(L);
+    }
+  (void)ns;
+  (void)nt;
+}
+
+// Sphere membership CSR (solver.hpp:44-67): points with dist(c, x) <= rad.
+void sphereCSR(const std::vector<double> &C, i64 K, const Grid &g, double rad,
+               std::vector<i64> &off, std::vector<i64> &idx, double r,
+               double *margin) {
+  std::vector<std::vector<i64>> lists(K);
+  double mg = std::numeric_limits<double>::infinity();
+#pragma omp parallel for schedule(dynamic, 64) reduction(min : mg)
+  for (i64 k = 0; k < K; ++k) {
+    const double *c = &C[3 * k];
+    auto &lst = lists[k];
+    g.candidates(c, rad, [&](i64 i) {
+      const double d = dist(c, g.P + 3 * i);
+      const double m = std::fabs(d - rad) / r;
+      if (m < mg)
+        mg = m;
+      if (d <= rad)
+        lst.push_back(i);
+    });
+    std::sort(lst.begin(), lst.end());
+  }
+  off.assign(K + 1, 0);
+  for (i64 k = 0; k < K; ++k)
+    off[k + 1] = off[k] + (i64)lists[k].size();
+  idx.resize(off[K]);
+  for (i64 k = 0; k < K; ++k)
+    std::copy(lists[k].begin(), lists[k].end(), idx.begin() + off[k]);
+  if (margin)
+    *margin = std::min(*margin, mg);
+}
+
+void runSetup(const Params &prm, const std::vector<double> &S, i64 ns,
+              const std::vector<double> &T, i64 nt, bool sequentialPrune,
+              Setup &st) {
+  const int D = prm.dim;
+  bbox(S, ns, D, st.lower, st.upper);
+  findRadius(prm, S, ns, st);
+  const double r = st.radius;
+  Grid gs, gt;
+  gs.build(S.data(), ns, r, prm.brute != 0);
+  gt.build(T.data(), nt, r, prm.brute != 0);
+  createClusters(prm, S, ns, T, nt, gs, gt, sequentialPrune, st);
+  const i64 K = (i64)st.centers.size() / 3;
+  st.minMargin = std::numeric_limits<double>::infinity();
+  sphereCSR(st.centers, K, gs, r, st.srcOff, st.srcIdx, r, &st.minMargin);
+  sphereCSR(st.centers, K, gt, r - kEps, st.tgtOff, st.tgtIdx, r,
+            &st.minMargin);
+  // PU weights (solver.hpp:128-169): per target, centres within r.
+  Grid gc;
+  gc.build(st.centers.data(), K, r, prm.brute != 0);
+  std::vector<std::vector<i64>> lists(nt);
+  const double rinv = 1.0 / r;
+  std::vector<std::vector<double>> wl(nt);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i64 t = 0; t < nt; ++t) {
+    const double *y = &T[3 * t];
+    auto &lst = lists[t];
+    gc.candidates(y, r, [&](i64 k) {
+      if (dist(y, &st.centers[3 * k]) <= r)
+        lst.push_back(k);
+    });
+    std::sort(lst.begin(), lst.end());
+    double sum = 0.0;
+    auto &ww = wl[t];
+    for (i64 k : lst) {
+      const double w = wendland(1, dist(y, &st.centers[3 * k]), rinv);
+      sum += w;
+      ww.push_back(w);
+    }
+    for (double &w : ww)
+      w /= sum;
+  }
+  st.wOff.assign(nt + 1, 0);
+  for (i64 t = 0; t < nt; ++t)
+    st.wOff[t + 1] = st.wOff[t] + (i64)lists[t].size();
+  st.wIdx.resize(st.wOff[nt]);
+  st.w.resize(st.wOff[nt]);
+  for (i64 t = 0; t < nt; ++t) {
+    std::copy(lists[t].begin(), lists[t].end(), st.wIdx.begin() + st.wOff[t]);
+    std::copy(wl[t].begin(), wl[t].end(), st.w.begin() + st.wOff[t]);
+  }
+}
+
+// Per-patch solve (solver.hpp:217-286) for F fields; writes the local
+// interpolant at each member target into loc (indexed like tgtIdx).
+struct PatchOut {
+  std::vector<double> local; // tgtIdx.size() x F
+  std::vector<int> pv;       // active polynomial columns per patch
+};
+
+void solvePatches(const Params &prm, const std::vector<double> &S,
+                  const double *vals, i64 ns, int F,
+                  const std::vector<double> &T, const Setup &st, double delta,
+                  i64 kBegin, i64 kEnd, PatchOut &po) {
+  const int D = prm.dim;
+  const i64 K = (i64)st.centers.size() / 3;
+  po.local.assign(st.tgtIdx.size() * (size_t)F, 0.0);
+  po.pv.assign(K, 0);
+  const double dinv = 1.0 / delta;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (i64 k = kBegin; k < kEnd; ++k) {
+    const i64 s0 = st.srcOff[k], n = st.srcOff[k + 1] - s0;
+    const i64 t0 = st.tgtOff[k], m = st.tgtOff[k + 1] - t0;
+    if (n == 0)
+      continue;
+    const i64 *src = &st.srcIdx[s0];
+    const i64 *tgt = &st.tgtIdx[t0];
+    // SerialFillQ (serial_ops.hpp:108-139): conditioning loop on [1, x_a].
+    bool active[3] = {D > 0, D > 1, D > 2};
+    int pv = D + 1;
+    std::vector<double> Q;
+    auto fillQ = [&]() {
+      int cols = 1;
+      for (int a = 0; a < D; ++a)
+        cols += active[a] ? 1 : 0;
+      Q.assign((size_t)n * cols, 0.0);
+      for (i64 i = 0; i < n; ++i) {
+        int c = 0;
+        Q[(size_t)i * cols + c++] = 1.0;
+        for (int a = 0; a < D; ++a)
+          if (active[a])
+            Q[(size_t)i * cols + c++] = S[3 * src[i] + a];
+      }
+      return cols;
+    };
+    do {
+      pv = fillQ();
+      double sv[4];
+      singularValues(Q, (int)n, pv, sv);
+      const double cond = sv[0] / std::max(sv[pv - 1], kEps);
+      if (cond > kCondMax) {
+        // SerialDeactivateOneAxis (serial_ops.hpp:28-57)
+        double best = std::numeric_limits<double>::max();
+        int bi = 0;
+        for (int a = 0; a < D; ++a) {
+          double rng = std::numeric_limits<double>::max();
+          if (active[a]) {
+            double mn = S[3 * src[0] + a], mx = mn;
+            for (i64 i = 1; i < n; ++i) {
+              mn = std::min(mn, S[3 * src[i] + a]);
+              mx = std::max(mx, S[3 * src[i] + a]);
+            }
+            rng = std::fabs(mx - mn);
+          }
+          if (rng < best) {
+            best = rng;
+            bi = a;
+          }
+        }
+        active[bi] = false;
+        --pv;
+        if (pv <= 1)
+          pv = fillQ();
+      } else {
+        break;
+      }
+    } while (pv > 1);
+    po.pv[k] = pv;
+    // A (team_ops.hpp:35-52): phi(|x_i - x_j|), symmetric.
+    std::vector<double> A0((size_t)n * n);
+    for (i64 i = 0; i < n; ++i)
+      for (i64 j = i; j < n; ++j) {
+        const double v =
+            wendland(prm.kind, dist(&S[3 * src[i]], &S[3 * src[j]]), dinv);
+        A0[(size_t)i * n + j] = v;
+        A0[(size_t)j * n + i] = v;
+      }
+    std::vector<double> b(n), beta(pv), A;
+    for (int f = 0; f < F; ++f) {
+      const double *vf = vals + (size_t)f * ns;
+      for (i64 i = 0; i < n; ++i)
+        b[i] = vf[src[i]];
+      householderLstsq(Q, (int)n, pv, b.data(), beta.data());
+      for (i64 i = 0; i < n; ++i) {  // b -= Q beta (solver.hpp:258-263)
+        double s = 0.0;
+        for (int c = 0; c < pv; ++c)
+          s += Q[(size_t)i * pv + c] * beta[c];
+        b[i] -= s;
+      }
+      A = A0;
+      luSolveInPlace(A, (int)n, b.data());
+      // t = E alpha + V beta (solver.hpp:265-273)
+      for (i64 j = 0; j < m; ++j) {
+        const double *y = &T[3 * tgt[j]];
+        double t = 0.0;
+        for (i64 i = 0; i < n; ++i)
+          t += wendland(prm.kind, dist(y, &S[3 * src[i]]), dinv) * b[i];
+        double pvv = beta[0];
+        int c = 1;
+        for (int a = 0; a < D; ++a)
+          if (active[a])
+            pvv += y[a] * beta[c++];
+        po.local[(size_t)(t0 + j) * F + f] = t + pvv;
+      }
+    }
+  }
+}
+
+// PU blend (solver.hpp:275-285), per target in ascending patch order.
+void blend(const Setup &st, const PatchOut &po, i64 nt, int F, double *out) {
+  const i64 K = (i64)st.centers.size() / 3;
+  // For each (patch, member target) find its local value.
+  std::vector<std::vector<std::pair<i64, i64>>> byTarget(nt);
+  for (i64 k = 0; k < K; ++k)
+    for (i64 j = st.tgtOff[k]; j < st.tgtOff[k + 1]; ++j)
+      byTarget[st.tgtIdx[j]].push_back({k, j});
+#pragma omp parallel for schedule(static)
+  for (i64 t = 0; t < nt; ++t) {
+    auto &lst = byTarget[t];
+    std::sort(lst.begin(), lst.end());
+    for (int f = 0; f < F; ++f) {
+      double acc = 0.0;
+      for (auto &kj : lst) {
+        double w = 0.0;
+        for (i64 e = st.wOff[t]; e < st.wOff[t + 1]; ++e)
+          if (st.wIdx[e] == kj.first)
+            w = st.w[e];
+        acc += w * po.local[(size_t)kj.second * F + f];
+      }
+      out[(size_t)f * nt + t] = acc;
+    }
+  }
+}
+
+int kindOf(int rbfFunction) {
+  if (rbfFunction >= 0x10 && rbfFunction <= 0x14)
+    return rbfFunction - 0x10;
+  if (rbfFunction >= 0 && rbfFunction <= 4)
+    return rbfFunction;
+  return -1;
+}
+
+} // namespace
+
+extern "C" {
+
+int oracle_abi_version(void) { return 1; }
+
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0)
+    omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+double oracle_wendland(int rbfFunction, double r, double radiusInv) {
+  return wendland(kindOf(rbfFunction), r, radiusInv);
+}
+
+int oracle_singular_values(const double *Q, int n, int p, double *S) {
+  if (p < 1 || p > 4 || n < 1)
+    return -1;
+  std::vector<double> q(Q, Q + (size_t)n * p);
+  singularValues(q, n, p, S);
+  return 0;
+}
+
+int oracle_lstsq(const double *Q, int n, int p, const double *b,
+                 double *beta) {
+  if (p < 1 || p > 4 || n < p)
+    return -1;
+  householderLstsq(std::vector<double>(Q, Q + (size_t)n * p), n, p, b, beta);
+  return 0;
+}
+
+int oracle_lu_solve(const double *A, int n, double *b) {
+  std::vector<double> a(A, A + (size_t)n * n);
+  luSolveInPlace(a, n, b);
+  return 0;
+}
+
+// Opaque setup handle so tests can inspect each stage.
+struct OracleSetup {
+  Params prm;
+  Setup st;
+  std::vector<double> S, T;
+  i64 ns = 0, nt = 0;
+};
+
+void *oracle_setup_create(int dim, int rbfFunction, int npc, double overlap,
+                          double support, const double *src, int64_t ns,
+                          const double *tgt, int64_t nt, int brute,
+                          int sequentialPrune) {
+  if (dim < 1 || dim > 3 || kindOf(rbfFunction) < 0 || ns <= 0 || nt < 0 ||
+      npc <= 0 || !(overlap > 0 && overlap < 1))
+    return nullptr;
+  auto *o = new OracleSetup;
+  o->prm.dim = dim;
+  o->prm.kind = kindOf(rbfFunction);
+  o->prm.npc = npc;
+  o->prm.overlap = overlap;
+  o->prm.support = support;
+  o->prm.brute = brute;
+  o->S = pad3(src, ns, dim);
+  o->T = pad3(tgt, nt, dim);
+  o->ns = ns;
+  o->nt = nt;
+  runSetup(o->prm, o->S, ns, o->T, nt, sequentialPrune != 0, o->st);
+  return o;
+}
+
+void oracle_setup_destroy(void *h) { delete static_cast<OracleSetup *>(h); }
+
+double oracle_setup_radius(void *h) {
+  return static_cast<OracleSetup *>(h)->st.radius;
+}
+double oracle_setup_min_margin(void *h) {
+  return static_cast<OracleSetup *>(h)->st.minMargin;
+}
+int64_t oracle_setup_num_candidates(void *h) {
+  return static_cast<OracleSetup *>(h)->st.nbCandidates;
+}
+int64_t oracle_setup_num_patches(void *h) {
+  return (int64_t)static_cast<OracleSetup *>(h)->st.centers.size() / 3;
+}
+void oracle_setup_lattice(void *h, int64_t *shape3, double *spacing3,
+                          double *lower3, double *upper3) {
+  auto &st = static_cast<OracleSetup *>(h)->st;
+  for (int a = 0; a < 3; ++a) {
+    shape3[a] = st.shape[a];
+    spacing3[a] = st.spacing[a];
+    lower3[a] = st.lower[a];
+    upper3[a] = st.upper[a];
+  }
+}
+void oracle_setup_probes(void *h, double *samples21, double *maxd2_7) {
+  auto &st = static_cast<OracleSetup *>(h)->st;
+  for (int q = 0; q < 7; ++q) {
+    for (int a = 0; a < 3; ++a)
+      samples21[3 * q + a] = st.samples[q][a];
+    maxd2_7[q] = st.sampleMax[q];
+  }
+}
+// centres (K x 3, padded) and their lattice indices
+void oracle_setup_centers(void *h, double *centers, int64_t *candidate) {
+  auto &st = static_cast<OracleSetup *>(h)->st;
+  std::copy(st.centers.begin(), st.centers.end(), centers);
+  std::copy(st.centerCandidate.begin(), st.centerCandidate.end(), candidate);
+}
+// which: 0 = source members, 1 = target members, 2 = per-target centres
+int64_t oracle_setup_csr_size(void *h, int which) {
+  auto &st = static_cast<OracleSetup *>(h)->st;
+  return which == 0 ? (int64_t)st.srcIdx.size()
+         : which == 1 ? (int64_t)st.tgtIdx.size()
+                      : (int64_t)st.wIdx.size();
+}
+void oracle_setup_csr(void *h, int which, int64_t *off, int64_t *idx) {
+  auto &st = static_cast<OracleSetup *>(h)->st;
+  const auto &o = which == 0 ? st.srcOff : which == 1 ? st.tgtOff : st.wOff;
+  const auto &i = which == 0 ? st.srcIdx : which == 1 ? st.tgtIdx : st.wIdx;
+  std::copy(o.begin(), o.end(), off);
+  std::copy(i.begin(), i.end(), idx);
+}
+void oracle_setup_weights(void *h, double *w) {
+  auto &st = static_cast<OracleSetup *>(h)->st;
+  std::copy(st.w.begin(), st.w.end(), w);
+}
+
+// Solve + blend for F fields (vals: F x ns, field-major). Patches
+// [kBegin, kEnd) only (kEnd < 0: all); out: F x nt. pvOut (K) optional.
+int oracle_setup_apply(void *h, const double *vals, int F, int64_t kBegin,
+                       int64_t kEnd, double *out, int *pvOut) {
+  auto *o = static_cast<OracleSetup *>(h);
+  const i64 K = (i64)o->st.centers.size() / 3;
+  if (kEnd < 0 || kEnd > K)
+    kEnd = K;
+  if (kBegin < 0)
+    kBegin = 0;
+  const double delta = o->prm.support > 0 ? o->prm.support : o->st.radius;
+  PatchOut po;
+  solvePatches(o->prm, o->S, vals, o->ns, F, o->T, o->st, delta, kBegin, kEnd,
+               po);
+  blend(o->st, po, o->nt, F, out);
+  if (pvOut)
+    std::copy(po.pv.begin(), po.pv.end(), pvOut);
+  return 0;
+}
+
+// One-call restatement of PACMAN::rbf_interpolate with explicit parameters.
+int oracle_rbf_interpolate(int dim, int rbfFunction, int npc, double overlap,
+                           double support, const double *src, int64_t ns,
+                           const double *vals, const double *tgt, int64_t nt,
+                           double *out) {
+  void *h = oracle_setup_create(dim, rbfFunction, npc, overlap, support, src,
+                                ns, tgt, nt, 0, 0);
+  if (!h)
+    return -1;
+  oracle_setup_apply(h, vals, 1, 0, -1, out, nullptr);
+  oracle_setup_destroy(h);
+  return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,78 @@
+"""Builds libpacman_rbf.so (HIP kernels + C ABI) in-tree for gfx950.
+
+Run as ``python -m pacman_amd.build`` or through ``__graft_entry__.build()``.
+Each translation unit is compiled by hipcc with its own floating-point
+contraction policy:
+
+* rbf_geometry.hip, rbf_plan.hip: ``-ffp-contract=off`` - distances,
+  lattice coordinates and probe points must round exactly as written so the
+  patch membership is bit-identical to the CPU restatement;
+* rbf_solve.hip: FMA contraction allowed (dense arithmetic).
+
+All HIP code is compiled with the packed-FP32 instructions disabled.
+"""
+
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libpacman_rbf.so")
+ARCH = os.environ.get("PACMAN_AMD_ARCH", "gfx950")
+
+COMMON = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
+]
+
+UNITS = {
+    "rbf_geometry.hip": ["-ffp-contract=off"],
+    "rbf_plan.hip": ["-ffp-contract=off"],
+    "capi.hip": ["-ffp-contract=off"],
+    "rbf_solve.hip": [],
+}
+
+
+def hipcc() -> str:
+    for c in (os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin",
+                           "hipcc"), shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found; the HIP build needs ROCm")
+
+
+def _compile(src: str, extra: list[str]) -> str:
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+        objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
+           tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

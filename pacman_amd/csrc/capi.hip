@@ -1,0 +1,367 @@
+// C ABI and C++ drop-in entry points of libpacman_rbf.so.
+//
+// pacman_rbf_interpolate_c / PACMAN::rbf_interpolate replace the reference's
+// src/fortran_interface.cpp:63-91 and src/rbf_interface.cpp:101-130: same
+// validation order and messages, same error codes, but an execution space
+// this build does not provide is an error, not exit(1).
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <iostream>
+#include <memory>
+#include <string>
+
+#include "../../include/fortran_interface.h"
+#include "../../include/interface.hpp"
+#include "../../include/pacman_rbf_plan.h"
+#include "rbf_plan.hpp"
+
+using pacman_amd::DevBuf;
+using pacman_amd::PlanParams;
+using pacman_amd::RbfPumPlan;
+
+struct pacman_rbf_plan_s {
+  std::unique_ptr<RbfPumPlan> plan;
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+void set_error(const char *where, const std::string &msg) {
+  g_lastError = std::string(where) + ": " + msg;
+}
+
+int kind_of(int rbfFunction) {
+  if (rbfFunction >= 0x10 && rbfFunction <= 0x14)
+    return rbfFunction - 0x10;
+  return -1;
+}
+
+const char *space_name(int e) {
+  switch (e) {
+  case 0:
+    return "SERIAL";
+  case 1:
+    return "OPENMP";
+  case 2:
+    return "THREADS";
+  case 3:
+    return "CUDA";
+  case 4:
+    return "HIP";
+  case 5:
+    return "SYCL";
+  default:
+    return nullptr;
+  }
+}
+
+bool device_present() {
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+// Runs the full transfer on host arrays with the reference's fixed
+// parameters (interpolator.hxx:39-41).
+void run_host_transfer(int dim, int kind, const double *src, int64_t ns,
+                       const double *vals, const double *tgt, int64_t nt,
+                       double *out) {
+  hipStream_t s = nullptr;
+  DevBuf<double> dsrc((size_t)ns * dim), dvals(ns), dtgt((size_t)nt * dim),
+      dout(std::max<int64_t>(nt, 1));
+  PACMAN_HIP_CHECK(hipMemcpyAsync(dsrc.get(), src, 8 * (size_t)ns * dim,
+                                  hipMemcpyHostToDevice, s));
+  PACMAN_HIP_CHECK(
+      hipMemcpyAsync(dvals.get(), vals, 8 * (size_t)ns, hipMemcpyHostToDevice, s));
+  if (nt > 0)
+    PACMAN_HIP_CHECK(hipMemcpyAsync(dtgt.get(), tgt, 8 * (size_t)nt * dim,
+                                    hipMemcpyHostToDevice, s));
+  PlanParams p;
+  p.dim = dim;
+  p.kind = kind;
+  RbfPumPlan plan(p, dsrc.get(), ns, dtgt.get(), nt, s);
+  plan.apply(dvals.get(), 1, dout.get(), s);
+  if (nt > 0)
+    PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
+                                    hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C++ drop-in
+// ---------------------------------------------------------------------------
+namespace PACMAN {
+
+RbfInterpolateResult rbf_interpolate(int_t spaceDimension,
+                                     unsigned char execSpace,
+                                     unsigned char rbfFunction,
+                                     coordinates_t *sourcePoints,
+                                     int_t nSourcePoints, fp_t *sourceValues,
+                                     coordinates_t *targetPoints,
+                                     int_t nTargetPoints) {
+  // src/rbf_interface.cpp:109-114
+  if (sourcePoints == nullptr && nSourcePoints > 0)
+    throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
+  if (targetPoints == nullptr && nTargetPoints > 0)
+    throw std::invalid_argument("targetPoints is null but nTargetPoints > 0");
+  if (sourceValues == nullptr && nSourcePoints > 0)
+    throw std::invalid_argument("sourceValues is null but nSourcePoints > 0");
+  if (spaceDimension < 1 || spaceDimension > 3)
+    throw std::runtime_error(
+        "The dimension of the points can only be: 1, 2 or 3.\n");
+  const char *name = space_name(execSpace);
+  if (name == nullptr)
+    throw std::invalid_argument(
+        "execspace: Serial, OpenMP, Threads, Cuda, HIP, SYCL");
+  if (execSpace != ExecSpaces::HIP)
+    throw std::invalid_argument(std::string("execution space ") + name +
+                                " is not enabled in this build (available: HIP)");
+  const int kind = kind_of(rbfFunction);
+  if (kind < 0)
+    throw std::invalid_argument("Unsupported RBF function selector: " +
+                                std::to_string(rbfFunction));
+  if (nSourcePoints < 0 || nTargetPoints < 0)
+    throw std::invalid_argument("negative number of points");
+  RbfInterpolateResult result;
+  result.targetValues.assign((size_t)nTargetPoints, 0.0);
+  if (nTargetPoints == 0)
+    return result;
+  run_host_transfer(spaceDimension, kind, sourcePoints, nSourcePoints,
+                    sourceValues, targetPoints, nTargetPoints,
+                    result.targetValues.data());
+  return result;
+}
+
+} // namespace PACMAN
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void pacman_kokkos_initialize_c(void) {
+  if (device_present())
+    (void)hipFree(nullptr);  // creates the device context
+}
+
+void pacman_kokkos_finalize_c(void) {}
+
+int pacman_best_execspace_c(void) {
+  return device_present() ? (int)PACMAN_HIP : -1;
+}
+
+int pacman_rbf_interpolate_c(int spaceDimension, int execSpace, int rbfFunction,
+                             const double *sourcePoints, int nSourcePoints,
+                             const double *sourceValues,
+                             const double *targetPoints, int nTargetPoints,
+                             double *targetValues) {
+  try {
+    if (targetValues == nullptr && nTargetPoints > 0)
+      throw std::invalid_argument("targetValues is null but nTargetPoints > 0");
+    PACMAN::RbfInterpolateResult r = PACMAN::rbf_interpolate(
+        spaceDimension, (unsigned char)execSpace, (unsigned char)rbfFunction,
+        const_cast<double *>(sourcePoints), nSourcePoints,
+        const_cast<double *>(sourceValues), const_cast<double *>(targetPoints),
+        nTargetPoints);
+    for (int i = 0; i < nTargetPoints; ++i)
+      targetValues[i] = r.targetValues[(size_t)i];
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_interpolate_c", e.what());
+    std::cerr << "pacman_rbf_interpolate_c: " << e.what() << "\n";
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_interpolate_c", "unknown exception");
+    std::cerr << "pacman_rbf_interpolate_c: unknown exception\n";
+    return -2;
+  }
+}
+
+const char *pacman_rbf_last_error(void) { return g_lastError.c_str(); }
+
+int pacman_rbf_plan_create(pacman_rbf_plan *plan, int spaceDimension,
+                           int rbfFunction, int nodesPerCluster,
+                           double relativeOverlap, double supportRadius,
+                           const double *sourcePoints, long long nSourcePoints,
+                           const double *targetPoints, long long nTargetPoints,
+                           int onDevice, void *stream) {
+  try {
+    if (!plan)
+      throw std::invalid_argument("plan is null");
+    *plan = nullptr;
+    if (sourcePoints == nullptr && nSourcePoints > 0)
+      throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
+    if (targetPoints == nullptr && nTargetPoints > 0)
+      throw std::invalid_argument("targetPoints is null but nTargetPoints > 0");
+    if (spaceDimension < 1 || spaceDimension > 3)
+      throw std::runtime_error(
+          "The dimension of the points can only be: 1, 2 or 3.");
+    const int kind = kind_of(rbfFunction);
+    if (kind < 0)
+      throw std::invalid_argument("Unsupported RBF function selector: " +
+                                  std::to_string(rbfFunction));
+    hipStream_t s = (hipStream_t)stream;
+    PlanParams p;
+    p.dim = spaceDimension;
+    p.kind = kind;
+    p.npc = nodesPerCluster;
+    p.overlap = relativeOverlap;
+    p.support = supportRadius;
+    auto h = std::make_unique<pacman_rbf_plan_s>();
+    if (onDevice) {
+      h->plan = std::make_unique<RbfPumPlan>(p, sourcePoints, nSourcePoints,
+                                             targetPoints, nTargetPoints, s);
+    } else {
+      DevBuf<double> ds((size_t)std::max<long long>(nSourcePoints, 0) *
+                        spaceDimension),
+          dt((size_t)std::max<long long>(nTargetPoints, 0) * spaceDimension);
+      if (nSourcePoints > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(ds.get(), sourcePoints,
+                                        8 * ds.n, hipMemcpyHostToDevice, s));
+      if (nTargetPoints > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(dt.get(), targetPoints,
+                                        8 * dt.n, hipMemcpyHostToDevice, s));
+      h->plan = std::make_unique<RbfPumPlan>(p, ds.get(), nSourcePoints,
+                                             dt.get(), nTargetPoints, s);
+    }
+    *plan = h.release();
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_create", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_create", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_rbf_plan_apply(pacman_rbf_plan plan, const double *sourceValues,
+                          int nFields, double *targetValues, int onDevice,
+                          void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    RbfPumPlan &P = *plan->plan;
+    hipStream_t s = (hipStream_t)stream;
+    if (nFields < 1)
+      throw std::invalid_argument("nFields must be >= 1");
+    if (sourceValues == nullptr || (targetValues == nullptr && P.numTargets() > 0))
+      throw std::invalid_argument("null values pointer");
+    if (onDevice) {
+      P.apply(sourceValues, nFields, targetValues, s);
+    } else {
+      DevBuf<double> dv((size_t)nFields * P.numSources()),
+          dout((size_t)nFields * std::max<int64_t>(P.numTargets(), 1));
+      PACMAN_HIP_CHECK(hipMemcpyAsync(dv.get(), sourceValues, 8 * dv.n,
+                                      hipMemcpyHostToDevice, s));
+      P.apply(dv.get(), nFields, dout.get(), s);
+      if (P.numTargets() > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(
+            targetValues, dout.get(),
+            8 * (size_t)nFields * P.numTargets(), hipMemcpyDeviceToHost, s));
+      PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_apply", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_apply", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_rbf_plan_destroy(pacman_rbf_plan plan) {
+  delete plan;
+  return 0;
+}
+
+double pacman_rbf_plan_radius(pacman_rbf_plan plan) {
+  return plan && plan->plan ? plan->plan->radius() : -1.0;
+}
+
+double pacman_rbf_plan_support(pacman_rbf_plan plan) {
+  return plan && plan->plan ? plan->plan->support() : -1.0;
+}
+
+long long pacman_rbf_plan_num_patches(pacman_rbf_plan plan) {
+  return plan && plan->plan ? plan->plan->numPatches() : -1;
+}
+
+long long pacman_rbf_plan_num_pairs(pacman_rbf_plan plan) {
+  return plan && plan->plan ? plan->plan->numPairs() : -1;
+}
+
+int pacman_rbf_plan_max_patch_size(pacman_rbf_plan plan) {
+  return plan && plan->plan ? plan->plan->maxPatchSize() : -1;
+}
+
+int pacman_rbf_plan_lattice(pacman_rbf_plan plan, long long *shape3,
+                            long long *nCandidates) {
+  if (!plan || !plan->plan)
+    return -1;
+  for (int a = 0; a < 3; ++a)
+    shape3[a] = plan->plan->latticeShape()[a];
+  *nCandidates = plan->plan->numCandidates();
+  return 0;
+}
+
+int pacman_rbf_plan_centers(pacman_rbf_plan plan, double *centers,
+                            long long *candidate) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    plan->plan->centers(centers, reinterpret_cast<int64_t *>(candidate),
+                        nullptr);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_centers", e.what());
+    return -1;
+  }
+}
+
+long long pacman_rbf_plan_csr_size(pacman_rbf_plan plan, int which) {
+  try {
+    if (!plan || !plan->plan || which < 0 || which > 2)
+      throw std::invalid_argument("bad plan or list selector");
+    return plan->plan->csrSize(which);
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_csr_size", e.what());
+    return -1;
+  }
+}
+
+int pacman_rbf_plan_csr(pacman_rbf_plan plan, int which, long long *offsets,
+                        long long *indices) {
+  try {
+    if (!plan || !plan->plan || which < 0 || which > 2)
+      throw std::invalid_argument("bad plan or list selector");
+    plan->plan->csr(which, reinterpret_cast<int64_t *>(offsets),
+                    reinterpret_cast<int64_t *>(indices), nullptr);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_csr", e.what());
+    return -1;
+  }
+}
+
+int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
+                               int *status) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    plan->plan->patchInfo(polyColumns, status, nullptr);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_patch_info", e.what());
+    return -1;
+  }
+}
+
+} // extern "C"

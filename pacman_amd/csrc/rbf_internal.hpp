@@ -1,0 +1,167 @@
+// Internal host-side interfaces between the plan orchestration and the
+// kernel translation units. Nothing here is exported from the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rbf_common.hpp"
+
+#define PACMAN_HIP_CHECK(expr)                                                 \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess)                                                      \
+      throw std::runtime_error(std::string("HIP error: ") +                    \
+                               hipGetErrorString(_e) + " at " + __FILE__ +     \
+                               ":" + std::to_string(__LINE__) + " (" #expr ")"); \
+  } while (0)
+
+namespace pacman_amd {
+
+// Owning device buffer (hipMalloc / hipFree).
+template <typename T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count)
+      PACMAN_HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T *get() const { return p; }
+};
+
+// Cell-sorted point set owned on the device.
+struct DeviceGrid {
+  GridView view{};
+  DevBuf<int> cell_start, perm;
+  DevBuf<double> x, y, z;
+  int64_t ncell = 0;
+};
+
+// ---- geometry (rbf_geometry.hip) ------------------------------------------
+void bbox_device(const double *pts, int64_t n, int dim, double *lo,
+                 double *hi, hipStream_t s);
+void build_grid(const double *pts, int64_t n, int dim, const double *lo,
+                const double *hi, double pointsPerCell, DeviceGrid &g,
+                hipStream_t s);
+// Exclusive scan of n int32 counts into n+1 int64 offsets; returns the total.
+int64_t exclusive_scan_i32_to_i64(const int *in, int64_t *out, int64_t n,
+                                  hipStream_t s);
+// Exclusive scan of n int32 flags into n+1 int32 offsets; returns the total.
+int exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s);
+
+struct ProbeResult {
+  double maxd2[7];
+  int nearest[7];  // cell-order position of each projected probe
+};
+void probe_radius(const GridView &src, const double *probes, int nprobe,
+                  int k, const double *lo, const double *hi, ProbeResult &res,
+                  hipStream_t s);
+
+struct Lattice {
+  int dim;
+  int64_t shape[3];
+  double lower[3];
+  double dist[3];
+  int64_t nb;
+  double r, r2, thr;
+  std::vector<int64_t> posOffsets;  // strictly positive neighbour offsets
+};
+// Tag, snap, prune, re-tag and compact; returns the centres (SoA, device)
+// and their candidate index.
+int64_t make_centers(const Lattice &L, const GridView &src,
+                     const GridView &tgt, DevBuf<double> &cx,
+                     DevBuf<double> &cy, DevBuf<double> &cz,
+                     DevBuf<int64_t> &cand, hipStream_t s);
+
+// Patch membership CSR for sources (dist <= r).
+void patch_sources(const GridView &src, const double *cx, const double *cy,
+                   const double *cz, int64_t K, double r,
+                   DevBuf<int64_t> &off, DevBuf<int> &idx, hipStream_t s);
+// Patch/target pairs (dist <= r) with the member flag (dist <= r - eps),
+// and the per-target transpose (slots in ascending patch order).
+struct PairSet {
+  DevBuf<int64_t> off;      // K + 1: pairs of patch k
+  DevBuf<int> tgt;          // P: target cell-order position | member << 31
+  DevBuf<int> slot;         // P: slot of the pair in its target's list
+  DevBuf<int64_t> toff;     // M + 1: slots of target t
+  DevBuf<int> slotPatch;    // P: patch of each slot
+  DevBuf<unsigned char> slotMember;  // P
+  int64_t P = 0;
+};
+void patch_targets(const GridView &tgt, const double *cx, const double *cy,
+                   const double *cz, int64_t K, double r, PairSet &ps,
+                   hipStream_t s);
+
+// ---- solve (rbf_solve.hip) ------------------------------------------------
+struct SolveArgs {
+  const int *patches;       // patch ids of this bin
+  int count;                // patches in this bin
+  const int64_t *srcOff;    // K + 1
+  const int *srcIdx;        // cell-order positions (source grid)
+  const double *sx, *sy, *sz;
+  const double *vals;       // F x N, cell order
+  int64_t N;
+  int F;
+  const int64_t *pairOff;   // K + 1
+  const int *pairTgt;       // target position | member << 31
+  const int *pairSlot;
+  const double *tx, *ty, *tz;  // target coordinates, cell order
+  double *contrib;          // P x F
+  double dinv;              // 1 / support radius
+  int dim;
+  int *pvOut;               // K (optional) active polynomial columns
+  int *status;              // K (optional) nonzero on a zero pivot
+};
+// Launches the fused FP64 per-patch kernel for the bin with R row blocks.
+void launch_solve_f64(int R, int kind, const SolveArgs &a, hipStream_t s);
+
+struct BlendArgs {
+  const int64_t *toff;      // M + 1
+  const int *slotPatch;
+  const unsigned char *slotMember;
+  const double *contrib;    // P x F
+  const double *tx, *ty, *tz;
+  const int *tperm;         // cell order -> original target index
+  const double *cx, *cy, *cz;
+  int64_t M;
+  int F;
+  double rinv;
+  double *out;              // F x M, original order
+};
+void launch_blend(const BlendArgs &a, hipStream_t s);
+
+void launch_gather_values(const double *vals, const int *perm, int64_t N,
+                          int F, double *out, hipStream_t s);
+
+} // namespace pacman_amd

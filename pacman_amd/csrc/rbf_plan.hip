@@ -1,0 +1,344 @@
+// Host orchestration of the RBF-PUM plan. Follows the constructor sequence
+// of src/rbf_pum/interpolator.hpp:58-94: FindRadius, SetRadiusInv on the RBF
+// (1 / support) and on the PU weight (1 / r), CreateClusters, Interpolate.
+// Compiled with -ffp-contract=off so every host-side formula (probe points,
+// lattice spacing) rounds exactly as the CPU restatement does.
+
+#include "rbf_plan.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+namespace pacman_amd {
+
+namespace {
+// Largest patch the register-resident kernel takes (16 x 12 rows).
+constexpr int kMaxPatch = 192;
+constexpr int kFieldsPerLaunch = 8;
+
+int tile_for(int n) {
+  // Register tile R (rows per thread x 16) rounded up to an even count.
+  int R = (n + 15) / 16;
+  R += R & 1;
+  return std::max(R, 2);
+}
+} // namespace
+
+RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
+                       const double *dTgt, int64_t M, hipStream_t s)
+    : prm_(p), N_(N), M_(M) {
+  const int D = p.dim;
+  if (D < 1 || D > 3)
+    throw std::runtime_error("The dimension of the points can only be: 1, 2 or 3.");
+  if (p.kind < 0 || p.kind > 4)
+    throw std::invalid_argument("Unsupported RBF function selector");
+  if (N <= 0)
+    throw std::invalid_argument("RBF-PUM needs at least one source point");
+  if (M < 0)
+    throw std::invalid_argument("negative number of target points");
+  if (p.npc <= 0)
+    throw std::invalid_argument("nodesPerCluster must be positive");
+  if (!(p.overlap > 0.0 && p.overlap < 1.0))
+    throw std::invalid_argument("relativeOverlap must lie in (0, 1)");
+  if (N > INT_MAX / 2 || M > INT_MAX / 2)
+    throw std::invalid_argument("point counts above 2^30 are not supported");
+
+  // Bounding boxes (the source box is the BVH bounds of the reference).
+  bbox_device(dSrc, N, D, lo_, hi_, s);
+  bbox_device(dTgt, M, D, tlo_, thi_, s);
+  build_grid(dSrc, N, D, lo_, hi_, 4.0, gs_, s);
+  build_grid(dTgt, M, D, tlo_, thi_, 4.0, gt_, s);
+
+  // FindRadius (clusters_radius.hpp:22-125)
+  double center[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < D; ++a)
+    center[a] = (lo_[a] + hi_[a]) / 2.;
+  const int nprobe = 2 * D + 1;
+  double probes[7][3];
+  for (int a = 0; a < 3; ++a)
+    probes[0][a] = center[a];
+  for (int ax = 0; ax < D; ++ax) {
+    double smp[3] = {center[0], center[1], center[2]};
+    const double len = std::fabs(hi_[ax] - lo_[ax]);
+    smp[ax] -= len * 0.25;
+    for (int a = 0; a < 3; ++a)
+      probes[ax + 1][a] = smp[a];
+    smp[ax] += len * 0.5;
+    for (int a = 0; a < 3; ++a)
+      probes[D + ax + 1][a] = smp[a];
+  }
+  ProbeResult pr{};
+  const int k = (int)std::min<int64_t>(p.npc, N);
+  probe_radius(gs_.view, &probes[0][0], nprobe, k, lo_, hi_, pr, s);
+  std::vector<double> m(pr.maxd2, pr.maxd2 + nprobe);
+  std::sort(m.begin(), m.end());
+  r_ = std::sqrt(m[D]);
+  if (!(r_ > 0.0))
+    throw std::runtime_error(
+        "patch radius is zero (all probe neighbourhoods are single points)");
+  delta_ = p.support > 0.0 ? p.support : r_;
+
+  // CreateClusters lattice (clustering.hpp:137-204)
+  lat_.dim = D;
+  const double spacing = std::sqrt(4.0 / D) * r_ * (1.0 - p.overlap);
+  double minDistance = DBL_MAX;
+  int64_t nb = 1;
+  for (int a = 0; a < 3; ++a) {
+    lat_.shape[a] = 1;
+    lat_.lower[a] = 0.0;
+    lat_.dist[a] = 0.0;
+  }
+  for (int a = 0; a < D; ++a) {
+    double edge = hi_[a] - lo_[a];
+    edge = (edge < 0) ? (-edge) : (edge);
+    const int shp = (int)std::ceil(std::max(1.0, edge / spacing));
+    lat_.dist[a] = edge / shp;
+    if (lat_.dist[a] < minDistance)
+      minDistance = lat_.dist[a];
+    lat_.shape[a] = shp + 1;
+    lat_.lower[a] = lo_[a];
+    nb *= lat_.shape[a];
+  }
+  lat_.nb = nb;
+  lat_.r = r_;
+  lat_.r2 = r_ * r_;
+  double thr = 0.4 * minDistance;
+  thr *= thr;
+  lat_.thr = thr;
+  // ZCurveNeighborOffsets<D> (clustering.hpp:34-116): the 3^D - 1 row-major
+  // unit offsets. Only the positive half is needed by the parallel rule.
+  const int total = D == 1 ? 3 : D == 2 ? 9 : 27;
+  for (int t = 0; t < total; ++t) {
+    int o[3] = {0, 0, 0}, tt = t;
+    bool zero = true;
+    for (int d = 0; d < D; ++d) {
+      o[d] = tt % 3 - 1;
+      tt /= 3;
+      zero = zero && o[d] == 0;
+    }
+    if (zero)
+      continue;
+    int64_t idx = 0, stride = 1;
+    for (int d = D - 1; d >= 0; --d) {
+      idx += o[d] * stride;
+      stride *= lat_.shape[d];
+    }
+    if (idx > 0)
+      lat_.posOffsets.push_back(idx);
+  }
+  K_ = make_centers(lat_, gs_.view, gt_.view, cx_, cy_, cz_, cand_, s);
+
+  // Interpolate(): membership (solver.hpp:44-67) and weights (:128-169)
+  patch_sources(gs_.view, cx_.get(), cy_.get(), cz_.get(), K_, r_, srcOff_,
+                srcIdx_, s);
+  patch_targets(gt_.view, cx_.get(), cy_.get(), cz_.get(), K_, r_, pairs_, s);
+
+  // Size bins by register tile.
+  std::vector<int64_t> off(K_ + 1);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(off.data(), srcOff_.get(), 8 * (K_ + 1),
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  maxN_ = 0;
+  std::vector<std::vector<int>> bins(kMaxPatch / 16 + 2);
+  for (int64_t kk = 0; kk < K_; ++kk) {
+    const int n = (int)(off[kk + 1] - off[kk]);
+    maxN_ = std::max(maxN_, n);
+    if (n > kMaxPatch)
+      continue;
+    bins[tile_for(n)].push_back((int)kk);
+  }
+  if (maxN_ > kMaxPatch)
+    throw std::runtime_error(
+        "a patch holds " + std::to_string(maxN_) +
+        " source points; this build solves patches of up to " +
+        std::to_string(kMaxPatch) + " points (lower nodesPerCluster)");
+  for (int R = 0; R < (int)bins.size(); ++R) {
+    if (bins[R].empty())
+      continue;
+    DevBuf<int> d(bins[R].size());
+    PACMAN_HIP_CHECK(hipMemcpyAsync(d.get(), bins[R].data(),
+                                    4 * bins[R].size(), hipMemcpyHostToDevice,
+                                    s));
+    binDev_.push_back(std::move(d));
+    binPatches_.push_back(std::move(bins[R]));
+    binR_.push_back(R);
+  }
+  pv_.alloc(std::max<int64_t>(K_, 1));
+  status_.alloc(std::max<int64_t>(K_, 1));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
+                       hipStream_t s) {
+  if (F < 1)
+    throw std::invalid_argument("apply needs at least one field");
+  if (M_ == 0)
+    return;
+  const int64_t P = std::max<int64_t>(pairs_.P, 1);
+  if (valsSorted_.n < (size_t)(F * N_))
+    valsSorted_.alloc((size_t)F * N_);
+  launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
+  const int Fc = std::min(F, kFieldsPerLaunch);
+  if (contrib_.n < (size_t)(P * Fc))
+    contrib_.alloc((size_t)P * Fc);
+  for (int f0 = 0; f0 < F; f0 += kFieldsPerLaunch) {
+    const int nf = std::min(kFieldsPerLaunch, F - f0);
+    SolveArgs a{};
+    a.srcOff = srcOff_.get();
+    a.srcIdx = srcIdx_.get();
+    a.sx = gs_.view.x;
+    a.sy = gs_.view.y;
+    a.sz = gs_.view.z;
+    a.vals = valsSorted_.get() + (int64_t)f0 * N_;
+    a.N = N_;
+    a.F = nf;
+    a.pairOff = pairs_.off.get();
+    a.pairTgt = pairs_.tgt.get();
+    a.pairSlot = pairs_.slot.get();
+    a.tx = gt_.view.x;
+    a.ty = gt_.view.y;
+    a.tz = gt_.view.z;
+    a.contrib = contrib_.get();
+    a.dinv = 1.0 / delta_;
+    a.dim = prm_.dim;
+    a.pvOut = pv_.get();
+    a.status = status_.get();
+    for (size_t b = 0; b < binR_.size(); ++b) {
+      a.patches = binDev_[b].get();
+      a.count = (int)binPatches_[b].size();
+      launch_solve_f64(binR_[b], prm_.kind, a, s);
+    }
+    BlendArgs bl{};
+    bl.toff = pairs_.toff.get();
+    bl.slotPatch = pairs_.slotPatch.get();
+    bl.slotMember = pairs_.slotMember.get();
+    bl.contrib = contrib_.get();
+    bl.tx = gt_.view.x;
+    bl.ty = gt_.view.y;
+    bl.tz = gt_.view.z;
+    bl.tperm = gt_.view.perm;
+    bl.cx = cx_.get();
+    bl.cy = cy_.get();
+    bl.cz = cz_.get();
+    bl.M = M_;
+    bl.F = nf;
+    bl.rinv = 1.0 / r_;  // interpolator.hpp:91
+    bl.out = dOut + (int64_t)f0 * M_;
+    launch_blend(bl, s);
+  }
+}
+
+void RbfPumPlan::centers(double *xyz, int64_t *candidate, hipStream_t s) const {
+  std::vector<double> x(K_), y(K_), z(K_);
+  if (K_ > 0) {
+    PACMAN_HIP_CHECK(hipMemcpyAsync(x.data(), cx_.get(), 8 * K_,
+                                    hipMemcpyDeviceToHost, s));
+    PACMAN_HIP_CHECK(hipMemcpyAsync(y.data(), cy_.get(), 8 * K_,
+                                    hipMemcpyDeviceToHost, s));
+    PACMAN_HIP_CHECK(hipMemcpyAsync(z.data(), cz_.get(), 8 * K_,
+                                    hipMemcpyDeviceToHost, s));
+    if (candidate)
+      PACMAN_HIP_CHECK(hipMemcpyAsync(candidate, cand_.get(), 8 * K_,
+                                      hipMemcpyDeviceToHost, s));
+  }
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  for (int64_t k = 0; k < K_; ++k) {
+    xyz[3 * k] = x[k];
+    xyz[3 * k + 1] = y[k];
+    xyz[3 * k + 2] = z[k];
+  }
+}
+
+int64_t RbfPumPlan::csrSize(int which) const {
+  if (which == 0) {
+    int64_t tot = 0;
+    PACMAN_HIP_CHECK(hipMemcpy(&tot, srcOff_.get() + K_, 8,
+                               hipMemcpyDeviceToHost));
+    return tot;
+  }
+  if (which == 2)
+    return pairs_.P;
+  // member pairs only
+  std::vector<int> t(pairs_.P);
+  if (pairs_.P)
+    PACMAN_HIP_CHECK(hipMemcpy(t.data(), pairs_.tgt.get(), 4 * pairs_.P,
+                               hipMemcpyDeviceToHost));
+  int64_t c = 0;
+  for (int v : t)
+    c += ((unsigned)v >> 31) ? 1 : 0;
+  return c;
+}
+
+void RbfPumPlan::csr(int which, int64_t *off, int64_t *idx,
+                     hipStream_t s) const {
+  if (which == 0) {
+    std::vector<int64_t> o(K_ + 1);
+    PACMAN_HIP_CHECK(hipMemcpyAsync(o.data(), srcOff_.get(), 8 * (K_ + 1),
+                                    hipMemcpyDeviceToHost, s));
+    PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<int> pos(o[K_]), perm(N_);
+    if (o[K_])
+      PACMAN_HIP_CHECK(hipMemcpy(pos.data(), srcIdx_.get(), 4 * o[K_],
+                                 hipMemcpyDeviceToHost));
+    PACMAN_HIP_CHECK(hipMemcpy(perm.data(), gs_.view.perm, 4 * N_,
+                               hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k <= K_; ++k)
+      off[k] = o[k];
+    for (int64_t k = 0; k < K_; ++k) {
+      for (int64_t e = o[k]; e < o[k + 1]; ++e)
+        idx[e] = perm[pos[e]];
+      std::sort(idx + o[k], idx + o[k + 1]);
+    }
+    return;
+  }
+  std::vector<int64_t> po(K_ + 1);
+  std::vector<int> t(pairs_.P), tperm(M_);
+  PACMAN_HIP_CHECK(hipMemcpy(po.data(), pairs_.off.get(), 8 * (K_ + 1),
+                             hipMemcpyDeviceToHost));
+  if (pairs_.P)
+    PACMAN_HIP_CHECK(hipMemcpy(t.data(), pairs_.tgt.get(), 4 * pairs_.P,
+                               hipMemcpyDeviceToHost));
+  if (M_)
+    PACMAN_HIP_CHECK(hipMemcpy(tperm.data(), gt_.view.perm, 4 * M_,
+                               hipMemcpyDeviceToHost));
+  if (which == 1) {
+    int64_t w = 0;
+    off[0] = 0;
+    for (int64_t k = 0; k < K_; ++k) {
+      const int64_t b = w;
+      for (int64_t e = po[k]; e < po[k + 1]; ++e)
+        if ((unsigned)t[e] >> 31)
+          idx[w++] = tperm[t[e] & 0x7fffffff];
+      std::sort(idx + b, idx + w);
+      off[k + 1] = w;
+    }
+    return;
+  }
+  // which == 2: per original target, the patches within r (ascending)
+  std::vector<std::vector<int64_t>> lists(M_);
+  for (int64_t k = 0; k < K_; ++k)
+    for (int64_t e = po[k]; e < po[k + 1]; ++e)
+      lists[tperm[t[e] & 0x7fffffff]].push_back(k);
+  off[0] = 0;
+  for (int64_t i = 0; i < M_; ++i) {
+    std::sort(lists[i].begin(), lists[i].end());
+    std::copy(lists[i].begin(), lists[i].end(), idx + off[i]);
+    off[i + 1] = off[i] + (int64_t)lists[i].size();
+  }
+}
+
+void RbfPumPlan::patchInfo(int *pv, int *status, hipStream_t s) const {
+  if (K_ == 0)
+    return;
+  if (pv)
+    PACMAN_HIP_CHECK(hipMemcpyAsync(pv, pv_.get(), 4 * K_,
+                                    hipMemcpyDeviceToHost, s));
+  if (status)
+    PACMAN_HIP_CHECK(hipMemcpyAsync(status, status_.get(), 4 * K_,
+                                    hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+} // namespace pacman_amd

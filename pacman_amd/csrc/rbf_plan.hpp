@@ -1,0 +1,69 @@
+// RBF-PUM transfer plan: setup (search, clustering, membership) once, apply
+// (per-patch solve + partition-of-unity blend) for any number of fields.
+// Host-side orchestration of the kernels in rbf_geometry.hip / rbf_solve.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "rbf_internal.hpp"
+
+namespace pacman_amd {
+
+struct PlanParams {
+  int dim = 3;
+  int kind = 1;            // Wendland C0..C8 -> 0..4
+  int npc = 50;            // nodes per cluster (interpolator.hxx:39)
+  double overlap = 0.15;   // relative overlap (interpolator.hxx:40)
+  double support = 0.1;    // Wendland support radius; <= 0 means the patch radius
+};
+
+class RbfPumPlan {
+public:
+  RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
+             const double *dTgt, int64_t M, hipStream_t s);
+
+  // dVals: F x N (field-major, original point order), dOut: F x M.
+  void apply(const double *dVals, int F, double *dOut, hipStream_t s);
+
+  const PlanParams &params() const { return prm_; }
+  int64_t numSources() const { return N_; }
+  int64_t numTargets() const { return M_; }
+  double radius() const { return r_; }
+  double support() const { return delta_; }
+  int64_t numPatches() const { return K_; }
+  int64_t numCandidates() const { return lat_.nb; }
+  const int64_t *latticeShape() const { return lat_.shape; }
+  int64_t numPairs() const { return pairs_.P; }
+  int maxPatchSize() const { return maxN_; }
+
+  // Host copies for inspection and parity tests (original point indices,
+  // each list sorted ascending).
+  void centers(double *xyz, int64_t *candidate, hipStream_t s) const;
+  int64_t csrSize(int which) const;  // 0 sources, 1 member targets, 2 weights
+  void csr(int which, int64_t *off, int64_t *idx, hipStream_t s) const;
+  void patchInfo(int *pv, int *status, hipStream_t s) const;
+
+private:
+  PlanParams prm_;
+  int64_t N_ = 0, M_ = 0, K_ = 0;
+  double lo_[3], hi_[3], tlo_[3], thi_[3];
+  double r_ = 0.0, delta_ = 0.0;
+  Lattice lat_{};
+  DeviceGrid gs_, gt_;
+  DevBuf<double> cx_, cy_, cz_;
+  DevBuf<int64_t> cand_;
+  DevBuf<int64_t> srcOff_;
+  DevBuf<int> srcIdx_;
+  PairSet pairs_;
+  std::vector<std::vector<int>> binPatches_;  // by register tile R
+  std::vector<DevBuf<int>> binDev_;
+  std::vector<int> binR_;
+  int maxN_ = 0;
+  DevBuf<int> pv_, status_;
+  DevBuf<double> valsSorted_, contrib_;
+};
+
+} // namespace pacman_amd

@@ -1,0 +1,237 @@
+"""RBF-PUM transfer: the drop-in ``pacman.rbf`` submodule plus a plan API.
+
+``interpolate`` mirrors ``pacman.rbf.interpolate``
+(src/pybindings/bindings.cpp:75-85, src/pybindings/rbf_bindings.cpp:74-120):
+same argument names and order, the same ValueError checks, float64 output.
+Two deliberate differences: an unsupported execution space or basis raises
+ValueError instead of ending the process (rbf_bindings.hpp:50-53), and a bad
+dimension raises RuntimeError instead of throwing a C++ pointer
+(rbf_bindings.cpp:117).
+
+``Plan`` exposes the setup/apply split with explicit parameters, device
+(torch) or host (numpy) arrays and F fields per apply.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import types
+
+import numpy as np
+
+from . import _lib
+from . import execspaces as _es
+
+functions = types.SimpleNamespace(
+    WENDLANDC0=0x10, WENDLANDC2=0x11, WENDLANDC4=0x12, WENDLANDC6=0x13,
+    WENDLANDC8=0x14)
+
+_BASES = {0x10, 0x11, 0x12, 0x13, 0x14}
+
+# Reference parameters of the drop-in call (src/rbf_pum/interpolator.hxx:39-41)
+DEFAULT_NODES_PER_CLUSTER = 50
+DEFAULT_RELATIVE_OVERLAP = 0.15
+DEFAULT_SUPPORT_RADIUS = 0.1
+
+
+def _as_f64(a) -> np.ndarray:
+    if hasattr(a, "detach"):  # torch tensor
+        a = a.detach().cpu().numpy()
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _check_selectors(space_dimension, execspace, rbf_function):
+    if int(execspace) not in _es._NAMES:
+        raise ValueError("execspace: Serial, OpenMP, Threads, Cuda, HIP, SYCL")
+    if _es.name(execspace) not in _es.available():
+        raise ValueError(
+            f"execution space {_es.name(execspace)} is not enabled in this "
+            f"build (available: {', '.join(_es.available())})")
+    if int(rbf_function) not in _BASES:
+        raise ValueError("function: WendlandC0, WendlandC2, WendlandC4, "
+                         "WendlandC6, WendlandC8")
+    if int(space_dimension) not in (1, 2, 3):
+        raise RuntimeError(
+            "The dimension of the points can only be: 1, 2 or 3.\n")
+
+
+def interpolate(space_dimension, execspace, rbf_function, source_points,
+                source_values, target_points):
+    """RBF-PUM transfer of source_values to target_points (float64 array)."""
+    sp = _as_f64(source_points)
+    sv = _as_f64(source_values)
+    tp = _as_f64(target_points)
+    if sp.ndim != 2:
+        raise ValueError("Source points must be a 2-d numpy array!")
+    if tp.ndim != 2:
+        raise ValueError("Target points must be a 2-d numpy array!")
+    if sp.shape[1] != tp.shape[1]:
+        raise ValueError(
+            "Dimensions mismatch between source points and target "
+            f"points!\nProvided source points dimension: {sp.shape[1]}"
+            f"\nProvided target points dimension: {tp.shape[1]}\n")
+    if sp.shape[0] != sv.shape[0]:
+        raise ValueError(
+            "Each source point must have its corresponding value in the "
+            f"source values!\nNumber of source points: {sp.shape[0]}"
+            f"\nNumber of source values: {sv.shape[0]}\n")
+    _check_selectors(space_dimension, execspace, rbf_function)
+    if sp.shape[1] != int(space_dimension):
+        raise ValueError(
+            f"space_dimension={space_dimension} but the points have "
+            f"{sp.shape[1]} coordinates")
+    lib = _lib.load()
+    out = np.zeros(tp.shape[0], dtype=np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib.pacman_rbf_interpolate_c(
+        int(space_dimension), int(execspace), int(rbf_function),
+        sp.ctypes.data_as(dp), sp.shape[0], sv.ctypes.data_as(dp),
+        tp.ctypes.data_as(dp), tp.shape[0], out.ctypes.data_as(dp))
+    if rc != 0:
+        raise RuntimeError(_lib.last_error())
+    return out
+
+
+def _ptr(a):
+    """(address, on_device) of a numpy array or a torch tensor."""
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr()), int(a.is_cuda)
+    return ctypes.c_void_p(a.ctypes.data), 0
+
+
+class Plan:
+    """Persistent RBF-PUM plan (setup once, apply to any number of fields).
+
+    Points may be numpy arrays (host) or float64 torch tensors on the GPU;
+    both point sets must live in the same place. ``support_radius <= 0``
+    uses the patch radius as the Wendland support.
+    """
+
+    def __init__(self, space_dimension, rbf_function, source_points,
+                 target_points, nodes_per_cluster=DEFAULT_NODES_PER_CLUSTER,
+                 relative_overlap=DEFAULT_RELATIVE_OVERLAP,
+                 support_radius=DEFAULT_SUPPORT_RADIUS, stream=None):
+        self._lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        self.dim = int(space_dimension)
+        if int(rbf_function) not in _BASES:
+            raise ValueError("unsupported rbf_function")
+        src, tgt = source_points, target_points
+        if not hasattr(src, "data_ptr"):
+            src, tgt = _as_f64(src), _as_f64(tgt)
+        else:
+            src, tgt = src.contiguous(), tgt.contiguous()
+            if str(src.dtype) != "torch.float64" or str(tgt.dtype) != "torch.float64":
+                raise ValueError("points must be float64")
+        self._keep = (src, tgt)
+        self.num_sources = int(src.shape[0])
+        self.num_targets = int(tgt.shape[0])
+        ps, dev = _ptr(src)
+        pt, devt = _ptr(tgt)
+        if dev != devt:
+            raise ValueError("source and target points must both be on the "
+                             "host or both on the device")
+        rc = self._lib.pacman_rbf_plan_create(
+            ctypes.byref(self._h), self.dim, int(rbf_function),
+            int(nodes_per_cluster), float(relative_overlap),
+            float(support_radius), ps, self.num_sources, pt,
+            self.num_targets, dev, ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        self.on_device = bool(dev)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.pacman_rbf_plan_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    # -- apply -------------------------------------------------------------
+    def apply(self, source_values, out=None, stream=None):
+        """Transfers F fields: values (F, N) or (N,) -> (F, M) or (M,)."""
+        v = source_values
+        squeeze = len(v.shape) == 1
+        F = 1 if squeeze else int(v.shape[0])
+        if hasattr(v, "data_ptr"):
+            import torch
+            v = v.contiguous()
+            if out is None:
+                out = torch.empty((F, self.num_targets), dtype=torch.float64,
+                                  device=v.device)
+        else:
+            v = _as_f64(v)
+            if out is None:
+                out = np.zeros((F, self.num_targets))
+        pv, dev = _ptr(v)
+        po, devo = _ptr(out)
+        if dev != devo:
+            raise ValueError("values and output must live in the same place")
+        rc = self._lib.pacman_rbf_plan_apply(self._h, pv, F, po, dev,
+                                             ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return out.reshape(-1) if squeeze else out
+
+    # -- introspection -----------------------------------------------------
+    @property
+    def radius(self) -> float:
+        return self._lib.pacman_rbf_plan_radius(self._h)
+
+    @property
+    def support(self) -> float:
+        return self._lib.pacman_rbf_plan_support(self._h)
+
+    @property
+    def num_patches(self) -> int:
+        return int(self._lib.pacman_rbf_plan_num_patches(self._h))
+
+    @property
+    def num_pairs(self) -> int:
+        return int(self._lib.pacman_rbf_plan_num_pairs(self._h))
+
+    @property
+    def max_patch_size(self) -> int:
+        return int(self._lib.pacman_rbf_plan_max_patch_size(self._h))
+
+    def lattice(self):
+        shape = (ctypes.c_longlong * 3)()
+        nb = ctypes.c_longlong()
+        self._lib.pacman_rbf_plan_lattice(self._h, shape, ctypes.byref(nb))
+        return np.array(shape[:], dtype=np.int64), int(nb.value)
+
+    def centers(self):
+        K = self.num_patches
+        c = np.zeros((K, 3))
+        cand = np.zeros(K, np.int64)
+        rc = self._lib.pacman_rbf_plan_centers(
+            self._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            cand.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return c, cand
+
+    def csr(self, which: int):
+        rows = self.num_patches if which in (0, 1) else self.num_targets
+        size = self._lib.pacman_rbf_plan_csr_size(self._h, int(which))
+        if size < 0:
+            raise RuntimeError(_lib.last_error())
+        off = np.zeros(rows + 1, np.int64)
+        idx = np.zeros(max(size, 1), np.int64)
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        rc = self._lib.pacman_rbf_plan_csr(self._h, int(which),
+                                           off.ctypes.data_as(llp),
+                                           idx.ctypes.data_as(llp))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return off, idx[:size]
+
+    def patch_info(self):
+        K = self.num_patches
+        pv = np.zeros(K, np.int32)
+        st = np.zeros(K, np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        rc = self._lib.pacman_rbf_plan_patch_info(self._h, pv.ctypes.data_as(ip),
+                                                  st.ctypes.data_as(ip))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return pv, st
