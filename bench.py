@@ -1,0 +1,171 @@
+#!/usr/bin/env python
+"""RBF-PUM setup+apply throughput on one MI355X.
+
+A step is one full transfer, setup (patch radius, patch centres,
+membership, weights) plus apply (per-patch solve and blend), on inputs
+already resident in HBM. Default workload: 4,096,000 sources on a jittered
+160^3 grid (seed 2) -> 4,096,000 uniform targets (seed 3), Wendland C2,
+about 150 points per patch (nodes per cluster 150, overlap 0.15), support
+radius = patch radius, FP64, one field (franke_3d).
+
+Prints one JSON line. The CPU baseline is the FP64 oracle (a restatement of
+the reference algorithm) timed on this host on a bounded sub-cube of the
+same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector = matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--rbf", type=lambda s: int(s, 0), default=0x11)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-edge", type=float, default=0.5,
+                    help="CPU baseline sample: sub-cube [0, edge)^3")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
+              "supported in this build (patch sharding across GPUs is a "
+              "design note only).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import pacman_amd as pacman
+    from pacman_amd import workloads as W
+
+    log(f"[bench] generating {args.config}")
+    S, T, npc, support = W.config(args.config)
+    v = W.franke_3d(*S.T)
+    N, M = len(S), len(T)
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(S).to(dev)
+    dT = torch.from_numpy(T).to(dev)
+    dv = torch.from_numpy(v).to(dev)
+    torch.cuda.synchronize()
+
+    def step():
+        plan = pacman.rbf.Plan(3, args.rbf, dS, dT, npc, 0.15, support)
+        out = plan.apply(dv)
+        return plan, out
+
+    for i in range(args.warmup):
+        plan, out = step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i} done: K={plan.num_patches}")
+        del plan, out
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        if i:
+            del plan, out
+        plan, out = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    log(f"[bench] {args.steps} steps: {ms:.2f} ms/step")
+
+    ph = plan.phase_times()
+    work = plan.work()
+    exact = torch.from_numpy(W.franke_3d(*T.T)).to(dev)
+    rel_l2 = float(torch.linalg.norm(out - exact) / torch.linalg.norm(exact))
+
+    # Dominant kernel: the fused per-patch solve (one launch per size bin).
+    # Algorithmic flops per patch (LU-equivalent count of the reference's
+    # per-patch chain): (2/3) n^3 + 2 n^2 + 2 n m.
+    f_alg = (2.0 / 3.0) * work["sum_n3"] + 2 * work["sum_n2"] + \
+        2 * work["sum_nm"]
+    f_exec = (1.0 / 3.0) * work["sum_n3"] + 2 * work["sum_n2"] + \
+        2 * work["sum_nm"]
+    solve_s = ph["solve"] / 1e3
+    achieved = f_alg / solve_s / 1e12
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"solve_f64_{args.config}_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_solve_phase")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        e = args.cpu_edge
+        ms_ = np.all(S < e, axis=1)
+        mt_ = np.all(T < e, axis=1)
+        Ss, Ts, vs = S[ms_], T[mt_], v[ms_]
+        cores = min(16, os.cpu_count() or 1)
+        oracle.set_threads(cores)
+        log(f"[bench] cpu baseline: oracle on {len(Ss)} -> {len(Ts)} "
+            f"({cores} threads)")
+        c0 = time.perf_counter()
+        o = oracle.Setup(3, args.rbf, Ss, Ts, npc, 0.15, support)
+        o.apply(vs)
+        c1 = time.perf_counter()
+        cpu = {"value": len(Ts) / (c1 - c0) / 1e6, "unit": "Mpts/s",
+               "cores": cores, "kind": "port",
+               "sample": f"{args.config} restricted to [0,{e})^3: "
+                         f"{len(Ss)} sources -> {len(Ts)} targets, same "
+                         f"parameters, FP64 oracle setup+apply, "
+                         f"{c1 - c0:.2f} s"}
+
+    res = {
+        "metric": "RBF-PUM setup+apply Mpts/s (FP64 plain path)",
+        "value": M / (ms / 1e3) / 1e6,
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: {N} jittered-grid sources -> {M} "
+                        f"uniform targets, Wendland 0x{args.rbf:02x}, "
+                        f"{npc} nodes per cluster, overlap 0.15, support = "
+                        "patch radius, 1 field",
+            "patches": int(work["patches"]),
+            "max_patch_size": int(work["max_n"]),
+            "phase_ms": {k: round(x, 3) for k, x in ph.items()},
+            "rel_l2_vs_franke_3d": rel_l2,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "solve_f64_kernel (all size-bin launches of one apply)",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": traffic,
+            "executed_flops_frac": f_exec / solve_s / 1e12 / PEAK_FP64_TFLOPS,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

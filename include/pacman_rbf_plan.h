@@ -62,6 +62,17 @@ int pacman_rbf_plan_csr(pacman_rbf_plan plan, int which, long long *offsets,
 int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
                                int *status);
 
+/* Phase times in milliseconds (up to n entries; returns the count written):
+ * setup phases 0 grids, 1 patch radius, 2 patch centres, 3 membership,
+ * 4 setup total (host clock); apply phases of the last apply (HIP events on
+ * its stream) 5 value gather, 6 per-patch solve, 7 blend, 8 apply total. */
+int pacman_rbf_plan_phase_times(pacman_rbf_plan plan, double *ms, int n);
+
+/* Work counters (up to n entries; returns the count written): 0 patches K,
+ * 1 sum n_k, 2 sum n_k^2, 3 sum n_k^3, 4 sum n_k m_k (member targets),
+ * 5 target/patch pairs within r, 6 largest n_k. */
+int pacman_rbf_plan_work(pacman_rbf_plan plan, double *work, int n);
+
 const char *pacman_rbf_last_error(void);
 
 #ifdef __cplusplus

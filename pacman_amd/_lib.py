@@ -44,6 +44,8 @@ SIGNATURES = {
     "pacman_rbf_plan_csr_size": (ctypes.c_longlong, [_vp, ctypes.c_int]),
     "pacman_rbf_plan_csr": (ctypes.c_int, [_vp, ctypes.c_int, _llp, _llp]),
     "pacman_rbf_plan_patch_info": (ctypes.c_int, [_vp, _ip, _ip]),
+    "pacman_rbf_plan_phase_times": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "pacman_rbf_plan_work": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "pacman_rbf_last_error": (ctypes.c_char_p, []),
 }
 

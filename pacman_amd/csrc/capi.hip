@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -360,6 +361,38 @@ int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
     return 0;
   } catch (const std::exception &e) {
     set_error("pacman_rbf_plan_patch_info", e.what());
+    return -1;
+  }
+}
+
+int pacman_rbf_plan_phase_times(pacman_rbf_plan plan, double *ms, int n) {
+  try {
+    if (!plan || !plan->plan || !ms)
+      throw std::invalid_argument("plan or output is null");
+    double t[RbfPumPlan::kPhases];
+    plan->plan->phaseTimes(t);
+    const int c = std::min(n, RbfPumPlan::kPhases);
+    for (int i = 0; i < c; ++i)
+      ms[i] = t[i];
+    return c;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_phase_times", e.what());
+    return -1;
+  }
+}
+
+int pacman_rbf_plan_work(pacman_rbf_plan plan, double *work, int n) {
+  try {
+    if (!plan || !plan->plan || !work)
+      throw std::invalid_argument("plan or output is null");
+    double w[RbfPumPlan::kWork];
+    plan->plan->work(w);
+    const int c = std::min(n, RbfPumPlan::kWork);
+    for (int i = 0; i < c; ++i)
+      work[i] = w[i];
+    return c;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_work", e.what());
     return -1;
   }
 }

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <climits>
 #include <cmath>
 
@@ -17,6 +18,12 @@ namespace {
 // Largest patch the register-resident kernel takes (16 x 12 rows).
 constexpr int kMaxPatch = 192;
 constexpr int kFieldsPerLaunch = 8;
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now() - t0)
+      .count();
+}
 
 int tile_for(int n) {
   // Register tile R (rows per thread x 16) rounded up to an even count.
@@ -45,11 +52,15 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   if (N > INT_MAX / 2 || M > INT_MAX / 2)
     throw std::invalid_argument("point counts above 2^30 are not supported");
 
+  const auto tSetup = std::chrono::steady_clock::now();
+  auto tPhase = tSetup;
   // Bounding boxes (the source box is the BVH bounds of the reference).
   bbox_device(dSrc, N, D, lo_, hi_, s);
   bbox_device(dTgt, M, D, tlo_, thi_, s);
   build_grid(dSrc, N, D, lo_, hi_, 4.0, gs_, s);
   build_grid(dTgt, M, D, tlo_, thi_, 4.0, gt_, s);
+  setupMs_[0] = ms_since(tPhase);
+  tPhase = std::chrono::steady_clock::now();
 
   // FindRadius (clusters_radius.hpp:22-125)
   double center[3] = {0.0, 0.0, 0.0};
@@ -79,6 +90,8 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
     throw std::runtime_error(
         "patch radius is zero (all probe neighbourhoods are single points)");
   delta_ = p.support > 0.0 ? p.support : r_;
+  setupMs_[1] = ms_since(tPhase);
+  tPhase = std::chrono::steady_clock::now();
 
   // CreateClusters lattice (clustering.hpp:137-204)
   lat_.dim = D;
@@ -129,6 +142,8 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
       lat_.posOffsets.push_back(idx);
   }
   K_ = make_centers(lat_, gs_.view, gt_.view, cx_, cy_, cz_, cand_, s);
+  setupMs_[2] = ms_since(tPhase);
+  tPhase = std::chrono::steady_clock::now();
 
   // Interpolate(): membership (solver.hpp:44-67) and weights (:128-169)
   patch_sources(gs_.view, cx_.get(), cy_.get(), cz_.get(), K_, r_, srcOff_,
@@ -167,7 +182,66 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   }
   pv_.alloc(std::max<int64_t>(K_, 1));
   status_.alloc(std::max<int64_t>(K_, 1));
+  for (auto &e : ev_)
+    PACMAN_HIP_CHECK(hipEventCreate(&e));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  setupMs_[3] = ms_since(tPhase);
+  setupMs_[4] = ms_since(tSetup);
+}
+
+RbfPumPlan::~RbfPumPlan() {
+  for (auto &e : ev_)
+    if (e)
+      (void)hipEventDestroy(e);
+}
+
+void RbfPumPlan::phaseTimes(double *ms) const {
+  for (int i = 0; i < 5; ++i)
+    ms[i] = setupMs_[i];
+  for (int i = 5; i < kPhases; ++i)
+    ms[i] = 0.0;
+  if (!applied_)
+    return;
+  PACMAN_HIP_CHECK(hipEventSynchronize(ev_[3]));
+  float t = 0.f;
+  PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[0], ev_[1]));
+  ms[5] = t;
+  PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[1], ev_[2]));
+  ms[6] = t;
+  PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[2], ev_[3]));
+  ms[7] = t;
+  PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[0], ev_[3]));
+  ms[8] = t;
+}
+
+void RbfPumPlan::work(double *w) const {
+  std::vector<int64_t> off(K_ + 1), po(K_ + 1);
+  PACMAN_HIP_CHECK(hipMemcpy(off.data(), srcOff_.get(), 8 * (K_ + 1),
+                             hipMemcpyDeviceToHost));
+  PACMAN_HIP_CHECK(hipMemcpy(po.data(), pairs_.off.get(), 8 * (K_ + 1),
+                             hipMemcpyDeviceToHost));
+  std::vector<int> t(pairs_.P);
+  if (pairs_.P)
+    PACMAN_HIP_CHECK(hipMemcpy(t.data(), pairs_.tgt.get(), 4 * pairs_.P,
+                               hipMemcpyDeviceToHost));
+  double sn = 0, sn2 = 0, sn3 = 0, snm = 0;
+  for (int64_t k = 0; k < K_; ++k) {
+    const double n = (double)(off[k + 1] - off[k]);
+    int64_t m = 0;
+    for (int64_t e = po[k]; e < po[k + 1]; ++e)
+      m += ((unsigned)t[e] >> 31) ? 1 : 0;
+    sn += n;
+    sn2 += n * n;
+    sn3 += n * n * n;
+    snm += n * (double)m;
+  }
+  w[0] = (double)K_;
+  w[1] = sn;
+  w[2] = sn2;
+  w[3] = sn3;
+  w[4] = snm;
+  w[5] = (double)pairs_.P;
+  w[6] = (double)maxN_;
 }
 
 void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
@@ -179,7 +253,10 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
   const int64_t P = std::max<int64_t>(pairs_.P, 1);
   if (valsSorted_.n < (size_t)(F * N_))
     valsSorted_.alloc((size_t)F * N_);
+  PACMAN_HIP_CHECK(hipEventRecord(ev_[0], s));
   launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
+  PACMAN_HIP_CHECK(hipEventRecord(ev_[1], s));
+  std::vector<BlendArgs> blends;
   const int Fc = std::min(F, kFieldsPerLaunch);
   if (contrib_.n < (size_t)(P * Fc))
     contrib_.alloc((size_t)P * Fc);
@@ -226,8 +303,17 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
     bl.F = nf;
     bl.rinv = 1.0 / r_;  // interpolator.hpp:91
     bl.out = dOut + (int64_t)f0 * M_;
-    launch_blend(bl, s);
+    if (F <= kFieldsPerLaunch) {
+      blends.push_back(bl);  // blend after all solves: phases stay separable
+    } else {
+      launch_blend(bl, s);
+    }
   }
+  PACMAN_HIP_CHECK(hipEventRecord(ev_[2], s));
+  for (const BlendArgs &bl : blends)
+    launch_blend(bl, s);
+  PACMAN_HIP_CHECK(hipEventRecord(ev_[3], s));
+  applied_ = true;
 }
 
 void RbfPumPlan::centers(double *xyz, int64_t *candidate, hipStream_t s) const {

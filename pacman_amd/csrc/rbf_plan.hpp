@@ -46,6 +46,18 @@ public:
   void csr(int which, int64_t *off, int64_t *idx, hipStream_t s) const;
   void patchInfo(int *pv, int *status, hipStream_t s) const;
 
+  // Phase times in ms. Setup phases are host-timed (each ends with a
+  // synchronisation): 0 grids, 1 radius, 2 centres, 3 membership, 4 setup
+  // total. Apply phases come from HIP events of the last apply on its
+  // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total.
+  static constexpr int kPhases = 9;
+  void phaseTimes(double *ms) const;
+  // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
+  // targets), 5 pairs (targets within r of a centre), 6 max n.
+  static constexpr int kWork = 7;
+  void work(double *w) const;
+  ~RbfPumPlan();
+
 private:
   PlanParams prm_;
   int64_t N_ = 0, M_ = 0, K_ = 0;
@@ -64,6 +76,9 @@ private:
   int maxN_ = 0;
   DevBuf<int> pv_, status_;
   DevBuf<double> valsSorted_, contrib_;
+  double setupMs_[5] = {0, 0, 0, 0, 0};
+  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool applied_ = false;
 };
 
 } // namespace pacman_amd

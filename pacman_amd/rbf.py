@@ -225,6 +225,28 @@ class Plan:
             raise RuntimeError(_lib.last_error())
         return off, idx[:size]
 
+    PHASES = ("grids", "radius", "centres", "membership", "setup",
+              "gather", "solve", "blend", "apply")
+    WORK = ("patches", "sum_n", "sum_n2", "sum_n3", "sum_nm", "pairs",
+            "max_n")
+
+    def phase_times(self) -> dict:
+        """Milliseconds per phase (setup: host clock; apply: HIP events)."""
+        t = np.zeros(len(self.PHASES))
+        c = self._lib.pacman_rbf_plan_phase_times(
+            self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(t))
+        if c < 0:
+            raise RuntimeError(_lib.last_error())
+        return dict(zip(self.PHASES, t.tolist()))
+
+    def work(self) -> dict:
+        w = np.zeros(len(self.WORK))
+        c = self._lib.pacman_rbf_plan_work(
+            self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(w))
+        if c < 0:
+            raise RuntimeError(_lib.last_error())
+        return dict(zip(self.WORK, w.tolist()))
+
     def patch_info(self):
         K = self.num_patches
         pv = np.zeros(K, np.int32)

@@ -1,0 +1,123 @@
+"""C-ABI / drop-in boundary checks that need no GPU: the library loads, it
+exports every symbol the public headers declare, the constants equal the
+reference's, and argument validation fails the way the reference does
+(before any device work)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pacman_amd as pacman
+from pacman_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def header_functions():
+    names = set()
+    for h in os.listdir(INCLUDE):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(pacman_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_library_exports_every_declared_symbol():
+    syms = exported()
+    declared = header_functions()
+    assert len(declared) >= 18
+    missing = sorted(declared - syms)
+    assert not missing, missing
+    # the C++ drop-in PACMAN::rbf_interpolate(int, uchar, uchar, double*, int,
+    # double*, double*, int)
+    assert "_ZN6PACMAN15rbf_interpolateEihhPdiS0_S0_i" in syms
+
+
+def test_ctypes_signature_table_covers_headers():
+    assert header_functions() == set(_lib.SIGNATURES)
+    lib = _lib.load()
+    for name in _lib.SIGNATURES:
+        getattr(lib, name)
+
+
+def test_constants_match_reference():
+    # src/common/execspaces.hpp:17-24, src/interface.hpp:25-31,
+    # src/fortran_interface.h:33-49
+    es = pacman.execspaces
+    assert (es.SERIAL, es.OPENMP, es.THREADS, es.CUDA, es.HIP, es.SYCL) == \
+        (0, 1, 2, 3, 4, 5)
+    f = pacman.rbf.functions
+    assert (f.WENDLANDC0, f.WENDLANDC2, f.WENDLANDC4, f.WENDLANDC6,
+            f.WENDLANDC8) == (0x10, 0x11, 0x12, 0x13, 0x14)
+    hdr = open(os.path.join(INCLUDE, "fortran_interface.h")).read()
+    for name, val in (("PACMAN_HIP", "0x04"), ("PACMAN_SERIAL", "0x00"),
+                      ("PACMAN_WENDLANDC0", "0x10"),
+                      ("PACMAN_WENDLANDC8", "0x14")):
+        assert re.search(rf"{name}\s*=\s*{val}", hdr)
+    assert es.available() == ["HIP"]
+
+
+P2 = np.zeros((4, 2))
+P3 = np.zeros((4, 3))
+V4 = np.zeros(4)
+
+
+@pytest.mark.parametrize("args,exc", [
+    ((3, 4, 0x11, np.zeros(12), V4, P3), ValueError),       # 1-d points
+    ((3, 4, 0x11, P3, V4, np.zeros(12)), ValueError),       # 1-d targets
+    ((3, 4, 0x11, P3, V4, P2), ValueError),                 # dim mismatch
+    ((3, 4, 0x11, P3, np.zeros(5), P3), ValueError),        # count mismatch
+    ((3, 0, 0x11, P3, V4, P3), ValueError),                 # SERIAL absent
+    ((3, 9, 0x11, P3, V4, P3), ValueError),                 # unknown space
+    ((3, 4, 0x33, P3, V4, P3), ValueError),                 # unknown basis
+    ((4, 4, 0x11, np.zeros((4, 4)), V4, np.zeros((4, 4))), RuntimeError),
+])
+def test_python_argument_validation(args, exc):
+    with pytest.raises(exc):
+        pacman.rbf.interpolate(*args)
+
+
+def test_c_abi_error_codes():
+    lib = _lib.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    p = P3.ctypes.data_as(dp)
+    v = V4.ctypes.data_as(dp)
+    out = np.zeros(4)
+    o = out.ctypes.data_as(dp)
+    # bad dimension, execution space not in this build, unknown basis
+    assert lib.pacman_rbf_interpolate_c(4, 4, 0x11, p, 4, v, p, 4, o) == -1
+    assert lib.pacman_rbf_interpolate_c(3, 1, 0x11, p, 4, v, p, 4, o) == -1
+    assert "not enabled" in _lib.last_error()
+    assert lib.pacman_rbf_interpolate_c(3, 4, 0x42, p, 4, v, p, 4, o) == -1
+    # null pointers with positive counts (src/rbf_interface.cpp:109-114)
+    assert lib.pacman_rbf_interpolate_c(3, 4, 0x11, None, 4, v, p, 4, o) == -1
+    assert "sourcePoints is null" in _lib.last_error()
+    assert lib.pacman_rbf_interpolate_c(3, 4, 0x11, p, 4, None, p, 4, o) == -1
+    assert lib.pacman_rbf_interpolate_c(3, 4, 0x11, p, 4, v, None, 4, o) == -1
+    # zero targets: nothing to compute, success
+    assert lib.pacman_rbf_interpolate_c(3, 4, 0x11, p, 4, v, p, 0, o) == 0
+
+
+def test_plan_create_validation():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.pacman_rbf_plan_create(ctypes.byref(h), 3, 0x99, 50, 0.15, 0.1,
+                                      None, 0, None, 0, 0, None) == -1
+    assert "RBF function" in _lib.last_error()
+    assert lib.pacman_rbf_plan_create(ctypes.byref(h), 0, 0x11, 50, 0.15, 0.1,
+                                      None, 0, None, 0, 0, None) == -1
+    assert not h.value

@@ -1,0 +1,324 @@
+"""Pins for the CPU FP64 oracle (runs without a GPU).
+
+Each oracle stage is checked against something it does not share code with:
+hand-computed values, textbook library routines (numpy / scipy), brute
+force in numpy, mathematical invariants of the transfer, and the
+reference's own known-answer tests (tests/test_cpp_rbf_interface.cpp and
+tests/test_rbf-pum.py of the reference).
+"""
+
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from pacman_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MESHES = os.path.join(ROOT, "tests", "golden", "meshes")
+BASES = [0x10, 0x11, 0x12, 0x13, 0x14]
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+# ---- Wendland kernels (rbf_functions.hpp:22-150) ---------------------------
+
+def test_wendland_hand_values(oracle_mod):
+    w = oracle_mod.wendland
+    # phi(0): 1, 1, 3, 1, 15 (the kernels are not normalised)
+    assert [w(f, 0.0, 1.0) for f in BASES] == [1.0, 1.0, 3.0, 1.0, 15.0]
+    # p = 1/2, computed by hand from the closed forms
+    half = [0.25, 0.1875, 0.32421875, 0.0595703125, 0.48504638671875]
+    for f, v in zip(BASES, half):
+        assert w(f, 0.5, 1.0) == pytest.approx(v, rel=1e-15)
+    # radiusInv scales the argument: phi(r; 1/delta) = phi(r / delta; 1)
+    for f in BASES:
+        assert w(f, 0.03, 1 / 0.1) == pytest.approx(w(f, 0.3, 1.0), rel=1e-14)
+    # support mask: zero for p >= 1 and for negative r
+    for f in BASES:
+        assert w(f, 1.0, 1.0) == 0.0
+        assert w(f, 1.5, 1.0) == 0.0
+        assert w(f, -0.1, 1.0) == 0.0
+
+
+@pytest.mark.parametrize("f,order", [(0x11, 2), (0x12, 4), (0x13, 6),
+                                     (0x14, 8)])
+def test_wendland_smoothness_at_support(f, order, oracle_mod):
+    """C^{2k} kernels vanish at p = 1 to order 2k + 1: phi(1 - e) = O(e^{2k+2})
+    for the C2..C8 members ((1-p)^{l} factors with l = 4, 6, 8, 10)."""
+    w = oracle_mod.wendland
+    e1, e2 = 1e-2, 5e-3
+    ratio = w(f, 1 - e1, 1.0) / w(f, 1 - e2, 1.0)
+    assert np.log2(ratio) == pytest.approx(order + 2, abs=0.1)
+
+
+@pytest.mark.parametrize("f", BASES)
+def test_wendland_positive_definite_in_3d(f, oracle_mod):
+    rng = np.random.default_rng(1)
+    P = rng.uniform(0, 1, (60, 3))
+    D = np.sqrt(((P[:, None, :] - P[None, :, :]) ** 2).sum(-1))
+    A = np.vectorize(lambda r: oracle_mod.wendland(f, r, 1 / 0.6))(D)
+    assert np.linalg.eigvalsh(A).min() > 0
+
+
+# ---- dense per-patch algebra vs textbook library routines -------------------
+
+def test_lu_solve_matches_scipy(oracle_mod):
+    rng = np.random.default_rng(2)
+    for n in (1, 5, 37, 120):
+        B = rng.normal(size=(n, n))
+        A = B @ B.T + n * np.eye(n)
+        b = rng.normal(size=n)
+        x = oracle_mod.lu_solve(A, b)
+        ref = scipy.linalg.solve(A, b)
+        assert np.linalg.norm(x - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_lstsq_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(3)
+    for n, p in ((10, 4), (150, 4), (150, 3), (7, 2), (3, 1)):
+        Q = np.hstack([np.ones((n, 1)), rng.uniform(0.4, 0.6, (n, p - 1))])
+        b = rng.normal(size=n)
+        beta = oracle_mod.lstsq(Q, b)
+        ref = np.linalg.lstsq(Q, b, rcond=None)[0]
+        assert np.allclose(beta, ref, rtol=1e-9, atol=1e-9)
+
+
+def test_singular_values_match_numpy(oracle_mod):
+    rng = np.random.default_rng(4)
+    for n, p in ((50, 4), (6, 3), (200, 2)):
+        Q = rng.normal(size=(n, p))
+        Q[:, -1] = Q[:, 0] * 1e-7 + Q[:, -1] * 1e-9  # ill-conditioned column
+        s = oracle_mod.singular_values(Q)
+        ref = np.linalg.svd(Q, compute_uv=False)
+        assert np.allclose(s, ref, rtol=1e-8, atol=1e-14 * ref[0])
+
+
+# ---- FindRadius, lattice, tag/snap/prune, membership -------------------------
+
+def _sq(a, b):
+    d = b - a
+    return (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+
+
+def pad3(P):
+    out = np.zeros((len(P), 3))
+    out[:, : P.shape[1]] = P
+    return out
+
+
+def find_radius_numpy(S, dim, npc):
+    """clusters_radius.hpp:22-125 recomputed by brute force in numpy."""
+    S3 = pad3(S)
+    lo, hi = np.zeros(3), np.zeros(3)
+    lo[:dim], hi[:dim] = S.min(0), S.max(0)
+    c = np.zeros(3)
+    c[:dim] = (lo[:dim] + hi[:dim]) / 2.0
+    probes = [c.copy()]
+    minus, plus = [], []
+    for ax in range(dim):
+        s = c.copy()
+        ln = abs(hi[ax] - lo[ax])
+        s[ax] -= ln * 0.25
+        minus.append(s.copy())
+        s[ax] += ln * 0.5
+        plus.append(s.copy())
+    probes += minus + plus
+    k = min(npc, len(S))
+    m = []
+    for q in probes:
+        nn = int(np.argmin(_sq(q, S3)))   # first index among ties
+        d2 = np.sort(_sq(S3[nn], S3))
+        m.append(d2[k - 1])
+    m.sort()
+    return np.sqrt(m[dim]), lo, hi
+
+
+@pytest.mark.parametrize("dim,n,npc", [(3, 1500, 30), (2, 800, 20),
+                                       (1, 300, 10), (3, 8000, 100)])
+def test_find_radius_and_lattice_bruteforce(dim, n, npc, oracle_mod):
+    rng = np.random.default_rng(dim * 100 + n)
+    S = rng.uniform(0, 1, (n, dim))
+    T = rng.uniform(0, 1, (n // 2, dim))
+    o = oracle_mod.Setup(dim, 0x11, S, T, npc, 0.15, -1.0)
+    r, lo, hi = find_radius_numpy(S, dim, npc)
+    assert o.radius == r
+    # lattice size: prod(ceil(max(1, edge / spacing)) + 1)
+    spacing = np.sqrt(4.0 / dim) * r * (1 - 0.15)
+    nb = 1
+    for a in range(dim):
+        nb *= int(np.ceil(max(1.0, abs(hi[a] - lo[a]) / spacing))) + 1
+    assert o.num_candidates == nb
+
+
+def test_membership_bruteforce(oracle_mod):
+    rng = np.random.default_rng(7)
+    S = rng.uniform(0, 1, (2000, 3))
+    T = rng.uniform(0, 1, (1500, 3))
+    o = oracle_mod.Setup(3, 0x11, S, T, 40, 0.15, -1.0)
+    r = o.radius
+    C, cand = o.centers()
+    assert np.all(np.diff(cand) > 0)
+    # every centre is a source point
+    key = {tuple(p) for p in S}
+    assert all(tuple(c) in key for c in C)
+    for which, P, rad in ((0, S, r), (1, T, r - 1e-14)):
+        off, idx = o.csr(which)
+        for k in range(len(C)):
+            d = np.sqrt(_sq(C[k], P))
+            np.testing.assert_array_equal(idx[off[k]:off[k + 1]],
+                                          np.nonzero(d <= rad)[0])
+    off, idx = o.csr(2)
+    for t in range(len(T)):
+        d = np.sqrt(_sq(T[t], C))
+        np.testing.assert_array_equal(idx[off[t]:off[t + 1]],
+                                      np.nonzero(d <= r)[0])
+    # Shepard weights sum to one on every covered target
+    w = o.weights()
+    sums = np.add.reduceat(w, off[:-1][np.diff(off) > 0])
+    assert np.allclose(sums, 1.0, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_grid_queries_equal_bruteforce(seed, oracle_mod):
+    rng = np.random.default_rng(seed)
+    S = rng.uniform(0, 1, (3000, 3)) ** (1 + seed)   # uneven densities
+    T = rng.uniform(0, 1, (2000, 3))
+    a = oracle_mod.Setup(3, 0x11, S, T, 35, 0.15, -1.0)
+    b = oracle_mod.Setup(3, 0x11, S, T, 35, 0.15, -1.0, brute=True)
+    np.testing.assert_array_equal(a.centers()[0], b.centers()[0])
+    for which in (0, 1, 2):
+        for x, y in zip(a.csr(which), b.csr(which)):
+            np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("cloud", ["random", "grid", "clustered", "plane"])
+def test_parallel_prune_equals_sequential_loop(cloud, oracle_mod):
+    """The reference prunes centres in a sequential host loop
+    (clustering.hpp:230-254); the parallel rule must keep the same set."""
+    rng = np.random.default_rng(12)
+    if cloud == "random":
+        S = rng.uniform(0, 1, (5000, 3))
+    elif cloud == "grid":
+        S = W.regular_grid(17)
+    elif cloud == "clustered":
+        S = np.vstack([rng.normal(c, 0.05, (1500, 3))
+                       for c in rng.uniform(0, 1, (4, 3))])
+    else:
+        S = rng.uniform(0, 1, (4000, 3))
+        S[:, 2] = 0.5 + 1e-9 * rng.normal(size=len(S))
+    T = rng.uniform(0, 1, (3000, 3))
+    for npc in (15, 60):
+        a = oracle_mod.Setup(3, 0x11, S, T, npc, 0.15, -1.0)
+        b = oracle_mod.Setup(3, 0x11, S, T, npc, 0.15, -1.0,
+                             sequential_prune=True)
+        np.testing.assert_array_equal(a.centers()[1], b.centers()[1])
+
+
+# ---- whole operator ----------------------------------------------------------
+
+def operator_numpy(S, T, vals, o, delta, basis):
+    """u(y) = sum_k w_k(y) [E_k A_k^{-1} (b - Q_k beta) + V_k beta] with
+    scipy/numpy dense algebra on the oracle's own patch lists."""
+    C, _ = o.centers()
+    soff, sidx = o.csr(0)
+    toff, tidx = o.csr(1)
+    woff, widx = o.csr(2)
+    w = o.weights()
+    phi = np.vectorize(lambda r: __import__("oracle").wendland(basis, r,
+                                                               1 / delta))
+    out = np.zeros(len(T))
+    for k in range(len(C)):
+        src = sidx[soff[k]:soff[k + 1]]
+        tgt = tidx[toff[k]:toff[k + 1]]
+        X, Y = S[src], T[tgt]
+        # conditioning loop of serial_ops.hpp:108-139 with numpy's SVD
+        active = [0, 1, 2]
+        while len(active) > 0:
+            Q = np.hstack([np.ones((len(X), 1)), X[:, active]])
+            s = np.linalg.svd(Q, compute_uv=False)
+            s = np.concatenate([s, np.zeros(Q.shape[1] - len(s))])
+            if s[0] / max(s[-1], 1e-14) <= 1e5:
+                break
+            rng_ = np.ptp(X[:, active], axis=0)
+            active.pop(int(np.argmin(rng_)))
+        Q = np.hstack([np.ones((len(X), 1)), X[:, active]])
+        V = np.hstack([np.ones((len(Y), 1)), Y[:, active]])
+        b = vals[src]
+        beta = np.linalg.lstsq(Q, b, rcond=None)[0]
+        A = phi(np.sqrt(((X[:, None] - X[None]) ** 2).sum(-1)))
+        alpha = scipy.linalg.solve(A, b - Q @ beta)
+        E = phi(np.sqrt(((Y[:, None] - X[None]) ** 2).sum(-1)))
+        t = E @ alpha + V @ beta
+        for j, y in enumerate(tgt):
+            lst = widx[woff[y]:woff[y + 1]]
+            out[y] += w[woff[y] + int(np.nonzero(lst == k)[0][0])] * t[j]
+    return out
+
+
+def test_operator_matches_dense_numpy(oracle_mod):
+    rng = np.random.default_rng(13)
+    S = rng.uniform(0, 1, (1200, 3))
+    T = rng.uniform(0, 1, (800, 3))
+    v = W.franke_3d(*S.T)
+    o = oracle_mod.Setup(3, 0x12, S, T, 30, 0.15, -1.0)
+    ours = o.apply(v)
+    ref = operator_numpy(S, T, v, o, o.radius, 0x12)
+    assert rel_l2(ours, ref) < 1e-11
+
+
+def test_linear_fields_reproduced(oracle_mod):
+    rng = np.random.default_rng(14)
+    S = rng.uniform(0, 1, (3000, 3))
+    T = rng.uniform(0.1, 0.9, (1000, 3))
+    f = lambda P: 0.3 - P[:, 0] + 2 * P[:, 1] + 4 * P[:, 2]
+    out = oracle_mod.rbf_interpolate(3, 0x11, S, f(S), T, 50, 0.15, -1.0)
+    assert np.max(np.abs(out - f(T))) < 1e-12
+
+
+@pytest.mark.parametrize("rbf", BASES)
+def test_reference_same_grid(rbf, oracle_mod):
+    """tests/test_cpp_rbf_interface.cpp:165-198 of the reference: 20^3 grid,
+    source = target, rel-L2 < 1e-8 for each basis."""
+    P = W.regular_grid(20)
+    v = W.franke_3d(*P.T)
+    assert rel_l2(oracle_mod.rbf_interpolate(3, rbf, P, v, P), v) < 1e-8
+
+
+CASES = {"coarse_to_fine_0": ("0.03.npz", "0.003.npz"),
+         "fine_to_coarse_0": ("0.003.npz", "0.03.npz"),
+         "same_0": ("0.03.npz", "0.03.npz")}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("rbf", BASES)
+def test_reference_npz_fixtures(case, rbf, oracle_mod):
+    """tests/test_rbf-pum.py:56-74,113-122 of the reference (< 1.5e-2)."""
+    a, b = CASES[case]
+    sp = np.load(os.path.join(MESHES, a))["points"][:, :3].astype(np.float64)
+    tp = np.load(os.path.join(MESHES, b))["points"][:, :3].astype(np.float64)
+    out = oracle_mod.rbf_interpolate(3, rbf, sp, W.franke_3d(*sp.T), tp)
+    assert rel_l2(out, W.franke_3d(*tp.T)) < 1.5e-2
+
+
+def test_fine_to_coarse_exercises_axis_drop(oracle_mod):
+    sp = np.load(os.path.join(MESHES, "0.003.npz"))["points"][:, :3]
+    tp = np.load(os.path.join(MESHES, "0.03.npz"))["points"][:, :3]
+    o = oracle_mod.Setup(3, 0x11, sp.astype(float), tp.astype(float))
+    _, pv = o.apply(W.franke_3d(*sp.T.astype(float)), return_pv=True)
+    assert (pv < 4).any()
+
+
+def test_coplanar_patch_drops_normal_axis(oracle_mod):
+    """A flat point set must lose exactly its normal axis (z)."""
+    rng = np.random.default_rng(15)
+    S = rng.uniform(0, 1, (2000, 3))
+    S[:, 2] = 0.25
+    T = S[:300].copy()
+    o = oracle_mod.Setup(3, 0x11, S, T, 40, 0.15, -1.0)
+    out, pv = o.apply(S[:, 0] + S[:, 1], return_pv=True)
+    assert np.all(pv == 3)
+    assert np.allclose(out, T[:, 0] + T[:, 1], atol=1e-10)
