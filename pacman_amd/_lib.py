@@ -56,6 +56,14 @@ def load() -> ctypes.CDLL:
     """Loads the HIP library (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # torch bundles its own libamdhip64.so.7; whichever copy is loaded
+        # first serves the whole process. Load torch's first so torch and
+        # this library share one HIP runtime (torch fails to initialise on
+        # the other copy).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with "
