@@ -138,6 +138,8 @@ struct SolveArgs {
   const int *pairSlot;
   const double *tx, *ty, *tz;  // target coordinates, cell order
   double *contrib;          // P x F
+  const double *cx, *cy, *cz;  // patch centres (polynomial basis origin)
+  double rinv;              // 1 / patch radius (polynomial basis scale)
   double dinv;              // 1 / support radius
   int dim;
   int *pvOut;               // K (optional) active polynomial columns

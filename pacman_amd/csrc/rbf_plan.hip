@@ -278,6 +278,10 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
     a.ty = gt_.view.y;
     a.tz = gt_.view.z;
     a.contrib = contrib_.get();
+    a.cx = cx_.get();
+    a.cy = cy_.get();
+    a.cz = cz_.get();
+    a.rinv = 1.0 / r_;
     a.dinv = 1.0 / delta_;
     a.dim = prm_.dim;
     a.pvOut = pv_.get();

@@ -2,22 +2,33 @@
 //
 // One 256-thread workgroup owns one patch (n sources, m targets) and runs
 // the whole per-patch chain of src/rbf_pum/solver/solver.hpp:180-286 without
-// touching HBM for any matrix:
+// writing any matrix to HBM:
 //   1. gather the patch's source coordinates and values into LDS;
-//   2. wave 0: polynomial least squares on Q = [1, x_a] by Householder QR,
-//      with the conditioning loop of SerialFillQ (serial_ops.hpp:108-139):
-//      singular values of R (= those of Q); cond > 1e5 drops the active axis
-//      with the smallest coordinate range, then Q is refitted on exactly the
-//      active columns; residual b - Q beta (solver.hpp:255-263);
+//   2. wave 0: least-squares fit of the linear polynomial
+//      (SerialSolveQR, serial_solve.hpp:46-67) with the conditioning loop of
+//      SerialFillQ (serial_ops.hpp:108-139). The Gram matrix of the basis
+//      [1, (x_a - c_a) / r] (c = patch centre, r = patch radius) is formed in
+//      one batched wave reduction; it spans the same space as the
+//      reference's [1, x_a] and is well conditioned. The condition number of
+//      the reference's raw Q = [1, x_a] is that of R'' T (R'' = Cholesky
+//      factor of the scaled Gram matrix, T the basis change): a Frobenius
+//      bound settles cond <= 1e5 at once, the exact Jacobi singular values
+//      are computed only otherwise. While cond > 1e5 the active axis with
+//      the smallest coordinate range is dropped (serial_ops.hpp:28-57); the
+//      fit then uses exactly the active columns. Residual b - Q beta
+//      (solver.hpp:255-263);
 //   3. all threads: assemble the Wendland matrix A straight into registers.
 //      Thread (tr, tc) of the 16 x 16 thread grid owns rows tr + 16a and
 //      columns tc + 16b for b <= a (every lower-triangle entry, cyclically);
 //   4. factor A with the symmetric form of the reference's unpivoted
 //      Doolittle LU (serial_solve.hpp:76-84): A = L D L^T has exactly the
 //      pivots d_j of that LU, half its flops and no upper triangle. The
-//      right-hand sides are eliminated in the same sweep (one barrier per
-//      column, the column broadcast double-buffered in LDS);
-//   5. blocked back substitution L^T alpha = D^{-1} y, 16 rows at a time;
+//      column loop is split into a compile-time loop over 16-column blocks
+//      and a run-time loop inside the block, so every thread's active
+//      register range is static; one barrier per column, the column and its
+//      pivot reciprocal double-buffered in LDS. The right-hand sides are
+//      eliminated in the same sweep;
+//   5. back substitution L^T alpha = D^{-1} y, 16 rows at a time;
 //   6. t(y) = E alpha + V beta at every member target (solver.hpp:265-273),
 //      written to that (target, patch) slot; the blend kernel sums the slots
 //      of each target in ascending patch order with the Shepard-normalised
@@ -38,14 +49,29 @@ constexpr int kThreads = 256;
 constexpr int kG = 16;      // thread grid is kG x kG
 constexpr int kFMax = 8;    // fields per launch
 
-__device__ inline double dist3(double ax, double ay, double az, double bx,
-                               double by, double bz) {
-  return sqrt(sqdiff3(ax, ay, az, bx, by, bz));
+// sqrt for the dense arithmetic: hardware reciprocal square root refined by
+// two Newton-Raphson (Goldschmidt) steps; within an ulp for the distances
+// used here (0 <= x < 1e300). Membership decisions never use it.
+__device__ inline double fast_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y;
+  double h = 0.5 * y;
+  const double r = fma(-s, h, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  const double d = fma(-s, s, x);
+  s = fma(d, h, s);
+  return x > 0.0 ? s : 0.0;
+}
+
+__device__ inline double dist_fast(double ax, double ay, double az, double bx,
+                                   double by, double bz) {
+  const double dx = bx - ax, dy = by - ay, dz = bz - az;
+  return fast_sqrt(dx * dx + dy * dy + dz * dz);
 }
 
 // Singular values (descending) of a p x p upper-triangular R by one-sided
-// Jacobi; p <= 4. Every lane runs it on identical data. All register arrays
-// are indexed by compile-time constants.
+// Jacobi; p <= 4. All register arrays are indexed by compile-time constants.
 __device__ inline void small_svd(double (&U)[4][4], int p, double (&S)[4]) {
   for (int sweep = 0; sweep < 40; ++sweep) {
     double off = 0.0;
@@ -101,88 +127,25 @@ __device__ inline void small_svd(double (&U)[4][4], int p, double (&S)[4]) {
       }
 }
 
-// Householder QR of the n x pv matrix held as q[c][s] (rows lane + 64 s),
-// applied to the first F right-hand sides x[f][s]. Same textbook reflector
-// as the oracle: v = col - alpha e_j with alpha = -sign(x0) |col|.
-template <int NR, int NF>
-__device__ inline void wave_householder(double (&q)[4][NR], int pv, int lane,
-                                        double (&x)[NF][NR], int F) {
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (j >= pv)
-      continue;
-    double loc = 0.0;
-#pragma unroll
-    for (int s = 0; s < NR; ++s) {
-      const int t = lane + 64 * s;
-      if (t >= j)
-        loc += q[j][s] * q[j][s];
-    }
-    const double nrm = sqrt(wave_sum(loc));
-    if (nrm == 0.0)
-      continue;
-    const double x0 = __shfl(q[j][0], j, kWave);
-    const double alpha = (x0 >= 0 ? -nrm : nrm);
-    double v[NR];
-    double vl = 0.0;
-#pragma unroll
-    for (int s = 0; s < NR; ++s) {
-      const int t = lane + 64 * s;
-      v[s] = (t > j) ? q[j][s] : (t == j ? x0 - alpha : 0.0);
-      vl += v[s] * v[s];
-    }
-    const double vv = wave_sum(vl);
-    if (vv == 0.0)
-      continue;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (c >= j && c < pv) {
-        double sl = 0.0;
-#pragma unroll
-        for (int s = 0; s < NR; ++s)
-          sl += v[s] * q[c][s];
-        const double f = 2.0 * wave_sum(sl) / vv;
-#pragma unroll
-        for (int s = 0; s < NR; ++s)
-          q[c][s] -= f * v[s];
-      }
-    }
-#pragma unroll
-    for (int ff = 0; ff < NF; ++ff) {
-      if (ff < F) {
-        double sl = 0.0;
-#pragma unroll
-        for (int s = 0; s < NR; ++s)
-          sl += v[s] * x[ff][s];
-        const double f = 2.0 * wave_sum(sl) / vv;
-#pragma unroll
-        for (int s = 0; s < NR; ++s)
-          x[ff][s] -= f * v[s];
-      }
-    }
-  }
-}
+// Column c (0..3) of the active basis holds axis ca[c] (ca[0] = -1: the
+// constant). Entries of a 4 x 4 matrix over the full basis [1, u0, u1, u2]
+// are gathered with selects so every register index stays static.
+__device__ inline int basis_index(int axis) { return axis + 1; }
 
-// Column c (1..3) of Q holds axis ca[c]; selects keep the indexing static.
-template <int NR>
-__device__ inline void fill_q(double (&q)[4][NR], const double (&X)[3][NR],
-                              const bool (&valid)[NR], const int (&ca)[4]) {
+__device__ inline double pick4(const double (&G)[4][4], int r, int c) {
+  double v = 0.0;
 #pragma unroll
-  for (int s = 0; s < NR; ++s) {
-    q[0][s] = valid[s] ? 1.0 : 0.0;
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int c = 1; c < 4; ++c)
-      q[c][s] = ca[c] == 0 ? X[0][s]
-                : ca[c] == 1 ? X[1][s]
-                : ca[c] == 2 ? X[2][s]
-                             : 0.0;
-  }
+    for (int j = 0; j < 4; ++j)
+      v = (i == r && j == c) ? G[i][j] : v;
+  return v;
 }
 
 __device__ inline int assign_columns(const bool (&active)[3], int (&ca)[4]) {
   int cols = 1;
   ca[0] = -1;
-  ca[1] = ca[2] = ca[3] = -1;
+  ca[1] = ca[2] = ca[3] = -2;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax)
     if (active[ax]) {
@@ -197,6 +160,38 @@ __device__ inline int assign_columns(const bool (&active)[3], int (&ca)[4]) {
   return cols;
 }
 
+// In-place Cholesky of the leading p x p block (lower, L L^T). Returns false
+// when a pivot is not positive (rank-deficient basis).
+__device__ inline bool chol4(double (&L)[4][4], int p) {
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j < p) {
+      double d = L[j][j];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < j)
+          d -= L[j][k] * L[j][k];
+      ok = ok && (d > 0.0);
+      const double ljj = sqrt(fmax(d, 0.0));
+      L[j][j] = ljj;
+      const double inv = ljj > 0.0 ? 1.0 / ljj : 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i > j && i < p) {
+          double s = L[i][j];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (k < j)
+              s -= L[i][k] * L[j][k];
+          L[i][j] = s * inv;
+        }
+      }
+    }
+  }
+  return ok;
+}
+
 template <int R, int KIND>
 __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
   constexpr int NMAX = kG * R;
@@ -206,10 +201,12 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
   __shared__ double px[NMAX], py[NMAX], pz[NMAX];
   __shared__ double rhs[kFMax][NMAX];
   __shared__ double colbuf[2][NMAX];
+  __shared__ double dinvs[NMAX];
+  __shared__ double cbinv[2];
   __shared__ double dblk[kG][kG + 1];
   __shared__ double spart[kFMax][kG];
   __shared__ double scoef[kFMax][4];
-  __shared__ int spv;
+  __shared__ int spv, spivot;
 
   const int patch = a.patches[xcd_remap(blockIdx.x, a.count)];
   const int64_t s0 = a.srcOff[patch];
@@ -236,61 +233,145 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
       for (int f = 0; f < F; ++f)
         rhs[f][t] = 0.0;
     }
+    dinvs[t] = 1.0;
   }
+  if (tid == 0)
+    spivot = 0;
   __syncthreads();
 
   // 2. polynomial least squares (wave 0)
   if (tid < kWave) {
     const int lane = tid;
-    double X[3][NR];
+    const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
+    const double sc = a.rinv;
+    double U3[3][NR];
     bool valid[NR];
 #pragma unroll
     for (int s = 0; s < NR; ++s) {
       const int t = lane + 64 * s;
       valid[s] = t < n;
-      X[0][s] = valid[s] ? px[t] : 0.0;
-      X[1][s] = valid[s] ? py[t] : 0.0;
-      X[2][s] = valid[s] ? pz[t] : 0.0;
+      U3[0][s] = valid[s] ? (px[t] - c[0]) * sc : 0.0;
+      U3[1][s] = valid[s] ? (py[t] - c[1]) * sc : 0.0;
+      U3[2][s] = valid[s] ? (pz[t] - c[2]) * sc : 0.0;
     }
-    // coordinate range per axis (SerialDeactivateOneAxis, serial_ops.hpp:28-57)
-    double range[3];
+    // one batched reduction: coordinate ranges and the Gram matrix
+    double mn[3], mx[3], G[4][4];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-      double mn = DBL_MAX, mx = -DBL_MAX;
+      mn[ax] = DBL_MAX;
+      mx[ax] = -DBL_MAX;
 #pragma unroll
       for (int s = 0; s < NR; ++s)
         if (valid[s]) {
-          mn = fmin(mn, X[ax][s]);
-          mx = fmax(mx, X[ax][s]);
+          const int t = lane + 64 * s;
+          const double x = ax == 0 ? px[t] : ax == 1 ? py[t] : pz[t];
+          mn[ax] = fmin(mn[ax], x);
+          mx[ax] = fmax(mx[ax], x);
         }
-      range[ax] = fabs(wave_max(mx) - wave_min(mn));
     }
-    bool active[3] = {dim > 0, dim > 1, dim > 2};
-    int ca[4];
-    double q[4][NR];
-    double xr[kFMax][NR];
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int s = 0; s < NR; ++s)
-        xr[f][s] = 0.0;
-    int pv = assign_columns(active, ca);
-    while (pv > 1 && n > 0) {
-      fill_q<NR>(q, X, valid, ca);
-      wave_householder<NR, kFMax>(q, pv, lane, xr, 0);
-      double U[4][4];
+      for (int j = 0; j < 4; ++j)
+        G[i][j] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NR; ++s) {
+      const double u[4] = {valid[s] ? 1.0 : 0.0, U3[0][s], U3[1][s], U3[2][s]};
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const double v = __shfl(q[c][0], i, kWave);
-          U[i][c] = (i < pv && c < pv && i <= c && i < n) ? v : 0.0;
+        for (int j = 0; j <= i; ++j)
+          G[i][j] += u[i] * u[j];
+    }
+    double range[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+      range[ax] = fabs(wave_max(mx[ax]) - wave_min(mn[ax]));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        G[i][j] = wave_sum(G[i][j]);
+        G[j][i] = G[i][j];
+      }
+
+    bool active[3] = {dim > 0, dim > 1, dim > 2};
+    int ca[4];
+    int pv = assign_columns(active, ca);
+    double L[4][4];
+    bool ok = true;
+    for (;;) {
+      // Gram of the active columns, its Cholesky factor, cond(raw Q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          L[i][j] = (i < pv && j < pv)
+                        ? pick4(G, ca[i] + 1, ca[j] + 1)
+                        : (i == j ? 1.0 : 0.0);
+      ok = chol4(L, pv);
+      if (pv <= 1)
+        break;
+      bool keep = false;
+      if (ok) {
+        // raw basis [1, x_a] = [1, u_a] T with T = [[1, c_a], [0, 1/sc]]
+        double Rr[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            double v = 0.0;
+            if (i < pv && j < pv && i <= j) {
+              // (R'' T)_ij = sum_m R''_im T_mj with R'' = L^T, T_00 = 1,
+              // T_0j = c_j, T_jj = 1 / sc (j >= 1), zero elsewhere.
+              double cj = 0.0;
+#pragma unroll
+              for (int ax = 0; ax < 3; ++ax)
+                cj = (ca[j] == ax) ? c[ax] : cj;
+              if (j == 0)
+                v = L[0][0];
+              else if (i == 0)
+                v = L[0][0] * cj + L[j][0] / sc;
+              else
+                v = L[j][i] / sc;
+            }
+            Rr[i][j] = v;
+          }
+        // ||R||_F * ||R^-1||_F >= cond_2(R): settles cond <= 1e5 cheaply
+        double Ri[4][4];
+        double fr = 0.0, fi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int i = 3; i >= 0; --i) {
+            double v = 0.0;
+            if (i <= j && j < pv) {
+              double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+              for (int k = 0; k < 4; ++k)
+                if (k > i && k <= j)
+                  s -= Rr[i][k] * Ri[k][j];
+              v = s / Rr[i][i];
+            }
+            Ri[i][j] = v;
+          }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            fr += Rr[i][j] * Rr[i][j];
+            fi += Ri[i][j] * Ri[i][j];
+          }
+        if (sqrt(fr) * sqrt(fi) <= kCondMax) {
+          keep = true;
+        } else {
+          double S[4];
+          small_svd(Rr, pv, S);
+          const double smin = pv == 4 ? S[3] : pv == 3 ? S[2] : S[1];
+          keep = !(S[0] / fmax(smin, kEps) > kCondMax);
         }
-      double S[4];
-      small_svd(U, pv, S);
-      const double smin = pv == 4 ? S[3] : pv == 3 ? S[2] : S[1];
-      const double cond = S[0] / fmax(smin, kEps);
-      if (!(cond > kCondMax))
+      }
+      if (keep)
         break;
       double best = DBL_MAX;
       int bi = 0;
@@ -308,35 +389,58 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
           active[ax] = false;
       pv = assign_columns(active, ca);
     }
-    // final fit on the active columns
-    fill_q<NR>(q, X, valid, ca);
-#pragma unroll
-    for (int f = 0; f < kFMax; ++f)
-      if (f < F)
-#pragma unroll
-        for (int s = 0; s < NR; ++s) {
-          const int t = lane + 64 * s;
-          xr[f][s] = valid[s] ? rhs[f][t] : 0.0;
-        }
-    wave_householder<NR, kFMax>(q, pv, lane, xr, F);
-    const int pvr = min(pv, n);
+    // beta'' = G^-1 h on the active columns (normal equations of the
+    // well-conditioned scaled basis), then the residual b - Q beta.
 #pragma unroll
     for (int f = 0; f < kFMax; ++f) {
       if (f >= F)
         continue;
-      double beta[4] = {0.0, 0.0, 0.0, 0.0};
+      // Q''^T b over the full scaled basis
+      double hf[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int j = 3; j >= 0; --j) {
-        if (j < pvr) {
-          double s = __shfl(xr[f][0], j, kWave);
+      for (int s = 0; s < NR; ++s) {
+        const int t = lane + 64 * s;
+        const double b = valid[s] ? rhs[f][t] : 0.0;
+        hf[0] += b;
+        hf[1] += U3[0][s] * b;
+        hf[2] += U3[1][s] * b;
+        hf[3] += U3[2][s] * b;
+      }
 #pragma unroll
-          for (int c = j + 1; c < 4; ++c)
-            if (c < pv)
-              s -= __shfl(q[c][0], j, kWave) * beta[c];
-          beta[j] = s / __shfl(q[j][0], j, kWave);
+      for (int k = 0; k < 4; ++k)
+        hf[k] = wave_sum(hf[k]);
+      double y[4], beta[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double hv = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          hv = (ca[i] + 1 == k) ? hf[k] : hv;
+        y[i] = hv;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (i < pv) {
+          double s = y[i];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (k < i)
+              s -= L[i][k] * y[k];
+          y[i] = L[i][i] > 0.0 ? s / L[i][i] : 0.0;
+        }
+#pragma unroll
+      for (int i = 3; i >= 0; --i) {
+        beta[i] = 0.0;
+        if (i < pv) {
+          double s = y[i];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (k > i && k < pv)
+              s -= L[k][i] * beta[k];
+          beta[i] = L[i][i] > 0.0 ? s / L[i][i] : 0.0;
         }
       }
-      // per-axis coefficients: inactive axes contribute exact zeros
+      // per-axis coefficients of the scaled basis (inactive axes: zero)
       double cax[3];
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax)
@@ -344,21 +448,18 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
                   : ca[2] == ax ? beta[2]
                   : ca[3] == ax ? beta[3]
                                 : 0.0;
-      // residual b - Q beta (solver.hpp:258-263)
 #pragma unroll
       for (int s = 0; s < NR; ++s) {
         const int t = lane + 64 * s;
-        if (valid[s]) {
-          const double qb =
-              beta[0] + cax[0] * X[0][s] + cax[1] * X[1][s] + cax[2] * X[2][s];
-          rhs[f][t] -= qb;
-        }
+        if (valid[s])
+          rhs[f][t] -= beta[0] + cax[0] * U3[0][s] + cax[1] * U3[1][s] +
+                       cax[2] * U3[2][s];
       }
       if (lane == 0) {
         scoef[f][0] = beta[0];
-        scoef[f][1] = cax[0];
-        scoef[f][2] = cax[1];
-        scoef[f][3] = cax[2];
+        scoef[f][1] = cax[0] * sc;
+        scoef[f][2] = cax[1] * sc;
+        scoef[f][3] = cax[2] * sc;
       }
     }
     if (lane == 0)
@@ -367,66 +468,71 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
 
   // 3. assemble A (lower block triangle, registers)
   double A[NA];
-  {
 #pragma unroll
-    for (int ai = 0; ai < R; ++ai) {
-      const int i = tr + kG * ai;
-      const double xi = px[i], yi = py[i], zi = pz[i];
+  for (int ai = 0; ai < R; ++ai) {
+    const int i = tr + kG * ai;
+    const double xi = px[i], yi = py[i], zi = pz[i];
 #pragma unroll
-      for (int bi = 0; bi <= ai; ++bi) {
-        const int l = tc + kG * bi;
-        double v;
-        if (i < n && l < n)
-          v = wendland<KIND, double>(dist3(xi, yi, zi, px[l], py[l], pz[l]),
-                                     a.dinv);
-        else
-          v = (i == l) ? 1.0 : 0.0;
-        A[ai * (ai + 1) / 2 + bi] = v;
-      }
+    for (int bi = 0; bi <= ai; ++bi) {
+      const int l = tc + kG * bi;
+      double v;
+      if (i < n && l < n)
+        v = wendland<KIND, double>(dist_fast(xi, yi, zi, px[l], py[l], pz[l]),
+                                   a.dinv);
+      else
+        v = (i == l) ? 1.0 : 0.0;
+      A[ai * (ai + 1) / 2 + bi] = v;
     }
   }
   __syncthreads();
 
   // 4. A = L D L^T with forward elimination of the right-hand sides
-  bool zeroPivot = false;
-  for (int j = 0; j < n; ++j) {
-    const int jb = j >> 4, jt = j & (kG - 1);
-    double *cb = colbuf[j & 1];
-    if (tc == jt) {
 #pragma unroll
-      for (int ai = 0; ai < R; ++ai)
+  for (int jb = 0; jb < R; ++jb) {
+    if (kG * jb >= n)
+      break;
+    for (int jt = 0; jt < kG; ++jt) {
+      const int j = kG * jb + jt;
+      if (j >= n)
+        break;
+      double *cb = colbuf[j & 1];
+      if (tc == jt) {
 #pragma unroll
-        for (int bi = 0; bi <= ai; ++bi)
-          if (bi == jb) {
-            const int i = tr + kG * ai;
-            if (i >= j)
-              cb[i] = A[ai * (ai + 1) / 2 + bi];
+        for (int ai = jb; ai < R; ++ai) {
+          const int i = tr + kG * ai;
+          if (ai > jb || i >= j) {
+            const double v = A[ai * (ai + 1) / 2 + jb];
+            cb[i] = v;
+            if (ai == jb && i == j) {
+              const double iv = 1.0 / v;
+              cbinv[j & 1] = iv;
+              dinvs[j] = iv;
+              if (v == 0.0)
+                spivot = 1;
+            }
           }
-    }
-    __syncthreads();
-    const double d = cb[j];
-    zeroPivot |= (d == 0.0);
-    const double inv = 1.0 / d;
-    if (tid > j && tid < n) {
-      const double ct = cb[tid];
-      for (int f = 0; f < F; ++f)
-        rhs[f][tid] -= ct * (rhs[f][j] * inv);
-    }
-    double wl[R];
+        }
+      }
+      __syncthreads();
+      const double inv = cbinv[j & 1];
+      if (tid > j && tid < n) {
+        const double ct = cb[tid];
+        for (int f = 0; f < F; ++f)
+          rhs[f][tid] -= ct * (rhs[f][j] * inv);
+      }
+      double wl[R];
 #pragma unroll
-    for (int bi = 0; bi < R; ++bi) {
-      const int l = tc + kG * bi;
-      wl[bi] = (kG * bi + kG - 1 > j && l > j) ? cb[l] * inv : 0.0;
-    }
+      for (int bi = jb; bi < R; ++bi) {
+        const int l = tc + kG * bi;
+        wl[bi] = (bi > jb || l > j) ? cb[l] * inv : 0.0;
+      }
 #pragma unroll
-    for (int ai = 0; ai < R; ++ai) {
-      if (kG * ai + kG - 1 > j) {
+      for (int ai = jb; ai < R; ++ai) {
         const int i = tr + kG * ai;
-        const double ci = (i > j) ? cb[i] : 0.0;
+        const double ci = (ai > jb || i > j) ? cb[i] : 0.0;
 #pragma unroll
-        for (int bi = 0; bi <= ai; ++bi)
-          if (kG * bi + kG - 1 > j)
-            A[ai * (ai + 1) / 2 + bi] -= ci * wl[bi];
+        for (int bi = jb; bi <= ai; ++bi)
+          A[ai * (ai + 1) / 2 + bi] = fma(-ci, wl[bi], A[ai * (ai + 1) / 2 + bi]);
       }
     }
   }
@@ -435,6 +541,8 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
   // 5. back substitution, 16 rows per block, bottom-up
 #pragma unroll
   for (int B = R - 1; B >= 0; --B) {
+    if (kG * B >= n)
+      continue;
     double pf[kFMax];
 #pragma unroll
     for (int f = 0; f < kFMax; ++f)
@@ -446,7 +554,7 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
         if (f < F)
-          pf[f] += c * rhs[f][i];
+          pf[f] = fma(c, rhs[f][i], pf[f]);
     }
 #pragma unroll
     for (int f = 0; f < kFMax; ++f) {
@@ -464,16 +572,17 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
     __syncthreads();
     if (tid < F) {
       const int f = tid;
-      double al[kG];
+      double sv[kG];
+#pragma unroll
+      for (int jj = 0; jj < kG; ++jj)
+        sv[jj] = rhs[f][kG * B + jj] - spart[f][jj];
 #pragma unroll
       for (int jj = kG - 1; jj >= 0; --jj) {
-        const int j = kG * B + jj;
-        double s = rhs[f][j] - spart[f][jj];
+        const double al = sv[jj] * dinvs[kG * B + jj];
+        rhs[f][kG * B + jj] = al;
 #pragma unroll
-        for (int ii = jj + 1; ii < kG; ++ii)
-          s -= dblk[ii][jj] * al[ii];
-        al[jj] = s / dblk[jj][jj];
-        rhs[f][j] = al[jj];
+        for (int k = 0; k < jj; ++k)
+          sv[k] = fma(-dblk[jj][k], al, sv[k]);
       }
     }
     __syncthreads();
@@ -483,10 +592,8 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
     if (a.pvOut)
       a.pvOut[patch] = spv;
     if (a.status)
-      a.status[patch] = 0;
+      a.status[patch] = spivot;
   }
-  if (zeroPivot && a.status)
-    a.status[patch] = 1;
 
   // 6. evaluate at the member targets and write each (target, patch) slot
   const int64_t p0 = a.pairOff[patch], p1 = a.pairOff[patch + 1];
@@ -501,21 +608,21 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
     for (int f = 0; f < kFMax; ++f)
       acc[f] = 0.0;
     for (int l = 0; l < n; ++l) {
-      const double phi =
-          wendland<KIND, double>(dist3(yx, yy, yz, px[l], py[l], pz[l]),
-                                 a.dinv);
+      const double phi = wendland<KIND, double>(
+          dist_fast(yx, yy, yz, px[l], py[l], pz[l]), a.dinv);
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
         if (f < F)
-          acc[f] += phi * rhs[f][l];
+          acc[f] = fma(phi, rhs[f][l], acc[f]);
     }
     const int slot = a.pairSlot[p];
+    const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
 #pragma unroll
     for (int f = 0; f < kFMax; ++f)
       if (f < F) {
-        // V beta (solver.hpp:272): inactive axes carry zero coefficients
-        const double vb = scoef[f][0] + scoef[f][1] * yx + scoef[f][2] * yy +
-                          scoef[f][3] * yz;
+        // V beta (solver.hpp:272) in the scaled basis around the centre
+        const double vb = scoef[f][0] + scoef[f][1] * (yx - cx) +
+                          scoef[f][2] * (yy - cy) + scoef[f][3] * (yz - cz);
         a.contrib[(int64_t)slot * F + f] = acc[f] + vb;
       }
   }
@@ -532,7 +639,7 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
   for (int64_t s = s0; s < s1; ++s) {
     const int k = a.slotPatch[s];
     // DistanceNoCheck(target, center), solver.hpp:158-161
-    const double d = dist3(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
+    const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
     sum += wendland<1, double>(d, a.rinv);
   }
   const int64_t o = a.tperm[t];
@@ -545,7 +652,7 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
       if (!a.slotMember[s])
         continue;
       const int k = a.slotPatch[s];
-      const double d = dist3(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
+      const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
       const double w = wendland<1, double>(d, a.rinv) / sum;
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
