@@ -29,7 +29,9 @@ __host__ __device__ inline double sqdiff3(double lx, double ly, double lz,
 
 // Wendland kernels of src/rbf_pum/solver/rbf_functions.hpp:22-150.
 // KIND 0..4 = C0, C2, C4, C6, C8; p = r * radiusInv, zero outside [0, 1).
-template <int KIND, typename T>
+// NONNEG: the caller guarantees r >= 0 and rinv > 0 (a distance), so the
+// mask reduces to p < 1 with the same result.
+template <int KIND, typename T, bool NONNEG = false>
 __host__ __device__ inline T wendland(T r, T rinv) {
   const T p = r * rinv;
   const T q = T(1) - p;
@@ -52,7 +54,10 @@ __host__ __device__ inline T wendland(T r, T rinv) {
     v = q10 * (T(1287) * p2 * p2 + T(1350) * p2 * p + T(630) * p2 +
                (T(150) * p + T(15)));
   }
-  return (r >= T(0) && p < T(1) && p >= T(0)) ? v : T(0);
+  if constexpr (NONNEG)
+    return p < T(1) ? v : T(0);
+  else
+    return (r >= T(0) && p < T(1) && p >= T(0)) ? v : T(0);
 }
 
 // A cell-sorted point set. Cells are a uniform grid over the set's bounding

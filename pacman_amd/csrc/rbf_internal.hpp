@@ -145,8 +145,13 @@ struct SolveArgs {
   int *pvOut;               // K (optional) active polynomial columns
   int *status;              // K (optional) nonzero on a zero pivot
 };
-// Launches the fused FP64 per-patch kernel for the bin with R row blocks.
-void launch_solve_f64(int R, int kind, const SolveArgs &a, hipStream_t s);
+// Fields handled by one poly/solve launch.
+int solve_fields_per_launch();
+// Polynomial fit of every patch (one wave per patch): coef[K][F][4], pvOut.
+void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s);
+// Launches the FP64 per-patch solve for the bin with R 16-row blocks.
+void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
+                      hipStream_t s);
 
 struct BlendArgs {
   const int64_t *toff;      // M + 1

@@ -17,7 +17,6 @@ namespace pacman_amd {
 namespace {
 // Largest patch the register-resident kernel takes (16 x 12 rows).
 constexpr int kMaxPatch = 192;
-constexpr int kFieldsPerLaunch = 8;
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(
@@ -257,11 +256,14 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
   launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
   PACMAN_HIP_CHECK(hipEventRecord(ev_[1], s));
   std::vector<BlendArgs> blends;
-  const int Fc = std::min(F, kFieldsPerLaunch);
+  const int perLaunch = solve_fields_per_launch();
+  const int Fc = std::min(F, perLaunch);
   if (contrib_.n < (size_t)(P * Fc))
     contrib_.alloc((size_t)P * Fc);
-  for (int f0 = 0; f0 < F; f0 += kFieldsPerLaunch) {
-    const int nf = std::min(kFieldsPerLaunch, F - f0);
+  if (coef_.n < (size_t)(std::max<int64_t>(K_, 1) * Fc * 4))
+    coef_.alloc((size_t)std::max<int64_t>(K_, 1) * Fc * 4);
+  for (int f0 = 0; f0 < F; f0 += perLaunch) {
+    const int nf = std::min(perLaunch, F - f0);
     SolveArgs a{};
     a.srcOff = srcOff_.get();
     a.srcIdx = srcIdx_.get();
@@ -286,10 +288,11 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
     a.dim = prm_.dim;
     a.pvOut = pv_.get();
     a.status = status_.get();
+    launch_poly(a, K_, coef_.get(), s);
     for (size_t b = 0; b < binR_.size(); ++b) {
       a.patches = binDev_[b].get();
       a.count = (int)binPatches_[b].size();
-      launch_solve_f64(binR_[b], prm_.kind, a, s);
+      launch_solve_f64(binR_[b], prm_.kind, a, coef_.get(), s);
     }
     BlendArgs bl{};
     bl.toff = pairs_.toff.get();
@@ -307,7 +310,7 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
     bl.F = nf;
     bl.rinv = 1.0 / r_;  // interpolator.hpp:91
     bl.out = dOut + (int64_t)f0 * M_;
-    if (F <= kFieldsPerLaunch) {
+    if (F <= perLaunch) {
       blends.push_back(bl);  // blend after all solves: phases stay separable
     } else {
       launch_blend(bl, s);

@@ -75,7 +75,7 @@ private:
   std::vector<int> binR_;
   int maxN_ = 0;
   DevBuf<int> pv_, status_;
-  DevBuf<double> valsSorted_, contrib_;
+  DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
   bool applied_ = false;

@@ -1,43 +1,47 @@
 // Per-patch RBF-PUM solve and partition-of-unity blend (gfx950, FP64).
 //
-// One 256-thread workgroup owns one patch (n sources, m targets) and runs
-// the whole per-patch chain of src/rbf_pum/solver/solver.hpp:180-286 without
-// writing any matrix to HBM:
-//   1. gather the patch's source coordinates and values into LDS;
-//   2. wave 0: least-squares fit of the linear polynomial
-//      (SerialSolveQR, serial_solve.hpp:46-67) with the conditioning loop of
-//      SerialFillQ (serial_ops.hpp:108-139). The Gram matrix of the basis
-//      [1, (x_a - c_a) / r] (c = patch centre, r = patch radius) is formed in
-//      one batched wave reduction; it spans the same space as the
-//      reference's [1, x_a] and is well conditioned. The condition number of
-//      the reference's raw Q = [1, x_a] is that of R'' T (R'' = Cholesky
-//      factor of the scaled Gram matrix, T the basis change): a Frobenius
-//      bound settles cond <= 1e5 at once, the exact Jacobi singular values
-//      are computed only otherwise. While cond > 1e5 the active axis with
-//      the smallest coordinate range is dropped (serial_ops.hpp:28-57); the
-//      fit then uses exactly the active columns. Residual b - Q beta
-//      (solver.hpp:255-263);
-//   3. all threads: assemble the Wendland matrix A straight into registers.
-//      Thread (tr, tc) of the 16 x 16 thread grid owns rows tr + 16a and
-//      columns tc + 16b for b <= a (every lower-triangle entry, cyclically);
-//   4. factor A with the symmetric form of the reference's unpivoted
-//      Doolittle LU (serial_solve.hpp:76-84): A = L D L^T has exactly the
-//      pivots d_j of that LU, half its flops and no upper triangle. The
-//      column loop is split into a compile-time loop over 16-column blocks
-//      and a run-time loop inside the block, so every thread's active
-//      register range is static; one barrier per column, the column and its
-//      pivot reciprocal double-buffered in LDS. The right-hand sides are
-//      eliminated in the same sweep;
-//   5. back substitution L^T alpha = D^{-1} y, 16 rows at a time;
-//   6. t(y) = E alpha + V beta at every member target (solver.hpp:265-273),
-//      written to that (target, patch) slot; the blend kernel sums the slots
-//      of each target in ascending patch order with the Shepard-normalised
-//      Wendland-C2 weights (solver.hpp:128-169, 275-285), deterministically.
+// Two kernels per size bin run the per-patch chain of
+// src/rbf_pum/solver/solver.hpp:180-286 without writing any matrix to HBM.
+//
+// poly_kernel (one wave per patch): least-squares fit of the linear
+// polynomial (SerialSolveQR, serial_solve.hpp:46-67) with the conditioning
+// loop of SerialFillQ (serial_ops.hpp:108-139). The Gram matrix of the basis
+// [1, (x_a - c_a) / r] (c = patch centre, r = patch radius) is reduced over
+// the wave; it spans the same space as the reference's [1, x_a] and is well
+// conditioned. The condition number of the reference's raw Q = [1, x_a] is
+// that of R'' T (R'' = Cholesky factor of the scaled Gram matrix, T the
+// basis change): a Frobenius bound settles cond <= 1e5 at once, the exact
+// Jacobi singular values are computed only otherwise. While cond > 1e5 the
+// active axis of smallest coordinate range is dropped (serial_ops.hpp:28-57)
+// and the fit uses exactly the active columns. It writes p(x) = c0 +
+// c1 (x - cx) + c2 (y - cy) + c3 (z - cz) per field and the active column
+// count.
+//
+// solve_f64_kernel (one workgroup of 16 x GC threads per patch):
+//   1. gathers the patch's source coordinates and values into LDS and
+//      subtracts the polynomial (the residual b - Q beta, solver.hpp:255-263);
+//   2. assembles the Wendland matrix A straight into registers: thread
+//      (tr, tc) owns rows tr + 16a and columns tc + GC b of every block that
+//      holds lower-triangle entries (cyclic, so every wave keeps work while
+//      the trailing matrix shrinks);
+//   3. factors A = L D L^T, the symmetric form of the reference's unpivoted
+//      Doolittle LU (serial_solve.hpp:76-84): the same pivots, half the flops,
+//      no upper triangle. KGRP columns per barrier: their owners publish the
+//      raw columns, every thread factors the small diagonal group itself,
+//      transforms its own row and column values and applies the rank-KGRP
+//      update; the right-hand sides are eliminated in the same sweep;
+//   4. back substitution L^T alpha = D^{-1} y, 16 rows at a time;
+//   5. t(y) = E alpha + V beta at every member target (solver.hpp:265-273),
+//      written to that (target, patch) slot.
+// blend_kernel sums the slots of each target in ascending patch order with
+// the Shepard-normalised Wendland-C2 weights (solver.hpp:128-169, 275-285),
+// deterministically (the reference uses atomic_add in arbitrary order).
 
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 #include <climits>
+#include <type_traits>
 
 #include "rbf_internal.hpp"
 
@@ -45,9 +49,18 @@ namespace pacman_amd {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kG = 16;      // thread grid is kG x kG
-constexpr int kFMax = 8;    // fields per launch
+constexpr int kFMax = 4;   // fields per launch
+constexpr int KGRP = 2;    // columns factored per barrier
+
+// Compile-time loop: f(integral_constant<int, I>) for I in [B, E). Keeps
+// register-array indices constant without relying on unrolling heuristics.
+template <int B, int E, class Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 // sqrt for the dense arithmetic: hardware reciprocal square root refined by
 // two Newton-Raphson (Goldschmidt) steps; within an ulp for the distances
@@ -69,6 +82,10 @@ __device__ inline double dist_fast(double ax, double ay, double az, double bx,
   const double dx = bx - ax, dy = by - ay, dz = bz - az;
   return fast_sqrt(dx * dx + dy * dy + dz * dz);
 }
+
+// ---------------------------------------------------------------------------
+// Polynomial fit
+// ---------------------------------------------------------------------------
 
 // Singular values (descending) of a p x p upper-triangular R by one-sided
 // Jacobi; p <= 4. All register arrays are indexed by compile-time constants.
@@ -127,11 +144,6 @@ __device__ inline void small_svd(double (&U)[4][4], int p, double (&S)[4]) {
       }
 }
 
-// Column c (0..3) of the active basis holds axis ca[c] (ca[0] = -1: the
-// constant). Entries of a 4 x 4 matrix over the full basis [1, u0, u1, u2]
-// are gathered with selects so every register index stays static.
-__device__ inline int basis_index(int axis) { return axis + 1; }
-
 __device__ inline double pick4(const double (&G)[4][4], int r, int c) {
   double v = 0.0;
 #pragma unroll
@@ -142,6 +154,8 @@ __device__ inline double pick4(const double (&G)[4][4], int r, int c) {
   return v;
 }
 
+// Column c (0..3) of the active basis holds axis ca[c] (ca[0] = -1: the
+// constant column).
 __device__ inline int assign_columns(const bool (&active)[3], int (&ca)[4]) {
   int cols = 1;
   ca[0] = -1;
@@ -192,255 +206,193 @@ __device__ inline bool chol4(double (&L)[4][4], int p) {
   return ok;
 }
 
-template <int R, int KIND>
-__global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
-  constexpr int NMAX = kG * R;
-  constexpr int NR = (NMAX + 63) / 64;
-  constexpr int NA = R * (R + 1) / 2;
+// cond(raw Q) > 1e5 for the active columns? L is the Cholesky factor of the
+// scaled Gram matrix, c the centre, sc = 1 / r.
+__device__ inline bool ill_conditioned(const double (&L)[4][4], int pv,
+                                       const int (&ca)[4], const double (&c)[3],
+                                       double sc) {
+  // raw basis [1, x_a] = [1, u_a] T, T_00 = 1, T_0j = c_j, T_jj = 1 / sc;
+  // R = R'' T with R'' = L^T (upper triangular)
+  double Rr[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double v = 0.0;
+      if (i < pv && j < pv && i <= j) {
+        double cj = 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+          cj = (ca[j] == ax) ? c[ax] : cj;
+        if (j == 0)
+          v = L[0][0];
+        else if (i == 0)
+          v = L[0][0] * cj + L[j][0] / sc;
+        else
+          v = L[j][i] / sc;
+      }
+      Rr[i][j] = v;
+    }
+  // ||R||_F ||R^-1||_F >= cond_2(R): settles cond <= 1e5 cheaply; R^-1 is
+  // accumulated one column at a time.
+  double fr = 0.0, fi = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      fr += Rr[i][j] * Rr[i][j];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double col[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+      if (i <= j && j < pv) {
+        double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k > i && k <= j)
+            s -= Rr[i][k] * col[k];
+        col[i] = s / Rr[i][i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      fi += col[i] * col[i];
+  }
+  if (sqrt(fr) * sqrt(fi) <= kCondMax)
+    return false;
+  double S[4];
+  small_svd(Rr, pv, S);
+  const double smin = pv == 4 ? S[3] : pv == 3 ? S[2] : S[1];
+  return S[0] / fmax(smin, kEps) > kCondMax;
+}
 
-  __shared__ double px[NMAX], py[NMAX], pz[NMAX];
-  __shared__ double rhs[kFMax][NMAX];
-  __shared__ double colbuf[2][NMAX];
-  __shared__ double dinvs[NMAX];
-  __shared__ double cbinv[2];
-  __shared__ double dblk[kG][kG + 1];
-  __shared__ double spart[kFMax][kG];
-  __shared__ double scoef[kFMax][4];
-  __shared__ int spv, spivot;
-
-  const int patch = a.patches[xcd_remap(blockIdx.x, a.count)];
+// One wave per patch; coef[(patch * F + f) * 4 + k], pvOut[patch].
+__global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
+                                                   double *coef) {
+  const int patch = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (patch >= K)
+    return;
+  const int lane = threadIdx.x & 63;
   const int64_t s0 = a.srcOff[patch];
   const int n = (int)(a.srcOff[patch + 1] - s0);
-  const int F = a.F;
-  const int tid = threadIdx.x;
-  const int tr = tid & (kG - 1);
-  const int tc = tid >> 4;
-  const int dim = a.dim;
-
-  // 1. gather
-  for (int t = tid; t < NMAX; t += kThreads) {
-    if (t < n) {
-      const int p = a.srcIdx[s0 + t];
-      px[t] = a.sx[p];
-      py[t] = a.sy[p];
-      pz[t] = a.sz[p];
-      for (int f = 0; f < F; ++f)
-        rhs[f][t] = a.vals[(int64_t)f * a.N + p];
-    } else {
-      px[t] = 0.0;
-      py[t] = 0.0;
-      pz[t] = 0.0;
-      for (int f = 0; f < F; ++f)
-        rhs[f][t] = 0.0;
-    }
-    dinvs[t] = 1.0;
-  }
-  if (tid == 0)
-    spivot = 0;
-  __syncthreads();
-
-  // 2. polynomial least squares (wave 0)
-  if (tid < kWave) {
-    const int lane = tid;
-    const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
-    const double sc = a.rinv;
-    double U3[3][NR];
-    bool valid[NR];
+  const int F = a.F, dim = a.dim;
+  const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
+  const double sc = a.rinv;
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+  double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  double G[4][4];
 #pragma unroll
-    for (int s = 0; s < NR; ++s) {
-      const int t = lane + 64 * s;
-      valid[s] = t < n;
-      U3[0][s] = valid[s] ? (px[t] - c[0]) * sc : 0.0;
-      U3[1][s] = valid[s] ? (py[t] - c[1]) * sc : 0.0;
-      U3[2][s] = valid[s] ? (pz[t] - c[2]) * sc : 0.0;
-    }
-    // one batched reduction: coordinate ranges and the Gram matrix
-    double mn[3], mx[3], G[4][4];
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      G[i][j] = 0.0;
+  for (int t = lane; t < n; t += 64) {
+    const int p = a.srcIdx[s0 + t];
+    const double x[3] = {a.sx[p], a.sy[p], a.sz[p]};
+    double u[4] = {1.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-      mn[ax] = DBL_MAX;
-      mx[ax] = -DBL_MAX;
-#pragma unroll
-      for (int s = 0; s < NR; ++s)
-        if (valid[s]) {
-          const int t = lane + 64 * s;
-          const double x = ax == 0 ? px[t] : ax == 1 ? py[t] : pz[t];
-          mn[ax] = fmin(mn[ax], x);
-          mx[ax] = fmax(mx[ax], x);
-        }
+      mn[ax] = fmin(mn[ax], x[ax]);
+      mx[ax] = fmax(mx[ax], x[ax]);
+      u[ax + 1] = (x[ax] - c[ax]) * sc;
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j)
+        G[i][j] += u[i] * u[j];
+  }
+  double range[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+    range[ax] = fabs(wave_max(mx[ax]) - wave_min(mn[ax]));
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      G[i][j] = wave_sum(G[i][j]);
+      G[j][i] = G[i][j];
+    }
+  bool active[3] = {dim > 0, dim > 1, dim > 2};
+  int ca[4];
+  int pv = assign_columns(active, ca);
+  double L[4][4];
+  for (;;) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        G[i][j] = 0.0;
+        L[i][j] = (i < pv && j < pv) ? pick4(G, ca[i] + 1, ca[j] + 1)
+                                     : (i == j ? 1.0 : 0.0);
+    const bool ok = chol4(L, pv);
+    if (pv <= 1)
+      break;
+    if (ok && !ill_conditioned(L, pv, ca, c, sc))
+      break;
+    double best = DBL_MAX;
+    int bi = 0;
 #pragma unroll
-    for (int s = 0; s < NR; ++s) {
-      const double u[4] = {valid[s] ? 1.0 : 0.0, U3[0][s], U3[1][s], U3[2][s]};
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j <= i; ++j)
-          G[i][j] += u[i] * u[j];
+    for (int ax = 0; ax < 3; ++ax) {
+      const double rg = (ax < dim && active[ax]) ? range[ax] : DBL_MAX;
+      if (rg < best) {
+        best = rg;
+        bi = ax;
+      }
     }
-    double range[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
-      range[ax] = fabs(wave_max(mx[ax]) - wave_min(mn[ax]));
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        G[i][j] = wave_sum(G[i][j]);
-        G[j][i] = G[i][j];
-      }
-
-    bool active[3] = {dim > 0, dim > 1, dim > 2};
-    int ca[4];
-    int pv = assign_columns(active, ca);
-    double L[4][4];
-    bool ok = true;
-    for (;;) {
-      // Gram of the active columns, its Cholesky factor, cond(raw Q)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          L[i][j] = (i < pv && j < pv)
-                        ? pick4(G, ca[i] + 1, ca[j] + 1)
-                        : (i == j ? 1.0 : 0.0);
-      ok = chol4(L, pv);
-      if (pv <= 1)
-        break;
-      bool keep = false;
-      if (ok) {
-        // raw basis [1, x_a] = [1, u_a] T with T = [[1, c_a], [0, 1/sc]]
-        double Rr[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            double v = 0.0;
-            if (i < pv && j < pv && i <= j) {
-              // (R'' T)_ij = sum_m R''_im T_mj with R'' = L^T, T_00 = 1,
-              // T_0j = c_j, T_jj = 1 / sc (j >= 1), zero elsewhere.
-              double cj = 0.0;
-#pragma unroll
-              for (int ax = 0; ax < 3; ++ax)
-                cj = (ca[j] == ax) ? c[ax] : cj;
-              if (j == 0)
-                v = L[0][0];
-              else if (i == 0)
-                v = L[0][0] * cj + L[j][0] / sc;
-              else
-                v = L[j][i] / sc;
-            }
-            Rr[i][j] = v;
-          }
-        // ||R||_F * ||R^-1||_F >= cond_2(R): settles cond <= 1e5 cheaply
-        double Ri[4][4];
-        double fr = 0.0, fi = 0.0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-          for (int i = 3; i >= 0; --i) {
-            double v = 0.0;
-            if (i <= j && j < pv) {
-              double s = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-              for (int k = 0; k < 4; ++k)
-                if (k > i && k <= j)
-                  s -= Rr[i][k] * Ri[k][j];
-              v = s / Rr[i][i];
-            }
-            Ri[i][j] = v;
-          }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            fr += Rr[i][j] * Rr[i][j];
-            fi += Ri[i][j] * Ri[i][j];
-          }
-        if (sqrt(fr) * sqrt(fi) <= kCondMax) {
-          keep = true;
-        } else {
-          double S[4];
-          small_svd(Rr, pv, S);
-          const double smin = pv == 4 ? S[3] : pv == 3 ? S[2] : S[1];
-          keep = !(S[0] / fmax(smin, kEps) > kCondMax);
-        }
-      }
-      if (keep)
-        break;
-      double best = DBL_MAX;
-      int bi = 0;
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax) {
-        const double rg = (ax < dim && active[ax]) ? range[ax] : DBL_MAX;
-        if (rg < best) {
-          best = rg;
-          bi = ax;
-        }
-      }
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax)
-        if (ax == bi)
-          active[ax] = false;
-      pv = assign_columns(active, ca);
+      if (ax == bi)
+        active[ax] = false;
+    pv = assign_columns(active, ca);
+  }
+  // beta'' = G^-1 Q''^T b on the active columns (normal equations of the
+  // well-conditioned scaled basis); coefficients returned around the centre.
+  for (int f = 0; f < F; ++f) {
+    double hf[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = lane; t < n; t += 64) {
+      const int p = a.srcIdx[s0 + t];
+      const double b = a.vals[(int64_t)f * a.N + p];
+      hf[0] += b;
+      hf[1] += (a.sx[p] - c[0]) * sc * b;
+      hf[2] += (a.sy[p] - c[1]) * sc * b;
+      hf[3] += (a.sz[p] - c[2]) * sc * b;
     }
-    // beta'' = G^-1 h on the active columns (normal equations of the
-    // well-conditioned scaled basis), then the residual b - Q beta.
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f) {
-      if (f >= F)
-        continue;
-      // Q''^T b over the full scaled basis
-      double hf[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 4; ++k)
+      hf[k] = wave_sum(hf[k]);
+    double y[4], beta[4];
 #pragma unroll
-      for (int s = 0; s < NR; ++s) {
-        const int t = lane + 64 * s;
-        const double b = valid[s] ? rhs[f][t] : 0.0;
-        hf[0] += b;
-        hf[1] += U3[0][s] * b;
-        hf[2] += U3[1][s] * b;
-        hf[3] += U3[2][s] * b;
-      }
+    for (int i = 0; i < 4; ++i) {
+      double hv = 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        hf[k] = wave_sum(hf[k]);
-      double y[4], beta[4];
+        hv = (ca[i] + 1 == k) ? hf[k] : hv;
+      y[i] = hv;
+    }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        double hv = 0.0;
+    for (int i = 0; i < 4; ++i)
+      if (i < pv) {
+        double s = y[i];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          hv = (ca[i] + 1 == k) ? hf[k] : hv;
-        y[i] = hv;
+          if (k < i)
+            s -= L[i][k] * y[k];
+        y[i] = L[i][i] > 0.0 ? s / L[i][i] : 0.0;
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (i < pv) {
-          double s = y[i];
+    for (int i = 3; i >= 0; --i) {
+      beta[i] = 0.0;
+      if (i < pv) {
+        double s = y[i];
 #pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (k < i)
-              s -= L[i][k] * y[k];
-          y[i] = L[i][i] > 0.0 ? s / L[i][i] : 0.0;
-        }
-#pragma unroll
-      for (int i = 3; i >= 0; --i) {
-        beta[i] = 0.0;
-        if (i < pv) {
-          double s = y[i];
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (k > i && k < pv)
-              s -= L[k][i] * beta[k];
-          beta[i] = L[i][i] > 0.0 ? s / L[i][i] : 0.0;
-        }
+        for (int k = 0; k < 4; ++k)
+          if (k > i && k < pv)
+            s -= L[k][i] * beta[k];
+        beta[i] = L[i][i] > 0.0 ? s / L[i][i] : 0.0;
       }
-      // per-axis coefficients of the scaled basis (inactive axes: zero)
+    }
+    if (lane == 0) {
       double cax[3];
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax)
@@ -448,113 +400,318 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
                   : ca[2] == ax ? beta[2]
                   : ca[3] == ax ? beta[3]
                                 : 0.0;
-#pragma unroll
-      for (int s = 0; s < NR; ++s) {
-        const int t = lane + 64 * s;
-        if (valid[s])
-          rhs[f][t] -= beta[0] + cax[0] * U3[0][s] + cax[1] * U3[1][s] +
-                       cax[2] * U3[2][s];
-      }
-      if (lane == 0) {
-        scoef[f][0] = beta[0];
-        scoef[f][1] = cax[0] * sc;
-        scoef[f][2] = cax[1] * sc;
-        scoef[f][3] = cax[2] * sc;
-      }
+      double *o = coef + ((int64_t)patch * F + f) * 4;
+      o[0] = beta[0];
+      o[1] = cax[0] * sc;
+      o[2] = cax[1] * sc;
+      o[3] = cax[2] * sc;
     }
-    if (lane == 0)
-      spv = pv;
   }
+  if (lane == 0 && a.pvOut)
+    a.pvOut[patch] = pv;
+}
 
-  // 3. assemble A (lower block triangle, registers)
-  double A[NA];
-#pragma unroll
-  for (int ai = 0; ai < R; ++ai) {
-    const int i = tr + kG * ai;
-    const double xi = px[i], yi = py[i], zi = pz[i];
-#pragma unroll
-    for (int bi = 0; bi <= ai; ++bi) {
-      const int l = tc + kG * bi;
-      double v;
-      if (i < n && l < n)
-        v = wendland<KIND, double>(dist_fast(xi, yi, zi, px[l], py[l], pz[l]),
-                                   a.dinv);
-      else
-        v = (i == l) ? 1.0 : 0.0;
-      A[ai * (ai + 1) / 2 + bi] = v;
+// ---------------------------------------------------------------------------
+// Per-patch solve
+// ---------------------------------------------------------------------------
+
+// Register layout for a 16 x GC thread grid and R 16-row blocks: column
+// block b (GC columns) holds lower-triangle entries from row block S b on,
+// S = GC / 16. Entries (a, b), a >= S b, are stored at off(b) + a - S b.
+template <int R, int GC> struct Layout {
+  static constexpr int S = GC / 16;
+  static constexpr int NMAX = 16 * R;
+  static constexpr int RB = (NMAX + GC - 1) / GC;
+  static constexpr int amin(int b) { return S * b; }
+  static constexpr int off(int b) { return b * R - S * b * (b - 1) / 2; }
+  static constexpr int idx(int a, int b) { return off(b) + a - S * b; }
+  static constexpr int NA = off(RB);
+};
+
+template <int R, int GC, int KIND>
+__global__ __launch_bounds__(16 * GC, GC == 32 ? 4 : 2) void solve_f64_kernel(SolveArgs a,
+                                                            const double *coef) {
+  using LY = Layout<R, GC>;
+  constexpr int NT = 16 * GC;
+  constexpr int NMAX = LY::NMAX;
+  constexpr int RB = LY::RB;
+  constexpr int NSL = NT / 32;  // evaluation slices
+
+  __shared__ double px[NMAX], py[NMAX], pz[NMAX];
+  __shared__ double rhs[kFMax][NMAX];
+  __shared__ double yv[kFMax][NMAX];
+  __shared__ double gcol[2][KGRP][NMAX];
+  __shared__ double dinvs[NMAX];
+  __shared__ double dblk[16][17];
+  __shared__ double spart[kFMax][16];
+  __shared__ double scoef[kFMax][4];
+  __shared__ double epart[NSL][kFMax][32];
+  __shared__ int spivot;
+
+  const int patch = a.patches[xcd_remap(blockIdx.x, a.count)];
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int F = a.F;
+  const int tid = threadIdx.x;
+  const int tr = tid & 15;
+  const int tc = tid >> 4;
+  const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
+
+  if (tid < kFMax * 4)
+    scoef[tid >> 2][tid & 3] =
+        (tid >> 2) < F ? coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)]
+                       : 0.0;
+  if (tid == 0)
+    spivot = 0;
+  __syncthreads();
+
+  // 1. gather; right-hand side = value - polynomial fit
+  for (int t = tid; t < NMAX; t += NT) {
+    double x = 0.0, y = 0.0, z = 0.0;
+    int p = 0;
+    if (t < n) {
+      p = a.srcIdx[s0 + t];
+      x = a.sx[p];
+      y = a.sy[p];
+      z = a.sz[p];
     }
+    px[t] = x;
+    py[t] = y;
+    pz[t] = z;
+    for (int f = 0; f < F; ++f) {
+      double v = 0.0;
+      if (t < n)
+        v = a.vals[(int64_t)f * a.N + p] -
+            (scoef[f][0] + scoef[f][1] * (x - cx) + scoef[f][2] * (y - cy) +
+             scoef[f][3] * (z - cz));
+      rhs[f][t] = v;
+      yv[f][t] = 0.0;
+    }
+    dinvs[t] = 1.0;
   }
   __syncthreads();
 
-  // 4. A = L D L^T with forward elimination of the right-hand sides
+  // 2. assemble A into registers
+  double A[LY::NA];
+  static_for<0, RB>([&](auto bc) {
+    constexpr int b = decltype(bc)::value;
+    const int l = tc + GC * b;
+    const int lc = l < NMAX ? l : 0;
+    const double xl = px[lc], yl = py[lc], zl = pz[lc];
+    static_for<LY::amin(b), R>([&](auto ac) {
+      constexpr int ai = decltype(ac)::value;
+      const int i = tr + 16 * ai;
+      double v;
+      if (i < n && l < n)
+        v = wendland<KIND, double, true>(
+            dist_fast(px[i], py[i], pz[i], xl, yl, zl), a.dinv);
+      else
+        v = (i == l) ? 1.0 : 0.0;
+      A[LY::idx(ai, b)] = v;
+    });
+  });
+
+  // 3. A = L D L^T, KGRP columns per barrier. Group g of column block cb
+  // holds columns j0 = GC cb + KGRP g ... j0 + KGRP - 1; their owners (tc in
+  // the group) publish the raw columns (rows >= j0). Every thread factors
+  // the KGRP x KGRP diagonal group itself, transforms its own row and column
+  // values (z = a L_g^-T, w = z D_g^-1) and applies A(i, l) -= z_i . w_l;
+  // the owners keep z (unnormalised L) as the final column values.
+  static_for<0, RB>([&](auto cbc) {
+    constexpr int cb = decltype(cbc)::value;
+    if (GC * cb >= n)
+      return;
+    static_for<0, GC / KGRP>([&](auto gc) {
+      constexpr int g = decltype(gc)::value;
+      constexpr int j0 = GC * cb + KGRP * g;
+      constexpr int je = j0 + KGRP;
+      if (j0 >= n)
+        return;
+      double(*buf)[NMAX] = gcol[(cb * (GC / KGRP) + g) & 1];
+      const bool owner = (tc / KGRP) == g;
+      const int qo = tc % KGRP;
+      if (owner) {
+        static_for<LY::amin(cb), R>([&](auto ac) {
+          constexpr int ai = decltype(ac)::value;
+          if constexpr (16 * ai + 15 >= j0) {
+            const int i = tr + 16 * ai;
+            if (i >= j0)
+              buf[qo][i] = A[LY::idx(ai, cb)];
+          }
+        });
+      }
+      __syncthreads();
+      // small L D L^T of the diagonal group (identical in every thread)
+      double Gc[KGRP][KGRP], Lg[KGRP][KGRP], dv[KGRP], di[KGRP];
 #pragma unroll
-  for (int jb = 0; jb < R; ++jb) {
-    if (kG * jb >= n)
-      break;
-    for (int jt = 0; jt < kG; ++jt) {
-      const int j = kG * jb + jt;
-      if (j >= n)
-        break;
-      double *cb = colbuf[j & 1];
-      if (tc == jt) {
+      for (int p = 0; p < KGRP; ++p)
 #pragma unroll
-        for (int ai = jb; ai < R; ++ai) {
-          const int i = tr + kG * ai;
-          if (ai > jb || i >= j) {
-            const double v = A[ai * (ai + 1) / 2 + jb];
-            cb[i] = v;
-            if (ai == jb && i == j) {
-              const double iv = 1.0 / v;
-              cbinv[j & 1] = iv;
-              dinvs[j] = iv;
-              if (v == 0.0)
-                spivot = 1;
+        for (int q = 0; q <= p; ++q)
+          Gc[p][q] = buf[q][j0 + p];
+#pragma unroll
+      for (int q = 0; q < KGRP; ++q) {
+        dv[q] = Gc[q][q];
+        di[q] = 1.0 / dv[q];
+#pragma unroll
+        for (int p = q + 1; p < KGRP; ++p)
+          Lg[p][q] = Gc[p][q] * di[q];
+#pragma unroll
+        for (int p = q + 1; p < KGRP; ++p)
+#pragma unroll
+          for (int r2 = q + 1; r2 <= p; ++r2)
+            Gc[p][r2] = fma(-Lg[p][q], Gc[r2][q], Gc[p][r2]);
+      }
+      if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < KGRP; ++q) {
+          dinvs[j0 + q] = di[q];
+          if (dv[q] == 0.0)
+            spivot = 1;
+        }
+      }
+      // column factors of this thread's columns l >= je
+      double w[RB][KGRP];
+      static_for<cb, RB>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        const int l = tc + GC * b;
+        const bool act = b > cb || tc >= KGRP * g + KGRP;
+        double z[KGRP];
+#pragma unroll
+        for (int q = 0; q < KGRP; ++q) {
+          double v = act ? buf[q][l] : 0.0;
+#pragma unroll
+          for (int p = 0; p < q; ++p)
+            v = fma(-z[p], Lg[q][p], v);
+          z[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < KGRP; ++q)
+          w[b][q] = z[q] * di[q];
+      });
+      // rows: transform, rank-KGRP update, owners keep the final columns
+      static_for<LY::amin(cb), R>([&](auto ac) {
+        constexpr int ai = decltype(ac)::value;
+        constexpr bool rowNever = 16 * ai + 15 < je;
+        constexpr bool rowAlways = 16 * ai >= je;
+        const int i = tr + 16 * ai;
+        if constexpr (!rowNever) {
+          const bool act = rowAlways || i >= je;
+          double z[KGRP];
+#pragma unroll
+          for (int q = 0; q < KGRP; ++q) {
+            double v = act ? buf[q][i] : 0.0;
+#pragma unroll
+            for (int p = 0; p < q; ++p)
+              v = fma(-z[p], Lg[q][p], v);
+            z[q] = v;
+          }
+          static_for<cb, RB>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            if constexpr (ai >= LY::amin(b)) {
+              double acc = A[LY::idx(ai, b)];
+#pragma unroll
+              for (int q = 0; q < KGRP; ++q)
+                acc = fma(-z[q], w[b][q], acc);
+              A[LY::idx(ai, b)] = acc;
             }
+          });
+          if (owner && act) {
+            double zq = z[0];
+#pragma unroll
+            for (int q = 1; q < KGRP; ++q)
+              zq = (qo == q) ? z[q] : zq;
+            A[LY::idx(ai, cb)] = zq;
+          }
+        }
+        if constexpr (16 * ai + 15 >= j0 && 16 * ai < je) {
+          // rows inside the group: unnormalised L below, pivots on the diagonal
+          if (owner && i >= j0 && i < je) {
+            const int pr = i - j0;
+            double v = A[LY::idx(ai, cb)];
+#pragma unroll
+            for (int p = 0; p < KGRP; ++p)
+#pragma unroll
+              for (int q = 0; q < KGRP; ++q) {
+                if (p == pr && q == qo && p > q)
+                  v = Lg[p][q] * dv[q];
+                if (p == pr && q == qo && p == q)
+                  v = dv[q];
+              }
+            A[LY::idx(ai, cb)] = v;
+          }
+        }
+      });
+      // right-hand sides: y_g = L_g^-1 b_g; later rows b_t -= w_t . y_g
+      if (tid < NMAX) {
+        const bool rowAct = tid >= je && tid < n;
+        const bool inGrp = tid >= j0 && tid < je;
+        double wt[KGRP];
+        {
+          double z[KGRP];
+#pragma unroll
+          for (int q = 0; q < KGRP; ++q) {
+            double v = rowAct ? buf[q][tid] : 0.0;
+#pragma unroll
+            for (int p = 0; p < q; ++p)
+              v = fma(-z[p], Lg[q][p], v);
+            z[q] = v;
+          }
+#pragma unroll
+          for (int q = 0; q < KGRP; ++q)
+            wt[q] = z[q] * di[q];
+        }
+        for (int f = 0; f < F; ++f) {
+          double yq[KGRP];
+#pragma unroll
+          for (int q = 0; q < KGRP; ++q) {
+            double v = rhs[f][j0 + q];
+#pragma unroll
+            for (int p = 0; p < q; ++p)
+              v = fma(-Lg[q][p], yq[p], v);
+            yq[q] = v;
+          }
+          if (rowAct) {
+            double b = rhs[f][tid];
+#pragma unroll
+            for (int q = 0; q < KGRP; ++q)
+              b = fma(-wt[q], yq[q], b);
+            rhs[f][tid] = b;
+          }
+          if (inGrp) {
+            double v = yq[0];
+#pragma unroll
+            for (int q = 1; q < KGRP; ++q)
+              v = (tid - j0 == q) ? yq[q] : v;
+            yv[f][tid] = v;
           }
         }
       }
-      __syncthreads();
-      const double inv = cbinv[j & 1];
-      if (tid > j && tid < n) {
-        const double ct = cb[tid];
-        for (int f = 0; f < F; ++f)
-          rhs[f][tid] -= ct * (rhs[f][j] * inv);
-      }
-      double wl[R];
-#pragma unroll
-      for (int bi = jb; bi < R; ++bi) {
-        const int l = tc + kG * bi;
-        wl[bi] = (bi > jb || l > j) ? cb[l] * inv : 0.0;
-      }
-#pragma unroll
-      for (int ai = jb; ai < R; ++ai) {
-        const int i = tr + kG * ai;
-        const double ci = (ai > jb || i > j) ? cb[i] : 0.0;
-#pragma unroll
-        for (int bi = jb; bi <= ai; ++bi)
-          A[ai * (ai + 1) / 2 + bi] = fma(-ci, wl[bi], A[ai * (ai + 1) / 2 + bi]);
-      }
-    }
-  }
+    });
+  });
   __syncthreads();
 
-  // 5. back substitution, 16 rows per block, bottom-up
-#pragma unroll
-  for (int B = R - 1; B >= 0; --B) {
-    if (kG * B >= n)
-      continue;
+  // 4. back substitution, 16 rows per block, bottom-up
+  static_for<0, R>([&](auto bc) {
+    constexpr int B = R - 1 - decltype(bc)::value;
+    constexpr int bb = (16 * B) / GC;        // column block of rows 16B..
+    constexpr int toff = 16 * B - GC * bb;   // first tc of those columns
+    if (16 * B >= n)
+      return;
+    const bool mine = tc >= toff && tc < toff + 16;
     double pf[kFMax];
 #pragma unroll
     for (int f = 0; f < kFMax; ++f)
       pf[f] = 0.0;
+    if (mine) {
+      static_for<B + 1, R>([&](auto ac) {
+        constexpr int ai = decltype(ac)::value;
+        const double c = A[LY::idx(ai, bb)];
+        const int i = tr + 16 * ai;
 #pragma unroll
-    for (int ai = B + 1; ai < R; ++ai) {
-      const int i = tr + kG * ai;
-      const double c = A[ai * (ai + 1) / 2 + B];
-#pragma unroll
-      for (int f = 0; f < kFMax; ++f)
-        if (f < F)
-          pf[f] = fma(c, rhs[f][i], pf[f]);
+        for (int f = 0; f < kFMax; ++f)
+          if (f < F)
+            pf[f] = fma(c, rhs[f][i], pf[f]);
+      });
     }
 #pragma unroll
     for (int f = 0; f < kFMax; ++f) {
@@ -564,67 +721,98 @@ __global__ __launch_bounds__(kThreads) void solve_f64_kernel(SolveArgs a) {
         v += __shfl_xor(v, 4, kWave);
         v += __shfl_xor(v, 2, kWave);
         v += __shfl_xor(v, 1, kWave);
-        if (tr == 0)
-          spart[f][tc] = v;
+        if (mine && tr == 0)
+          spart[f][tc - toff] = v;
       }
     }
-    dblk[tr][tc] = A[B * (B + 1) / 2 + B];
+    if (mine)
+      dblk[tr][tc - toff] = A[LY::idx(B, bb)];
     __syncthreads();
-    if (tid < F) {
-      const int f = tid;
-      double sv[kG];
+    // 16 lanes of wave 0 solve the 16 x 16 unit-lower system, one row per
+    // lane; the solved value of row jj is broadcast from lane jj.
+    if (tid < kWave) {
+      const int k = tid & 15;
+      double col[16];
 #pragma unroll
-      for (int jj = 0; jj < kG; ++jj)
-        sv[jj] = rhs[f][kG * B + jj] - spart[f][jj];
+      for (int jj = 0; jj < 16; ++jj)
+        col[jj] = dblk[jj][k];
+      const double dk = dinvs[16 * B + k];
+      for (int f = 0; f < F; ++f) {
+        double sv = yv[f][16 * B + k] - spart[f][k];
 #pragma unroll
-      for (int jj = kG - 1; jj >= 0; --jj) {
-        const double al = sv[jj] * dinvs[kG * B + jj];
-        rhs[f][kG * B + jj] = al;
-#pragma unroll
-        for (int k = 0; k < jj; ++k)
-          sv[k] = fma(-dblk[jj][k], al, sv[k]);
+        for (int jj = 15; jj >= 0; --jj) {
+          const double al = __shfl(sv * dk, jj, kWave);
+          if (k < jj)
+            sv = fma(-col[jj], al, sv);
+          if (k == jj && tid < 16)
+            rhs[f][16 * B + jj] = al;
+        }
       }
     }
     __syncthreads();
-  }
+  });
 
-  if (tid == 0) {
-    if (a.pvOut)
-      a.pvOut[patch] = spv;
-    if (a.status)
-      a.status[patch] = spivot;
-  }
+  if (tid == 0 && a.status)
+    a.status[patch] = spivot;
 
-  // 6. evaluate at the member targets and write each (target, patch) slot
-  const int64_t p0 = a.pairOff[patch], p1 = a.pairOff[patch + 1];
-  for (int64_t p = p0 + tid; p < p1; p += kThreads) {
-    const int e = a.pairTgt[p];
-    if (((unsigned)e >> 31) == 0)
-      continue;
-    const int tp = e & 0x7fffffff;
-    const double yx = a.tx[tp], yy = a.ty[tp], yz = a.tz[tp];
-    double acc[kFMax];
+  // 5. evaluate at the member targets and write each (target, patch) slot.
+  // Tiles of 32 pairs; thread = (pair, slice): slice s of NSL sums sources
+  // [s n / NSL, (s + 1) n / NSL); slices are added in order (deterministic).
+  {
+    const int64_t p0 = a.pairOff[patch], p1 = a.pairOff[patch + 1];
+    const int mall = (int)(p1 - p0);
+    const int lane = tid & 31, sl = tid >> 5;
+    const int qn = (n + NSL - 1) / NSL;
+    const int l0 = min(n, sl * qn), l1 = min(n, l0 + qn);
+    for (int tile = 0; tile < mall; tile += 32) {
+      const int pi = tile + lane;
+      int e = 0;
+      if (pi < mall)
+        e = a.pairTgt[p0 + pi];
+      const bool mem = pi < mall && ((unsigned)e >> 31) != 0;
+      const int tp = e & 0x7fffffff;
+      double yx = 0.0, yy = 0.0, yz = 0.0;
+      if (mem) {
+        yx = a.tx[tp];
+        yy = a.ty[tp];
+        yz = a.tz[tp];
+      }
+      double acc[kFMax];
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
-      acc[f] = 0.0;
-    for (int l = 0; l < n; ++l) {
-      const double phi = wendland<KIND, double>(
-          dist_fast(yx, yy, yz, px[l], py[l], pz[l]), a.dinv);
+      for (int f = 0; f < kFMax; ++f)
+        acc[f] = 0.0;
+      if (mem) {
+        for (int l = l0; l < l1; ++l) {
+          const double phi = wendland<KIND, double, true>(
+              dist_fast(yx, yy, yz, px[l], py[l], pz[l]), a.dinv);
+#pragma unroll
+          for (int f = 0; f < kFMax; ++f)
+            if (f < F)
+              acc[f] = fma(phi, rhs[f][l], acc[f]);
+        }
+      }
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
         if (f < F)
-          acc[f] = fma(phi, rhs[f][l], acc[f]);
-    }
-    const int slot = a.pairSlot[p];
-    const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
+          epart[sl][f][lane] = acc[f];
+      __syncthreads();
+      if (sl == 0 && mem) {
+        const int slot = a.pairSlot[p0 + pi];
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
-      if (f < F) {
-        // V beta (solver.hpp:272) in the scaled basis around the centre
-        const double vb = scoef[f][0] + scoef[f][1] * (yx - cx) +
-                          scoef[f][2] * (yy - cy) + scoef[f][3] * (yz - cz);
-        a.contrib[(int64_t)slot * F + f] = acc[f] + vb;
+        for (int f = 0; f < kFMax; ++f)
+          if (f < F) {
+            double t = epart[0][f][lane];
+#pragma unroll
+            for (int q = 1; q < NSL; ++q)
+              t += epart[q][f][lane];
+            // V beta (solver.hpp:272), around the centre
+            const double vb = scoef[f][0] + scoef[f][1] * (yx - cx) +
+                              scoef[f][2] * (yy - cy) + scoef[f][3] * (yz - cz);
+            a.contrib[(int64_t)slot * F + f] = t + vb;
+          }
       }
+      __syncthreads();
+    }
   }
 }
 
@@ -640,27 +828,28 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
     const int k = a.slotPatch[s];
     // DistanceNoCheck(target, center), solver.hpp:158-161
     const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-    sum += wendland<1, double>(d, a.rinv);
+    sum += wendland<1, double, true>(d, a.rinv);
   }
   const int64_t o = a.tperm[t];
-  for (int f0 = 0; f0 < a.F; f0 += kFMax) {
-    double acc[kFMax];
+  constexpr int kB = 8;
+  for (int f0 = 0; f0 < a.F; f0 += kB) {
+    double acc[kB];
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
+    for (int f = 0; f < kB; ++f)
       acc[f] = 0.0;
     for (int64_t s = s0; s < s1; ++s) {
       if (!a.slotMember[s])
         continue;
       const int k = a.slotPatch[s];
       const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-      const double w = wendland<1, double>(d, a.rinv) / sum;
+      const double w = wendland<1, double, true>(d, a.rinv) / sum;
 #pragma unroll
-      for (int f = 0; f < kFMax; ++f)
+      for (int f = 0; f < kB; ++f)
         if (f0 + f < a.F)
           acc[f] += w * a.contrib[s * a.F + f0 + f];
     }
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
+    for (int f = 0; f < kB; ++f)
       if (f0 + f < a.F)
         a.out[(int64_t)(f0 + f) * a.M + o] = acc[f];
   }
@@ -676,24 +865,25 @@ __global__ void gather_values_kernel(const double *vals, const int *perm,
     out[(int64_t)f * N + p] = vals[(int64_t)f * N + i];
 }
 
-template <int R>
-void launch_r(int kind, const SolveArgs &a, hipStream_t s) {
-  const dim3 grid((unsigned)a.count), block(kThreads);
+template <int R, int GC>
+void launch_r(int kind, const SolveArgs &a, const double *coef,
+              hipStream_t s) {
+  const dim3 grid((unsigned)a.count), block(16 * GC);
   switch (kind) {
   case 0:
-    solve_f64_kernel<R, 0><<<grid, block, 0, s>>>(a);
+    solve_f64_kernel<R, GC, 0><<<grid, block, 0, s>>>(a, coef);
     break;
   case 1:
-    solve_f64_kernel<R, 1><<<grid, block, 0, s>>>(a);
+    solve_f64_kernel<R, GC, 1><<<grid, block, 0, s>>>(a, coef);
     break;
   case 2:
-    solve_f64_kernel<R, 2><<<grid, block, 0, s>>>(a);
+    solve_f64_kernel<R, GC, 2><<<grid, block, 0, s>>>(a, coef);
     break;
   case 3:
-    solve_f64_kernel<R, 3><<<grid, block, 0, s>>>(a);
+    solve_f64_kernel<R, GC, 3><<<grid, block, 0, s>>>(a, coef);
     break;
   default:
-    solve_f64_kernel<R, 4><<<grid, block, 0, s>>>(a);
+    solve_f64_kernel<R, GC, 4><<<grid, block, 0, s>>>(a, coef);
     break;
   }
   PACMAN_HIP_CHECK(hipGetLastError());
@@ -701,29 +891,41 @@ void launch_r(int kind, const SolveArgs &a, hipStream_t s) {
 
 } // namespace
 
-void launch_solve_f64(int R, int kind, const SolveArgs &a, hipStream_t s) {
+int solve_fields_per_launch() { return kFMax; }
+
+void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s) {
+  if (K <= 0)
+    return;
+  if (a.F < 1 || a.F > kFMax)
+    throw std::runtime_error("poly: 1 <= fields <= 4 per launch");
+  poly_kernel<<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, coef);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
+                      hipStream_t s) {
   if (a.count <= 0)
     return;
   if (a.F < 1 || a.F > kFMax)
-    throw std::runtime_error("solve: 1 <= fields <= 8 per launch");
+    throw std::runtime_error("solve: 1 <= fields <= 4 per launch");
   switch (R) {
   case 2:
-    launch_r<2>(kind, a, s);
+    launch_r<2, 16>(kind, a, coef, s);
     break;
   case 4:
-    launch_r<4>(kind, a, s);
+    launch_r<4, 16>(kind, a, coef, s);
     break;
   case 6:
-    launch_r<6>(kind, a, s);
+    launch_r<6, 16>(kind, a, coef, s);
     break;
   case 8:
-    launch_r<8>(kind, a, s);
+    launch_r<8, 32>(kind, a, coef, s);
     break;
   case 10:
-    launch_r<10>(kind, a, s);
+    launch_r<10, 32>(kind, a, coef, s);
     break;
   case 12:
-    launch_r<12>(kind, a, s);
+    launch_r<12, 32>(kind, a, coef, s);
     break;
   default:
     throw std::runtime_error("solve: unsupported register tile R");
