@@ -65,7 +65,8 @@ int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
 /* Phase times in milliseconds (up to n entries; returns the count written):
  * setup phases 0 grids, 1 patch radius, 2 patch centres, 3 membership,
  * 4 setup total (host clock); apply phases of the last apply (HIP events on
- * its stream) 5 value gather, 6 per-patch solve, 7 blend, 8 apply total. */
+ * its stream) 5 value gather, 6 per-patch solve (polynomial fit included),
+ * 7 blend, 8 apply total, 9 polynomial fit of the first field chunk. */
 int pacman_rbf_plan_phase_times(pacman_rbf_plan plan, double *ms, int n);
 
 /* Work counters (up to n entries; returns the count written): 0 patches K,

@@ -211,6 +211,8 @@ void RbfPumPlan::phaseTimes(double *ms) const {
   ms[7] = t;
   PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[0], ev_[3]));
   ms[8] = t;
+  PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[1], ev_[4]));
+  ms[9] = t;
 }
 
 void RbfPumPlan::work(double *w) const {
@@ -289,6 +291,8 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
     a.pvOut = pv_.get();
     a.status = status_.get();
     launch_poly(a, K_, coef_.get(), s);
+    if (f0 == 0)
+      PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
     for (size_t b = 0; b < binR_.size(); ++b) {
       a.patches = binDev_[b].get();
       a.count = (int)binPatches_[b].size();

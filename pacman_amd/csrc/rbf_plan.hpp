@@ -50,7 +50,7 @@ public:
   // synchronisation): 0 grids, 1 radius, 2 centres, 3 membership, 4 setup
   // total. Apply phases come from HIP events of the last apply on its
   // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total.
-  static constexpr int kPhases = 9;
+  static constexpr int kPhases = 10;
   void phaseTimes(double *ms) const;
   // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
   // targets), 5 pairs (targets within r of a centre), 6 max n.
@@ -77,7 +77,7 @@ private:
   DevBuf<int> pv_, status_;
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};
-  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool applied_ = false;
 };
 

@@ -226,7 +226,7 @@ class Plan:
         return off, idx[:size]
 
     PHASES = ("grids", "radius", "centres", "membership", "setup",
-              "gather", "solve", "blend", "apply")
+              "gather", "solve", "blend", "apply", "poly")
     WORK = ("patches", "sum_n", "sum_n2", "sum_n3", "sum_nm", "pairs",
             "max_n")
 
