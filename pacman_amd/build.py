@@ -7,7 +7,8 @@ contraction policy:
 * rbf_geometry.hip, rbf_plan.hip: ``-ffp-contract=off`` - distances,
   lattice coordinates and probe points must round exactly as written so the
   patch membership is bit-identical to the CPU restatement;
-* rbf_solve.hip: FMA contraction allowed (dense arithmetic).
+* rbf_solve.hip and the per-kind solve units rbf_solve_c0.hip ...
+  rbf_solve_c8.hip: FMA contraction allowed (dense arithmetic).
 
 All HIP code is compiled with the packed-FP32 instructions disabled.
 """
@@ -37,6 +38,11 @@ UNITS = {
     "rbf_plan.hip": ["-ffp-contract=off"],
     "capi.hip": ["-ffp-contract=off"],
     "rbf_solve.hip": [],
+    "rbf_solve_c0.hip": [],
+    "rbf_solve_c2.hip": [],
+    "rbf_solve_c4.hip": [],
+    "rbf_solve_c6.hip": [],
+    "rbf_solve_c8.hip": [],
 }
 
 
@@ -59,8 +65,12 @@ def _compile(src: str, extra: list[str]) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
+    workers = max(1, min(len(UNITS), os.cpu_count() or 1))
+    # The solve units take longest: start them first.
+    order = sorted(UNITS.items(), key=lambda kv: not kv[0].startswith(
+        "rbf_solve_c"))
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(lambda kv: _compile(*kv), order))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
            tmp]

@@ -152,6 +152,11 @@ void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s);
 // Launches the FP64 per-patch solve for the bin with R 16-row blocks.
 void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
                       hipStream_t s);
+// The same for one Wendland kind (explicitly instantiated per kind in
+// rbf_solve_c0.hip ... rbf_solve_c8.hip).
+template <int KIND>
+void launch_solve_kind(int R, const SolveArgs &a, const double *coef,
+                       hipStream_t s);
 
 struct BlendArgs {
   const int64_t *toff;      // M + 1
