@@ -7,8 +7,8 @@ contraction policy:
 * rbf_geometry.hip, rbf_plan.hip: ``-ffp-contract=off`` - distances,
   lattice coordinates and probe points must round exactly as written so the
   patch membership is bit-identical to the CPU restatement;
-* rbf_solve.hip and the per-kind solve units rbf_solve_c0.hip ...
-  rbf_solve_c8.hip: FMA contraction allowed (dense arithmetic).
+* rbf_solve.hip and the size-bin units rbf_solve_tile_a.hip ...
+  rbf_solve_tile_d.hip: FMA contraction allowed (dense arithmetic).
 
 All HIP code is compiled with the packed-FP32 instructions disabled.
 """
@@ -38,11 +38,10 @@ UNITS = {
     "rbf_plan.hip": ["-ffp-contract=off"],
     "capi.hip": ["-ffp-contract=off"],
     "rbf_solve.hip": [],
-    "rbf_solve_c0.hip": [],
-    "rbf_solve_c2.hip": [],
-    "rbf_solve_c4.hip": [],
-    "rbf_solve_c6.hip": [],
-    "rbf_solve_c8.hip": [],
+    "rbf_solve_tile_a.hip": [],
+    "rbf_solve_tile_b.hip": [],
+    "rbf_solve_tile_c.hip": [],
+    "rbf_solve_tile_d.hip": [],
 }
 
 
@@ -68,7 +67,7 @@ def build(verbose: bool = False) -> str:
     workers = max(1, min(len(UNITS), os.cpu_count() or 1))
     # The solve units take longest: start them first.
     order = sorted(UNITS.items(), key=lambda kv: not kv[0].startswith(
-        "rbf_solve_c"))
+        "rbf_solve_tile"))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), order))
     tmp = LIB + ".tmp"

@@ -149,13 +149,15 @@ struct SolveArgs {
 int solve_fields_per_launch();
 // Polynomial fit of every patch (one wave per patch): coef[K][F][4], pvOut.
 void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s);
-// Launches the FP64 per-patch solve for the bin with R 16-row blocks.
+// Size bin (16-row tile blocks R) of a patch of n sources: room for the
+// patch and the right-hand-side rows of one launch.
+int solve_bin_rows(int n);
+// Launches the FP64 per-patch solve for the patches of bin R.
 void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
                       hipStream_t s);
-// The same for one Wendland kind (explicitly instantiated per kind in
-// rbf_solve_c0.hip ... rbf_solve_c8.hip).
-template <int KIND>
-void launch_solve_kind(int R, const SolveArgs &a, const double *coef,
+// The same for one bin (explicitly instantiated in rbf_solve_tile_*.hip).
+template <int R>
+void launch_solve_tile(int kind, const SolveArgs &a, const double *coef,
                        hipStream_t s);
 
 struct BlendArgs {

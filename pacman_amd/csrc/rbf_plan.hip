@@ -15,7 +15,7 @@
 namespace pacman_amd {
 
 namespace {
-// Largest patch the register-resident kernel takes (16 x 12 rows).
+// Largest patch the register-resident solve takes.
 constexpr int kMaxPatch = 192;
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -24,12 +24,6 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
       .count();
 }
 
-int tile_for(int n) {
-  // Register tile R (rows per thread x 16) rounded up to an even count.
-  int R = (n + 15) / 16;
-  R += R & 1;
-  return std::max(R, 2);
-}
 } // namespace
 
 RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
@@ -155,13 +149,13 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
                                   hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   maxN_ = 0;
-  std::vector<std::vector<int>> bins(kMaxPatch / 16 + 2);
+  std::vector<std::vector<int>> bins(solve_bin_rows(kMaxPatch) + 1);
   for (int64_t kk = 0; kk < K_; ++kk) {
     const int n = (int)(off[kk + 1] - off[kk]);
     maxN_ = std::max(maxN_, n);
     if (n > kMaxPatch)
       continue;
-    bins[tile_for(n)].push_back((int)kk);
+    bins[solve_bin_rows(n)].push_back((int)kk);
   }
   if (maxN_ > kMaxPatch)
     throw std::runtime_error(

@@ -429,30 +429,32 @@ void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s) {
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
+int solve_bin_rows(int n) { return (n + kFMax + 15) / 16; }
+
 void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
                       hipStream_t s) {
   if (a.count <= 0)
     return;
   if (a.F < 1 || a.F > kFMax)
     throw std::runtime_error("solve: 1 <= fields <= 4 per launch");
-  switch (kind) {
-  case 0:
-    launch_solve_kind<0>(R, a, coef, s);
-    break;
-  case 1:
-    launch_solve_kind<1>(R, a, coef, s);
-    break;
-  case 2:
-    launch_solve_kind<2>(R, a, coef, s);
-    break;
-  case 3:
-    launch_solve_kind<3>(R, a, coef, s);
-    break;
-  case 4:
-    launch_solve_kind<4>(R, a, coef, s);
-    break;
-  default:
+  if (kind < 0 || kind > 4)
     throw std::runtime_error("solve: unknown Wendland kind");
+  switch (R) {
+  case 1: launch_solve_tile<1>(kind, a, coef, s); break;
+  case 2: launch_solve_tile<2>(kind, a, coef, s); break;
+  case 3: launch_solve_tile<3>(kind, a, coef, s); break;
+  case 4: launch_solve_tile<4>(kind, a, coef, s); break;
+  case 5: launch_solve_tile<5>(kind, a, coef, s); break;
+  case 6: launch_solve_tile<6>(kind, a, coef, s); break;
+  case 7: launch_solve_tile<7>(kind, a, coef, s); break;
+  case 8: launch_solve_tile<8>(kind, a, coef, s); break;
+  case 9: launch_solve_tile<9>(kind, a, coef, s); break;
+  case 10: launch_solve_tile<10>(kind, a, coef, s); break;
+  case 11: launch_solve_tile<11>(kind, a, coef, s); break;
+  case 12: launch_solve_tile<12>(kind, a, coef, s); break;
+  case 13: launch_solve_tile<13>(kind, a, coef, s); break;
+  default:
+    throw std::runtime_error("solve: unsupported size bin");
   }
 }
 

@@ -1,0 +1,540 @@
+// Blocked per-patch solve on 16 x 16 tiles with FP64 matrix-core updates.
+//
+// One workgroup of 8 waves per patch. The patch matrix A (n x n, n <= 16R)
+// is held in registers as its lower-triangle 16 x 16 tiles (I, J), I >= J,
+// in the accumulator layout of v_mfma_f64_16x16x4f64: lane l, register v
+// holds element (16 I + (l >> 4) + 4 v, 16 J + (l & 15)) of tile (I, J).
+// Waves 1..7 (the tile waves) hold the tiles, dealt in column-major order,
+// cyclically, so every wave keeps work while the trailing matrix shrinks.
+// Wave 0 (the control wave) holds none: it eliminates the diagonal blocks.
+//
+// The right-hand sides ride along as F extra rows n .. n+F-1 holding the
+// residual b - Q beta (the bordered matrix [[A, B^T], [B, I]]): the
+// factorisation then leaves L[n + f][j] = (D^-1 L^-1 b_f)_j in those rows,
+// exactly the right-hand side of the back substitution
+// L^T alpha = D^-1 L^-1 b. The extra rows never pivot, and all of them take
+// the same arithmetic wherever they sit.
+//
+// Factorisation: A = L D L^T by 16-column panels, the block form of the
+// reference's unpivoted Doolittle LU (src/rbf_pum/solver/utils/
+// serial_solve.hpp:76-84): the same pivots in the same order. The control
+// wave eliminates the diagonal block of panel p in registers (rows of the
+// tile exchanged through lane shuffles), applying the same row operations
+// to an identity tile, which yields Linv = L_pp^-1 and the pivots d. Per
+// panel, three barriers:
+//   A. the tile waves turn their panel tiles into X = A_Ip Linv^T = L_Ip D
+//      (four MFMAs each) and publish W = X D^-1 = L_Ip to LDS;
+//   B. the owner of the next diagonal tile applies the panel's update to it
+//      first and hands it to the control wave;
+//   C. the control wave eliminates that block while the tile waves update
+//      their trailing tiles, A_IJ -= (W_I D) W_J^T (four MFMAs each), and
+//      publish the next panel's tiles: the sequential elimination overlaps
+//      the trailing update.
+// Back substitution over 16-row blocks from the bottom: the tile waves add
+// the L_IB^T alpha_I partials of the blocks below, the control wave applies
+// alpha_B = Linv_B^T x_B as a matrix-vector product. Evaluation at the member
+// targets then follows src/rbf_pum/solver/solver.hpp:265-273.
+#pragma once
+
+#include "rbf_solve_common.hpp"
+
+namespace pacman_amd {
+namespace {
+
+using namespace solve_detail;
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Wendland kernel of runtime kind (the branch is uniform across the wave).
+__device__ __forceinline__ double phi_kind(int kind, double r, double dinv) {
+  switch (kind) {
+  case 0:
+    return wendland<0, double, true>(r, dinv);
+  case 1:
+    return wendland<1, double, true>(r, dinv);
+  case 2:
+    return wendland<2, double, true>(r, dinv);
+  case 3:
+    return wendland<3, double, true>(r, dinv);
+  default:
+    return wendland<4, double, true>(r, dinv);
+  }
+}
+
+// 1 / d: hardware reciprocal refined by two Newton steps (about 1 ulp).
+__device__ __forceinline__ double fast_rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+template <int R> struct TileShape {
+  static constexpr int NMAX = 16 * R;
+  static constexpr int NW = 8;              // waves per workgroup
+  static constexpr int NT = 64 * NW;
+  static constexpr int NWT = NW - 1;        // tile waves
+  static constexpr int NTILE = R * (R + 1) / 2;
+  static constexpr int TPW = (NTILE + NWT - 1) / NWT;
+  static constexpr int LDW = NMAX + 4;      // panel column stride (doubles)
+  static constexpr int PANEL = 2 * 16 * LDW;
+  static constexpr int EVAL = NW * kFMax * 64;
+  static constexpr int SBUF = PANEL > EVAL ? PANEL : EVAL;
+};
+
+template <int R>
+__global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
+                                                            const double *coef,
+                                                            int kind) {
+  using TS = TileShape<R>;
+  constexpr int NMAX = TS::NMAX;
+  constexpr int NT = TS::NT;
+  constexpr int NWT = TS::NWT;
+  constexpr int NTILE = TS::NTILE;
+  constexpr int TPW = TS::TPW;
+  constexpr int LDW = TS::LDW;
+
+  __shared__ double px[NMAX], py[NMAX], pz[NMAX];
+  __shared__ double rhs[kFMax][NMAX];
+  __shared__ double sbuf[TS::SBUF];          // panels, later eval partials
+  __shared__ double linv[2][16][17];         // Linv of the diagonal block
+  __shared__ double linvAll[R][120];         // strictly-lower Linv, packed
+  __shared__ double dtile[16][17];           // diagonal tile hand-over
+  __shared__ double dpan[2][16], dinvp[2][16];
+  __shared__ double spart[R][kFMax][16];
+  __shared__ double scoef[kFMax][4];
+  __shared__ int spivot;
+  double(*wp)[16][LDW] = reinterpret_cast<double(*)[16][LDW]>(sbuf);
+
+  const int patch = a.patches[xcd_remap(blockIdx.x, a.count)];
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int F = a.F;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane >> 4, lc = lane & 15;
+  const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
+  const double dinv = a.dinv;
+
+  if (tid < kFMax * 4)
+    scoef[tid >> 2][tid & 3] =
+        (tid >> 2) < F ? coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)]
+                       : 0.0;
+  if (tid == 0)
+    spivot = 0;
+  __syncthreads();
+
+  // gather; right-hand side = value - polynomial fit
+  for (int t = tid; t < NMAX; t += NT) {
+    double x = 0.0, y = 0.0, z = 0.0;
+    int p = 0;
+    if (t < n) {
+      p = a.srcIdx[s0 + t];
+      x = a.sx[p];
+      y = a.sy[p];
+      z = a.sz[p];
+    }
+    px[t] = x;
+    py[t] = y;
+    pz[t] = z;
+    for (int f = 0; f < F; ++f) {
+      double v = 0.0;
+      if (t < n)
+        v = a.vals[(int64_t)f * a.N + p] -
+            (scoef[f][0] + scoef[f][1] * (x - cx) + scoef[f][2] * (y - cy) +
+             scoef[f][3] * (z - cz));
+      rhs[f][t] = v;
+    }
+  }
+  __syncthreads();
+
+  const int NB = (n + F + 15) >> 4;  // row blocks holding data
+  const int P = (n + 15) >> 4;       // pivot panels
+
+  // Tile of the bordered, padded matrix: A among the patch rows, the
+  // residual rows below them, identity on the padding.
+  auto assemble = [&](int I, int J) {
+    dbl4 t4;
+    const int j = 16 * J + lc;
+    const double xj = px[j], yj = py[j], zj = pz[j];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * I + lr + 4 * v;
+      double val;
+      if (i < n && j < n)
+        val = phi_kind(kind, dist_fast(px[i], py[i], pz[i], xj, yj, zj), dinv);
+      else if (i >= n && i < n + F && j < n)
+        val = rhs[i - n][j];
+      else
+        val = (i == j) ? 1.0 : 0.0;
+      t4[v] = val;
+    }
+    return t4;
+  };
+
+  // t - (W_I D) W_J^T for the panel in buffer `buf` (four 16x16x4 MFMAs;
+  // operand lane (i + 16 q) holds column 4 s + q of row i).
+  auto schur = [&](dbl4 t4, int I, int J, int buf) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * s + lr;
+      const double av = -wp[buf][k][16 * I + lc] * dpan[buf][k];
+      const double bv = wp[buf][k][16 * J + lc];
+      t4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, t4, 0, 0, 0);
+    }
+    return t4;
+  };
+
+  // W = (A_rows Linv^T) D^-1 for 16 raw rows stored column-major at
+  // base[k * stride + i]; zero past the pivots.
+  auto to_w = [&](const double *base, int stride, int buf, int kk) {
+    dbl4 t4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * s + lr;
+      t4 = __builtin_amdgcn_mfma_f64_16x16x4f64(base[k * stride + lc],
+                                                linv[buf][lc][k], t4, 0, 0, 0);
+    }
+    const double sc = lc < kk ? dinvp[buf][lc] : 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      t4[v] *= sc;
+    return t4;
+  };
+
+  if (wave == 0) {
+    // ---------------- control wave: eliminations -------------------------
+    // Elimination of the diagonal block in dtile (rows < kk pivot) applied
+    // to the tile and to an identity; writes Linv, d and 1/d to LDS buffer
+    // `buf` and the packed Linv of block p.
+    auto factor = [&](int kk, int buf, int p) {
+      dbl4 T, Y;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        T[v] = dtile[lr + 4 * v][lc];
+        Y[v] = (lr + 4 * v == lc) ? 1.0 : 0.0;
+      }
+      static_for<0, 16>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        constexpr int kr = k & 3, kv = k >> 2;
+        if (k < kk) {
+          double colk[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            colk[v] = __shfl(T[v], 16 * lr + k, kWave);           // A(r, k)
+          const double rowk = __shfl(T[kv], 16 * kr + lc, kWave);  // A(k, c)
+          const double yrow = __shfl(Y[kv], 16 * kr + lc, kWave);  // Y(k, c)
+          const double d = readlane_f64(T[kv], 16 * kr + k);
+          const double rd = fast_rcp(d);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = lr + 4 * v;
+            if (r > k && r < kk) {
+              const double l = colk[v] * rd;
+              if (lc > k)
+                T[v] = fma(-l, rowk, T[v]);
+              Y[v] = fma(-l, yrow, Y[v]);
+            }
+          }
+          if (lane == 0) {
+            dpan[buf][k] = d;
+            dinvp[buf][k] = rd;
+            if (d == 0.0)
+              spivot = 1;
+          }
+        } else if (lane == 0) {
+          dpan[buf][k] = 1.0;
+          dinvp[buf][k] = 1.0;
+        }
+      });
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = lr + 4 * v;
+        linv[buf][r][lc] = Y[v];
+        if (r > lc)
+          linvAll[p][r * (r - 1) / 2 + lc] = Y[v];
+      }
+    };
+
+    __syncthreads();  // first diagonal tile handed over
+    if (P > 0)
+      factor(min(16, n), 0, 0);
+    for (int p = 0; p < P; ++p) {
+      __syncthreads();  // A
+      __syncthreads();  // B
+      __syncthreads();  // C
+      if (p + 1 < P)
+        factor(min(16, n - 16 * (p + 1)), (p + 1) & 1, p + 1);
+    }
+    __syncthreads();  // E
+    __syncthreads();  // F
+    // back substitution: alpha_B = Linv_B^T (x_B - sum_I L_IB^T alpha_I)
+    for (int B = P - 1; B >= 0; --B) {
+      __syncthreads();  // partials ready
+      for (int f = 0; f < F; ++f) {
+        double part = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = lr + 4 * v;
+          if (16 * B + r < n && r >= lc) {
+            double x = rhs[f][16 * B + r];
+            for (int I = B + 1; I < P; ++I)
+              x -= spart[I][f][r];
+            part = (r == lc) ? part + x
+                             : fma(linvAll[B][r * (r - 1) / 2 + lc], x, part);
+          }
+        }
+        part += __shfl_xor(part, 16, kWave);
+        part += __shfl_xor(part, 32, kWave);
+        if (lr == 0 && 16 * B + lc < n)
+          rhs[f][16 * B + lc] = part;
+      }
+      __syncthreads();  // alpha_B ready
+    }
+  } else {
+    // ---------------- tile waves: the lower-triangle tiles ---------------
+    const int tw = wave - 1;
+    int tI[TPW], tJ[TPW];
+    static_for<0, TPW>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      const int t = s * NWT + tw;
+      int J = 0, off = 0;
+      while (J < R && t >= off + (R - J)) {
+        off += R - J;
+        ++J;
+      }
+      const bool ok = t < NTILE;
+      tI[s] = ok ? J + (t - off) : R;
+      tJ[s] = ok ? J : R;
+    });
+
+    dbl4 acc[TPW];
+    static_for<0, TPW>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      const int I = tI[s], J = tJ[s];
+      acc[s] = (I < NB && J < P) ? assemble(I, J)
+                                 : dbl4{0.0, 0.0, 0.0, 0.0};
+      // hand over the first diagonal tile, publish the first panel (raw)
+      if (J == 0 && I == 0 && P > 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          dtile[lr + 4 * v][lc] = acc[s][v];
+      } else if (J == 0 && I < NB) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          wp[0][lc][16 * I + lr + 4 * v] = acc[s][v];
+      }
+    });
+    __syncthreads();  // first diagonal tile handed over
+
+    for (int p = 0; p < P; ++p) {
+      const int buf = p & 1;
+      const int kk = min(16, n - 16 * p);
+      __syncthreads();  // A
+      // panel tiles: W = (A_Ip Linv^T) D^-1, kept and published
+      static_for<0, TPW>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int I = tI[s], J = tJ[s];
+        if (J == p && I > p && I < NB) {
+          const dbl4 w4 = to_w(&wp[buf][0][16 * I], LDW, buf, kk);
+          acc[s] = w4;
+          // every lane has read its raw values before any lane writes
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            wp[buf][lc][16 * I + lr + 4 * v] = w4[v];
+        } else if (J == p && I == p && kk < 16 && 16 * p + kk < n + F) {
+          // bordered rows inside the last diagonal block
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            dtile[lr + 4 * v][lc] = acc[s][v];
+          __builtin_amdgcn_wave_barrier();
+          dbl4 t4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int k = 4 * q + lr;
+            t4 = __builtin_amdgcn_mfma_f64_16x16x4f64(dtile[lc][k],
+                                                      linv[buf][lc][k], t4, 0,
+                                                      0, 0);
+          }
+          const double sc2 = lc < kk ? dinvp[buf][lc] : 0.0;
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (lr + 4 * v >= kk)
+              acc[s][v] = t4[v] * sc2;
+        }
+      });
+      __syncthreads();  // B
+      // the next diagonal tile first: update it and hand it over
+      static_for<0, TPW>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int I = tI[s], J = tJ[s];
+        if (I == p + 1 && J == p + 1 && J < P) {
+          acc[s] = schur(acc[s], I, J, buf);
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            dtile[lr + 4 * v][lc] = acc[s][v];
+        }
+      });
+      __syncthreads();  // C
+      // trailing update; publish the next panel's tiles
+      static_for<0, TPW>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int I = tI[s], J = tJ[s];
+        if (I < NB && J > p && J < P && !(I == p + 1 && J == p + 1)) {
+          acc[s] = schur(acc[s], I, J, buf);
+          if (J == p + 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              wp[buf ^ 1][lc][16 * I + lr + 4 * v] = acc[s][v];
+          }
+        }
+      });
+    }
+    __syncthreads();  // E
+    // bordered rows -> rhs
+    static_for<0, TPW>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      const int I = tI[s], J = tJ[s];
+      if (I < NB && J < P) {
+        const int c = 16 * J + lc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = 16 * I + lr + 4 * v;
+          if (r >= n && r < n + F && c < n)
+            rhs[r - n][c] = acc[s][v];
+        }
+      }
+    });
+    __syncthreads();  // F
+    for (int B = P - 1; B >= 0; --B) {
+      static_for<0, TPW>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const int I = tI[s], J = tJ[s];
+        if (J == B && I > B && I < P) {
+          for (int f = 0; f < F; ++f) {
+            double part = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int r = 16 * I + lr + 4 * v;
+              if (r < n)
+                part = fma(acc[s][v], rhs[f][r], part);
+            }
+            part += __shfl_xor(part, 16, kWave);
+            part += __shfl_xor(part, 32, kWave);
+            if (lr == 0)
+              spart[I][f][lc] = part;
+          }
+        }
+      });
+      __syncthreads();  // partials ready
+      __syncthreads();  // alpha_B ready
+    }
+  }
+
+  if (tid == 0 && a.status)
+    a.status[patch] = spivot;
+
+  // evaluation at the member targets: tiles of 64 pairs, thread = (pair,
+  // slice); slices of ceil(n / 8) sources, two at a time into two partial
+  // sums, added in a fixed order. The next tile's targets are loaded while
+  // the current tile is summed.
+  {
+    double(*epart)[kFMax][64] =
+        reinterpret_cast<double(*)[kFMax][64]>(sbuf);
+    constexpr int TP = 64;
+    constexpr int ES = NT / TP;
+    const int64_t p0 = a.pairOff[patch], p1 = a.pairOff[patch + 1];
+    const int mall = (int)(p1 - p0);
+    const int sl = tid / TP;
+    const int qn = (n + ES - 1) / ES;
+    const int l0 = min(n, sl * qn), l1 = min(n, l0 + qn);
+    int e = 0;
+    double yx = 0.0, yy = 0.0, yz = 0.0;
+    if (lane < mall) {
+      e = a.pairTgt[p0 + lane];
+      const int tp = e & 0x7fffffff;
+      yx = a.tx[tp];
+      yy = a.ty[tp];
+      yz = a.tz[tp];
+    }
+    for (int tile = 0; tile < mall; tile += TP) {
+      const int pi = tile + lane;
+      const bool mem = pi < mall && ((unsigned)e >> 31) != 0;
+      const double qx = yx, qy = yy, qz = yz;
+      const int pn = pi + TP;
+      if (pn < mall) {
+        e = a.pairTgt[p0 + pn];
+        const int tp = e & 0x7fffffff;
+        yx = a.tx[tp];
+        yy = a.ty[tp];
+        yz = a.tz[tp];
+      }
+      double acc0[kFMax], acc1[kFMax];
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f) {
+        acc0[f] = 0.0;
+        acc1[f] = 0.0;
+      }
+      if (mem) {
+        int l = l0;
+        for (; l + 1 < l1; l += 2) {
+          const double phi0 = phi_kind(
+              kind, dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
+          const double phi1 = phi_kind(
+              kind, dist_fast(qx, qy, qz, px[l + 1], py[l + 1], pz[l + 1]),
+              dinv);
+#pragma unroll
+          for (int f = 0; f < kFMax; ++f)
+            if (f < F) {
+              acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
+              acc1[f] = fma(phi1, rhs[f][l + 1], acc1[f]);
+            }
+        }
+        if (l < l1) {
+          const double phi0 = phi_kind(
+              kind, dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
+#pragma unroll
+          for (int f = 0; f < kFMax; ++f)
+            if (f < F)
+              acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f)
+        if (f < F)
+          epart[sl][f][lane] = acc0[f] + acc1[f];
+      __syncthreads();
+      if (sl == 0 && mem) {
+        const int slot = a.pairSlot[p0 + pi];
+#pragma unroll
+        for (int f = 0; f < kFMax; ++f)
+          if (f < F) {
+            double t = epart[0][f][lane];
+#pragma unroll
+            for (int q = 1; q < ES; ++q)
+              t += epart[q][f][lane];
+            // V beta (solver.hpp:272), around the centre
+            const double vb = scoef[f][0] + scoef[f][1] * (qx - cx) +
+                              scoef[f][2] * (qy - cy) + scoef[f][3] * (qz - cz);
+            a.contrib[(int64_t)slot * F + f] = t + vb;
+          }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+} // namespace
+
+template <int R>
+void launch_solve_tile(int kind, const SolveArgs &a, const double *coef,
+                       hipStream_t s) {
+  solve_tile_kernel<R><<<dim3((unsigned)a.count), dim3(TileShape<R>::NT), 0,
+                         s>>>(a, coef, kind);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace pacman_amd

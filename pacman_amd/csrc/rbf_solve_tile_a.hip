@@ -1,0 +1,17 @@
+// Blocked FP64 per-patch solve, size bins R = 1 .. 6.
+#include "rbf_solve_tile.hpp"
+
+namespace pacman_amd {
+template void launch_solve_tile<1>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+template void launch_solve_tile<2>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+template void launch_solve_tile<3>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+template void launch_solve_tile<4>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+template void launch_solve_tile<5>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+template void launch_solve_tile<6>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+} // namespace pacman_amd

@@ -1,0 +1,9 @@
+// Blocked FP64 per-patch solve, size bins R = 10 .. 11.
+#include "rbf_solve_tile.hpp"
+
+namespace pacman_amd {
+template void launch_solve_tile<10>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+template void launch_solve_tile<11>(int, const SolveArgs &, const double *,
+                                     hipStream_t);
+} // namespace pacman_amd
