@@ -46,6 +46,13 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
+// Value of v in lane K of each 16-lane row, for every lane of that row
+// (DPP row_newbcast).
+template <int K> __device__ __forceinline__ double row_bcast(double v) {
+  static_assert(K >= 0 && K < 16, "row lane");
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
+}
+
 __device__ inline double dist_fast(double ax, double ay, double az, double bx,
                                    double by, double bz) {
   const double dx = bx - ax, dy = by - ay, dz = bz - az;

@@ -18,9 +18,10 @@
 // Factorisation: A = L D L^T by 16-column panels, the block form of the
 // reference's unpivoted Doolittle LU (src/rbf_pum/solver/utils/
 // serial_solve.hpp:76-84): the same pivots in the same order. The control
-// wave eliminates the diagonal block of panel p in registers (rows of the
-// tile exchanged through lane shuffles), applying the same row operations
-// to an identity tile, which yields Linv = L_pp^-1 and the pivots d. Per
+// wave eliminates the diagonal block of panel p in registers (one row per
+// lane, pivot rows broadcast by DPP), applying the same
+// row operations to an identity, which yields Linv = L_pp^-1 and the
+// pivots d. Per
 // panel, three barriers:
 //   A. the tile waves turn their panel tiles into X = A_Ip Linv^T = L_Ip D
 //      (four MFMAs each) and publish W = X D^-1 = L_Ip to LDS;
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 
   // Tile of the bordered, padded matrix: A among the patch rows, the
   // residual rows below them, identity on the padding.
-  auto assemble = [&](int I, int J) {
+  auto assemble = [&](int I, int J) __attribute__((always_inline)) {
     dbl4 t4;
     const int j = 16 * J + lc;
     const double xj = px[j], yj = py[j], zj = pz[j];
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 
   // t - (W_I D) W_J^T for the panel in buffer `buf` (four 16x16x4 MFMAs;
   // operand lane (i + 16 q) holds column 4 s + q of row i).
-  auto schur = [&](dbl4 t4, int I, int J, int buf) {
+  auto schur = [&](dbl4 t4, int I, int J, int buf) __attribute__((always_inline)) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * s + lr;
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 
   // W = (A_rows Linv^T) D^-1 for 16 raw rows stored column-major at
   // base[k * stride + i]; zero past the pivots.
-  auto to_w = [&](const double *base, int stride, int buf, int kk) {
+  auto to_w = [&](const double *base, int stride, int buf, int kk) __attribute__((always_inline)) {
     dbl4 t4 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -206,58 +207,57 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 
   if (wave == 0) {
     // ---------------- control wave: eliminations -------------------------
-    // Elimination of the diagonal block in dtile (rows < kk pivot) applied
-    // to the tile and to an identity; writes Linv, d and 1/d to LDS buffer
-    // `buf` and the packed Linv of block p.
-    auto factor = [&](int kk, int buf, int p) {
-      dbl4 T, Y;
+    // Elimination of the diagonal block in dtile (rows < kk pivot): lane i
+    // (mod 16) holds row i of the block and row i of an identity; the pivot
+    // row is broadcast along each 16-lane row (DPP row_newbcast). The
+    // identity turns into Linv. Writes Linv, d and 1/d to LDS buffer `buf`
+    // and the packed Linv of block p.
+    auto factor = [&](int kk, int buf, int p) __attribute__((always_inline)) {
+      const int i = lane & 15;
+      double row[16], yr[16];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        T[v] = dtile[lr + 4 * v][lc];
-        Y[v] = (lr + 4 * v == lc) ? 1.0 : 0.0;
+      for (int c = 0; c < 16; ++c) {
+        row[c] = dtile[i][c];
+        yr[c] = (c == i) ? 1.0 : 0.0;
       }
-      static_for<0, 16>([&](auto kc) {
+      double myd = 1.0, myrd = 1.0;
+      static_for<0, 16>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
-        constexpr int kr = k & 3, kv = k >> 2;
         if (k < kk) {
-          double colk[4];
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            colk[v] = __shfl(T[v], 16 * lr + k, kWave);           // A(r, k)
-          const double rowk = __shfl(T[kv], 16 * kr + lc, kWave);  // A(k, c)
-          const double yrow = __shfl(Y[kv], 16 * kr + lc, kWave);  // Y(k, c)
-          const double d = readlane_f64(T[kv], 16 * kr + k);
+          const double d = row_bcast<k>(row[k]);
           const double rd = fast_rcp(d);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int r = lr + 4 * v;
-            if (r > k && r < kk) {
-              const double l = colk[v] * rd;
-              if (lc > k)
-                T[v] = fma(-l, rowk, T[v]);
-              Y[v] = fma(-l, yrow, Y[v]);
-            }
+          if (i == k) {
+            myd = d;
+            myrd = rd;
           }
-          if (lane == 0) {
-            dpan[buf][k] = d;
-            dinvp[buf][k] = rd;
-            if (d == 0.0)
-              spivot = 1;
-          }
-        } else if (lane == 0) {
-          dpan[buf][k] = 1.0;
-          dinvp[buf][k] = 1.0;
+          const double l = (i > k && i < kk) ? row[k] * rd : 0.0;
+          static_for<k + 1, 16>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(cc)::value;
+            row[c] = fma(-l, row_bcast<k>(row[c]), row[c]);
+          });
+          static_for<0, k + 1>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int c = decltype(cc)::value;
+            yr[c] = fma(-l, row_bcast<k>(yr[c]), yr[c]);
+          });
         }
       });
+      if (lane < 16) {
+        dpan[buf][i] = myd;
+        dinvp[buf][i] = myrd;
+        if (i < kk && myd == 0.0)
+          spivot = 1;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = lr + 4 * v;
-        linv[buf][r][lc] = Y[v];
-        if (r > lc)
-          linvAll[p][r * (r - 1) / 2 + lc] = Y[v];
+        for (int c = 0; c < 16; ++c) {
+          linv[buf][i][c] = yr[c];
+          if (c < i)
+            linvAll[p][i * (i - 1) / 2 + c] = yr[c];
+        }
       }
     };
 
+    // The eliminations are the critical path of the factorisation: let this
+    // wave win the instruction arbitration against the tile waves.
+    __builtin_amdgcn_s_setprio(3);
     __syncthreads();  // first diagonal tile handed over
     if (P > 0)
       factor(min(16, n), 0, 0);
@@ -268,6 +268,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
       if (p + 1 < P)
         factor(min(16, n - 16 * (p + 1)), (p + 1) & 1, p + 1);
     }
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();  // E
     __syncthreads();  // F
     // back substitution: alpha_B = Linv_B^T (x_B - sum_I L_IB^T alpha_I)
