@@ -119,7 +119,7 @@ def main() -> int:
     kern_ms = ph["solve"] - ph["poly"]
     achieved = f_alg / (kern_ms / 1e3) / 1e12
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"solve_f64_{args.config}_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", f"solve_tile_{args.config}_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_apply")
@@ -171,7 +171,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "solve_f64_kernel (all size-bin launches of one "
+            "kernel": "solve_tile_kernel (all size-bin launches of one "
                       "apply)",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
