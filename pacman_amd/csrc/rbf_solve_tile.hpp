@@ -31,10 +31,12 @@
 //      their trailing tiles, A_IJ -= (W_I D) W_J^T (four MFMAs each), and
 //      publish the next panel's tiles: the sequential elimination overlaps
 //      the trailing update.
-// Back substitution over 16-row blocks from the bottom: the tile waves add
-// the L_IB^T alpha_I partials of the blocks below, the control wave applies
-// alpha_B = Linv_B^T x_B as a matrix-vector product. Evaluation at the member
-// targets then follows src/rbf_pum/solver/solver.hpp:265-273.
+// Back substitution over 16-row blocks from the bottom, one barrier per
+// block: the control wave computes alpha_B = Linv_B^T x_B as matrix-vector
+// products (with the sub-diagonal tile staged in LDS), the tile waves add
+// L_IJ^T alpha_I to the sums of the blocks further up as soon as alpha_I is
+// known. Evaluation at the member targets then follows
+// src/rbf_pum/solver/solver.hpp:265-273.
 #pragma once
 
 #include "rbf_solve_common.hpp"
@@ -45,22 +47,6 @@ namespace {
 using namespace solve_detail;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-// Wendland kernel of runtime kind (the branch is uniform across the wave).
-__device__ __forceinline__ double phi_kind(int kind, double r, double dinv) {
-  switch (kind) {
-  case 0:
-    return wendland<0, double, true>(r, dinv);
-  case 1:
-    return wendland<1, double, true>(r, dinv);
-  case 2:
-    return wendland<2, double, true>(r, dinv);
-  case 3:
-    return wendland<3, double, true>(r, dinv);
-  default:
-    return wendland<4, double, true>(r, dinv);
-  }
-}
 
 // 1 / d: hardware reciprocal refined by two Newton steps (about 1 ulp).
 __device__ __forceinline__ double fast_rcp(double d) {
@@ -103,7 +89,8 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   __shared__ double linvAll[R][120];         // strictly-lower Linv, packed
   __shared__ double dtile[16][17];           // diagonal tile hand-over
   __shared__ double dpan[2][16], dinvp[2][16];
-  __shared__ double spart[R][kFMax][16];
+  __shared__ double sacc[R][kFMax][16];      // back-substitution sums
+  __shared__ double xs[16];
   __shared__ double scoef[kFMax][4];
   __shared__ int spivot;
   double(*wp)[16][LDW] = reinterpret_cast<double(*)[16][LDW]>(sbuf);
@@ -154,9 +141,32 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   const int NB = (n + F + 15) >> 4;  // row blocks holding data
   const int P = (n + 15) >> 4;       // pivot panels
 
+  // fn(integral_constant<int, kind>): one code path per Wendland kind, so
+  // the kernel evaluations inside the loops carry no branch.
+  auto with_kind = [&](auto &&fn) __attribute__((always_inline)) {
+    switch (kind) {
+    case 0:
+      fn(std::integral_constant<int, 0>{});
+      break;
+    case 1:
+      fn(std::integral_constant<int, 1>{});
+      break;
+    case 2:
+      fn(std::integral_constant<int, 2>{});
+      break;
+    case 3:
+      fn(std::integral_constant<int, 3>{});
+      break;
+    default:
+      fn(std::integral_constant<int, 4>{});
+      break;
+    }
+  };
+
   // Tile of the bordered, padded matrix: A among the patch rows, the
   // residual rows below them, identity on the padding.
-  auto assemble = [&](int I, int J) __attribute__((always_inline)) {
+  auto assemble = [&](auto kc, int I, int J) __attribute__((always_inline)) {
+    constexpr int K = decltype(kc)::value;
     dbl4 t4;
     const int j = 16 * J + lc;
     const double xj = px[j], yj = py[j], zj = pz[j];
@@ -165,7 +175,8 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
       const int i = 16 * I + lr + 4 * v;
       double val;
       if (i < n && j < n)
-        val = phi_kind(kind, dist_fast(px[i], py[i], pz[i], xj, yj, zj), dinv);
+        val = wendland<K, double, true>(
+            dist_fast(px[i], py[i], pz[i], xj, yj, zj), dinv);
       else if (i >= n && i < n + F && j < n)
         val = rhs[i - n][j];
       else
@@ -271,26 +282,45 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     __builtin_amdgcn_s_setprio(0);
     __syncthreads();  // E
     __syncthreads();  // F
-    // back substitution: alpha_B = Linv_B^T (x_B - sum_I L_IB^T alpha_I)
+    // Back substitution L^T alpha = D^-1 L^-1 b, one barrier per block:
+    // alpha_B = Linv_B^T (x_B - S_B - L_{B+1,B}^T alpha_{B+1}), where S_B holds
+    // the contributions of the blocks below B + 1 (added by the tile waves as
+    // soon as their alpha was known) and the sub-diagonal tile is staged in
+    // LDS, so the chain from one block to the next stays in this wave.
+    double(*sub)[16][17] = reinterpret_cast<double(*)[16][17]>(sbuf);
     for (int B = P - 1; B >= 0; --B) {
-      __syncthreads();  // partials ready
+      const bool valid = 16 * B + lc < n;
       for (int f = 0; f < F; ++f) {
+        double y = 0.0;
+        if (B + 1 < P) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r2 = lr + 4 * v;
+            if (16 * (B + 1) + r2 < n)
+              y = fma(sub[B][r2][lc], rhs[f][16 * (B + 1) + r2], y);
+          }
+          y += __shfl_xor(y, 16, kWave);
+          y += __shfl_xor(y, 32, kWave);
+        }
+        const double x = valid ? rhs[f][16 * B + lc] - sacc[B][f][lc] - y : 0.0;
+        if (lr == 0)
+          xs[lc] = x;
+        __builtin_amdgcn_wave_barrier();
         double part = 0.0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = lr + 4 * v;
-          if (16 * B + r < n && r >= lc) {
-            double x = rhs[f][16 * B + r];
-            for (int I = B + 1; I < P; ++I)
-              x -= spart[I][f][r];
-            part = (r == lc) ? part + x
-                             : fma(linvAll[B][r * (r - 1) / 2 + lc], x, part);
+          if (r >= lc) {
+            const double xr = xs[r];
+            part = (r == lc) ? part + xr
+                             : fma(linvAll[B][r * (r - 1) / 2 + lc], xr, part);
           }
         }
         part += __shfl_xor(part, 16, kWave);
         part += __shfl_xor(part, 32, kWave);
-        if (lr == 0 && 16 * B + lc < n)
+        if (lr == 0 && valid)
           rhs[f][16 * B + lc] = part;
+        __builtin_amdgcn_wave_barrier();
       }
       __syncthreads();  // alpha_B ready
     }
@@ -312,10 +342,11 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     });
 
     dbl4 acc[TPW];
+    with_kind([&](auto kc) __attribute__((always_inline)) {
     static_for<0, TPW>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
       const int I = tI[s], J = tJ[s];
-      acc[s] = (I < NB && J < P) ? assemble(I, J)
+      acc[s] = (I < NB && J < P) ? assemble(kc, I, J)
                                  : dbl4{0.0, 0.0, 0.0, 0.0};
       // hand over the first diagonal tile, publish the first panel (raw)
       if (J == 0 && I == 0 && P > 0) {
@@ -327,6 +358,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
         for (int v = 0; v < 4; ++v)
           wp[0][lc][16 * I + lr + 4 * v] = acc[s][v];
       }
+    });
     });
     __syncthreads();  // first diagonal tile handed over
 
@@ -409,12 +441,26 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
         }
       }
     });
+    // stage the sub-diagonal tiles for the control wave
+    double(*sub)[16][17] = reinterpret_cast<double(*)[16][17]>(sbuf);
+    static_for<0, TPW>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      const int I = tI[s], J = tJ[s];
+      if (I == J + 1 && I < P) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          sub[J][lr + 4 * v][lc] = acc[s][v];
+      }
+    });
+    for (int t = tid - 64; t < R * kFMax * 16; t += NT - 64)
+      (&sacc[0][0][0])[t] = 0.0;
     __syncthreads();  // F
     for (int B = P - 1; B >= 0; --B) {
+      // alpha_{B+1} is known: add L_{B+1,J}^T alpha_{B+1} to S_J, J < B
       static_for<0, TPW>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         const int I = tI[s], J = tJ[s];
-        if (J == B && I > B && I < P) {
+        if (I == B + 1 && J < B && I < P) {
           for (int f = 0; f < F; ++f) {
             double part = 0.0;
 #pragma unroll
@@ -426,11 +472,10 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
             part += __shfl_xor(part, 16, kWave);
             part += __shfl_xor(part, 32, kWave);
             if (lr == 0)
-              spart[I][f][lc] = part;
+              sacc[J][f][lc] += part;
           }
         }
       });
-      __syncthreads();  // partials ready
       __syncthreads();  // alpha_B ready
     }
   }
@@ -480,28 +525,30 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
         acc1[f] = 0.0;
       }
       if (mem) {
-        int l = l0;
-        for (; l + 1 < l1; l += 2) {
-          const double phi0 = phi_kind(
-              kind, dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
-          const double phi1 = phi_kind(
-              kind, dist_fast(qx, qy, qz, px[l + 1], py[l + 1], pz[l + 1]),
-              dinv);
+        with_kind([&](auto kc) __attribute__((always_inline)) {
+          constexpr int K = decltype(kc)::value;
+          int l = l0;
+          for (; l + 1 < l1; l += 2) {
+            const double phi0 = wendland<K, double, true>(
+                dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
+            const double phi1 = wendland<K, double, true>(
+                dist_fast(qx, qy, qz, px[l + 1], py[l + 1], pz[l + 1]), dinv);
 #pragma unroll
-          for (int f = 0; f < kFMax; ++f)
-            if (f < F) {
-              acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
-              acc1[f] = fma(phi1, rhs[f][l + 1], acc1[f]);
-            }
-        }
-        if (l < l1) {
-          const double phi0 = phi_kind(
-              kind, dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
+            for (int f = 0; f < kFMax; ++f)
+              if (f < F) {
+                acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
+                acc1[f] = fma(phi1, rhs[f][l + 1], acc1[f]);
+              }
+          }
+          if (l < l1) {
+            const double phi0 = wendland<K, double, true>(
+                dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
 #pragma unroll
-          for (int f = 0; f < kFMax; ++f)
-            if (f < F)
-              acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
-        }
+            for (int f = 0; f < kFMax; ++f)
+              if (f < F)
+                acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
+          }
+        });
       }
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
