@@ -42,6 +42,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-edge", type=float, default=0.5,
                     help="CPU baseline sample: sub-cube [0, edge)^3")
+    ap.add_argument("--mode", default="rbf", choices=["rbf", "fe-nearest"],
+                    help="rbf: RBF-PUM setup+apply (default); fe-nearest: "
+                         "the nearest-source transfer on the same clouds")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB "
                          "(calibrates HBM byte counters) and exit")
@@ -71,6 +74,8 @@ def main() -> int:
 
     log(f"[bench] generating {args.config}")
     S, T, npc, support = W.config(args.config)
+    if args.mode == "fe-nearest":
+        return bench_fe_nearest(args, S, T)
     v = W.franke_3d(*S.T)
     N, M = len(S), len(T)
     dev = torch.device("cuda:0")
@@ -183,6 +188,67 @@ def main() -> int:
             "executed_frac": f_exec / (kern_ms / 1e3) / 1e12 /
             PEAK_FP64_TFLOPS,
         },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_fe_nearest(args, S, T) -> int:
+    """Nearest-source transfer (FE NEAREST_NEAREST), inputs in HBM.
+
+    A step = grid build over the sources + one nearest query per target.
+    Compulsory HBM bytes: sources 24 B coordinates + 8 B value read, sorted
+    copy written (24 B + 4 B index); targets 24 B read, 8 B value + 4 B
+    status written.
+    """
+    import numpy as np
+    import torch
+
+    import pacman_amd as pacman
+    from pacman_amd import workloads as W
+
+    v = W.franke_3d(*S.T)
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(S).to(dev)
+    dT = torch.from_numpy(T).to(dev)
+    dv = torch.from_numpy(v).to(dev)
+    N, M = len(S), len(T)
+    for _ in range(args.warmup):
+        out, st = pacman.fe.nearest_device(3, dS, dv, dT)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st = pacman.fe.nearest_device(3, dS, dv, dT)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    byts = N * (24 + 8 + 24 + 4) + M * (24 + 8 + 4)
+    gbs = byts / (ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        cores = min(16, os.cpu_count() or 1)
+        oracle.set_threads(cores)
+        c0 = time.perf_counter()
+        oracle.fe_nearest(3, S, v, T)
+        c1 = time.perf_counter()
+        cpu = {"value": M / (c1 - c0) / 1e6, "unit": "Mpts/s",
+               "cores": cores, "kind": "port",
+               "sample": f"the full workload ({N} -> {M}), FP64 oracle grid "
+                         f"search, {c1 - c0:.2f} s"}
+    res = {
+        "metric": "FE NEAREST_NEAREST transfer Mpts/s",
+        "value": M / (ms / 1e3) / 1e6, "unit": "Mpts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config} clouds: {N} jittered-grid "
+                               f"sources -> {M} uniform targets, nearest "
+                               "source value"},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                     "traffic": None, "kernel": "grid build + "
+                     "nearest_value_kernel, compulsory bytes"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -4,8 +4,9 @@
  * Same symbols, argument order, constants and return codes as the
  * reference's plain-C shim (src/fortran_interface.h), so the reference's
  * Fortran module (src/pacman_fortran.f90:97-113, 215-235) and any C caller
- * bind to this library unchanged. Only the RBF-PUM entry points exist here;
- * the FE and MLS entry points of that header belong to other paths.
+ * bind to this library unchanged. Of the FE entry point only the method
+ * NEAREST_NEAREST is served (it shares the RBF path's point search); the
+ * other FE methods and MLS return an error code.
  */
 #ifndef PACMAN_AMD_FORTRAN_INTERFACE_H
 #define PACMAN_AMD_FORTRAN_INTERFACE_H
@@ -32,6 +33,17 @@ enum PACMAN_RbfFunction {
   PACMAN_WENDLANDC4 = 0x12,
   PACMAN_WENDLANDC6 = 0x13,
   PACMAN_WENDLANDC8 = 0x14
+};
+
+/* FE method selectors (src/fortran_interface.h:51-59). This build serves
+ * PACMAN_FE_NEAREST_NEAREST; the other selectors return an error code. */
+enum PACMAN_FeMethod {
+  PACMAN_FE_NEAREST_NEAREST = 0xF0,
+  PACMAN_FE_INTERP_CLAMP = 0xF1,
+  PACMAN_FE_INTERP_NEAREST = 0xF2,
+  PACMAN_FE_INTERP_ZEROFILL = 0xF3,
+  PACMAN_FE_INTERP_EXTRAP = 0xF4,
+  PACMAN_MLS = 0xF6
 };
 
 /* Replaces pacman_kokkos_initialize_c (src/fortran_interface.h:69,
@@ -61,6 +73,20 @@ int pacman_rbf_interpolate_c(int spaceDimension, int execSpace, int rbfFunction,
                              const double *sourceValues,
                              const double *targetPoints, int nTargetPoints,
                              double *targetValues);
+
+/* Replaces pacman_fe_interpolate_c (src/fortran_interface.h:148-153,
+ * src/fortran_interface.cpp) for method PACMAN_FE_NEAREST_NEAREST: every
+ * target takes the value of its nearest source (ties: lowest source index)
+ * and status 0 (OUTSIDE); connectivity arguments are not read (NULL / 0
+ * allowed). Caller-allocated outputs of length nTargetPoints. Returns 0, -1
+ * on an error (message in pacman_rbf_last_error), -2 on an unknown failure. */
+int pacman_fe_interpolate_c(int spaceDimension, int execSpace, int method,
+                            const double *sourcePoints, int nSourcePoints,
+                            const double *sourceValues, const int *connVal,
+                            int connValSize, const int *connOff,
+                            int connOffSize, const int *cellTypes,
+                            const double *targetPoints, int nTargetPoints,
+                            double *targetValues, int *targetStatus);
 
 #ifdef __cplusplus
 }

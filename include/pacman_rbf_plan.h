@@ -76,6 +76,18 @@ int pacman_rbf_plan_work(pacman_rbf_plan plan, double *work, int n);
 
 const char *pacman_rbf_last_error(void);
 
+/* Nearest-source transfer (FE method NEAREST_NEAREST) on device-resident
+ * arrays: targetValues[i] = sourceValues[nearest source of target i] (ties:
+ * lowest source index), targetStatus[i] = 0. No reference counterpart with
+ * device pointers (the reference's entry is pacman_fe_interpolate_c, which
+ * this library also exports). Synchronous; returns 0 or -1. */
+int pacman_fe_nearest_device(int spaceDimension, const double *sourcePoints,
+                             long long nSourcePoints,
+                             const double *sourceValues,
+                             const double *targetPoints,
+                             long long nTargetPoints, double *targetValues,
+                             int *targetStatus, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

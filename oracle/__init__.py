@@ -78,6 +78,9 @@ def lib():
             ctypes.c_double, _dp, ctypes.c_int64, _dp, _dp, ctypes.c_int64,
             _dp]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_fe_nearest.argtypes = [
+            ctypes.c_int, _dp, ctypes.c_int64, _dp, _dp, ctypes.c_int64,
+            ctypes.c_int, _dp, _ip]
         _lib = L
     return _lib
 
@@ -223,6 +226,22 @@ def rbf_interpolate(dim, rbf_function, source_points, source_values,
     if rc != 0:
         raise ValueError("oracle rejected the parameters")
     return out
+
+
+def fe_nearest(dim, source_points, source_values, target_points,
+               brute=False):
+    """Oracle of the FE NEAREST_NEAREST transfer: (values, status)."""
+    src = _f64(source_points).reshape(-1, dim)
+    tgt = _f64(target_points).reshape(-1, dim)
+    vals = _f64(source_values)
+    out = np.zeros(tgt.shape[0])
+    st = np.zeros(tgt.shape[0], np.int32)
+    rc = lib().oracle_fe_nearest(int(dim), _ptr(src), src.shape[0],
+                                 _ptr(vals), _ptr(tgt), tgt.shape[0],
+                                 int(bool(brute)), _ptr(out), _ptr(st, _ip))
+    if rc != 0:
+        raise ValueError("oracle rejected the parameters")
+    return out, st
 
 
 def franke_3d(x, y, z):

@@ -959,6 +959,74 @@ int oracle_setup_apply(void *h, const double *vals, int F, int64_t kBegin,
   return 0;
 }
 
+// FE method NEAREST_NEAREST (src/finite_elements/FTNearest.hpp:28-52): the
+// value of each target's nearest source (NearestExtractIndex,
+// src/finite_elements/utils/ArborXCallbacks.hpp:172-182; ties -> lowest
+// source index), status OUTSIDE = 0 for every target and value 0 without
+// sources (src/common/transfer.hxx:100-109). brute != 0: plain O(N M) scan.
+int oracle_fe_nearest(int dim, const double *src, int64_t ns,
+                      const double *vals, const double *tgt, int64_t nt,
+                      int brute, double *out, int *status) {
+  if (dim < 1 || dim > 3)
+    return -1;
+  std::vector<double> S(3 * (size_t)std::max<int64_t>(ns, 0), 0.0);
+  for (i64 i = 0; i < ns; ++i)
+    for (int a = 0; a < dim; ++a)
+      S[3 * i + a] = src[i * dim + a];
+  Grid g;
+  g.build(S.data(), ns, 0.0, brute != 0);
+  if (!g.brute) {
+    // cell edge ~ two points per cell over the bounding box
+    double lo[3], hi[3], vol = 1.0;
+    int dd = 0;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::numeric_limits<double>::max();
+      hi[a] = -lo[a];
+    }
+    for (i64 i = 0; i < ns; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], S[3 * i + a]);
+        hi[a] = std::max(hi[a], S[3 * i + a]);
+      }
+    for (int a = 0; a < 3; ++a)
+      if (hi[a] > lo[a]) {
+        vol *= hi[a] - lo[a];
+        ++dd;
+      }
+    const double cell =
+        dd ? std::pow(vol * 2.0 / (double)ns, 1.0 / dd) : 1.0;
+    g = Grid();
+    g.build(S.data(), ns, cell, false);
+  }
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i64 t = 0; t < nt; ++t) {
+    double q[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < dim; ++a)
+      q[a] = tgt[t * dim + a];
+    status[t] = 0;
+    i64 best = -1;
+    if (ns > 0) {
+      if (g.brute) {
+        double bd = std::numeric_limits<double>::infinity();
+        for (i64 i = 0; i < ns; ++i) {
+          const double d2 = sqdiff(q, &S[3 * i]);
+          if (d2 < bd) {
+            bd = d2;
+            best = i;
+          }
+        }
+      } else {
+        // nearestWithin is exact for every point within rad: grow rad until
+        // one is found
+        for (double rad = g.h; best < 0; rad *= 2.0)
+          best = nearestWithin(g, q, rad);
+      }
+    }
+    out[t] = best >= 0 ? vals[best] : 0.0;
+  }
+  return 0;
+}
+
 // One-call restatement of PACMAN::rbf_interpolate with explicit parameters.
 int oracle_rbf_interpolate(int dim, int rbfFunction, int npc, double overlap,
                            double support, const double *src, int64_t ns,

@@ -3,16 +3,18 @@
 Drop-in for the reference module ``pacman`` on its RBF-PUM path
 (src/pybindings/bindings.cpp:17-93): ``initialize``, ``finalize``,
 ``execspaces`` and ``rbf`` (``interpolate``, ``functions``), so
-``import pacman_amd as pacman`` keeps callers of that path working. The
+``import pacman_amd as pacman`` keeps callers of that path working. ``fe``
+serves the nearest-source FE method (NEAREST_NEAREST) on the same search. The
 compute runs in hand-written HIP kernels for gfx950 inside
 ``libpacman_rbf.so``; there is no CPU fallback.
 """
 
 from . import execspaces
+from . import fe
 from . import rbf
 from . import _lib
 
-__all__ = ["execspaces", "rbf", "initialize", "finalize"]
+__all__ = ["execspaces", "fe", "rbf", "initialize", "finalize"]
 
 
 def initialize() -> None:

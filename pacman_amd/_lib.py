@@ -26,6 +26,10 @@ SIGNATURES = {
     "pacman_rbf_interpolate_c": (ctypes.c_int, [
         ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
         ctypes.c_int, _dp]),
+    "pacman_fe_interpolate_c": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
+        _ip, ctypes.c_int, _ip, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp,
+        _ip]),
     # include/pacman_rbf_plan.h
     "pacman_rbf_plan_create": (ctypes.c_int, [
         ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -46,6 +50,9 @@ SIGNATURES = {
     "pacman_rbf_plan_patch_info": (ctypes.c_int, [_vp, _ip, _ip]),
     "pacman_rbf_plan_phase_times": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "pacman_rbf_plan_work": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "pacman_fe_nearest_device": (ctypes.c_int, [
+        ctypes.c_int, _vp, ctypes.c_longlong, _vp, _vp, ctypes.c_longlong,
+        _vp, _vp, _vp]),
     "pacman_rbf_last_error": (ctypes.c_char_p, []),
 }
 

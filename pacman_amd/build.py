@@ -4,9 +4,10 @@ Run as ``python -m pacman_amd.build`` or through ``__graft_entry__.build()``.
 Each translation unit is compiled by hipcc with its own floating-point
 contraction policy:
 
-* rbf_geometry.hip, rbf_plan.hip: ``-ffp-contract=off`` - distances,
-  lattice coordinates and probe points must round exactly as written so the
-  patch membership is bit-identical to the CPU restatement;
+* rbf_geometry.hip, rbf_plan.hip, capi.hip, fe_nearest.hip:
+  ``-ffp-contract=off`` - distances, lattice coordinates and probe points
+  must round exactly as written so the patch membership and nearest
+  neighbours are bit-identical to the CPU restatement;
 * rbf_solve.hip and the size-bin units rbf_solve_tile_a.hip ...
   rbf_solve_tile_d.hip: FMA contraction allowed (dense arithmetic).
 
@@ -37,6 +38,7 @@ UNITS = {
     "rbf_geometry.hip": ["-ffp-contract=off"],
     "rbf_plan.hip": ["-ffp-contract=off"],
     "capi.hip": ["-ffp-contract=off"],
+    "fe_nearest.hip": ["-ffp-contract=off"],
     "rbf_solve.hip": [],
     "rbf_solve_tile_a.hip": [],
     "rbf_solve_tile_b.hip": [],

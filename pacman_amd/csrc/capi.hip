@@ -4,10 +4,14 @@
 // src/fortran_interface.cpp:63-91 and src/rbf_interface.cpp:101-130: same
 // validation order and messages, same error codes, but an execution space
 // this build does not provide is an error, not exit(1).
+// pacman_fe_interpolate_c / PACMAN::fe_interpolate replace
+// src/fortran_interface.cpp and src/interface.cpp:93-125 for the method
+// NEAREST_NEAREST.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -92,6 +96,32 @@ void run_host_transfer(int dim, int kind, const double *src, int64_t ns,
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Nearest-source transfer on host arrays.
+void run_host_nearest(int dim, const double *src, int64_t ns,
+                      const double *vals, const double *tgt, int64_t nt,
+                      double *out, int *status) {
+  hipStream_t s = nullptr;
+  DevBuf<double> dsrc(std::max<int64_t>(ns * dim, 1)),
+      dvals(std::max<int64_t>(ns, 1)), dtgt(std::max<int64_t>(nt * dim, 1)),
+      dout(std::max<int64_t>(nt, 1));
+  DevBuf<int> dst(std::max<int64_t>(nt, 1));
+  if (ns > 0) {
+    PACMAN_HIP_CHECK(hipMemcpyAsync(dsrc.get(), src, 8 * (size_t)ns * dim,
+                                    hipMemcpyHostToDevice, s));
+    PACMAN_HIP_CHECK(hipMemcpyAsync(dvals.get(), vals, 8 * (size_t)ns,
+                                    hipMemcpyHostToDevice, s));
+  }
+  PACMAN_HIP_CHECK(hipMemcpyAsync(dtgt.get(), tgt, 8 * (size_t)nt * dim,
+                                  hipMemcpyHostToDevice, s));
+  pacman_amd::fe_nearest_transfer(dim, dsrc.get(), ns, dvals.get(), dtgt.get(),
+                                  nt, dout.get(), dst.get(), s);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipMemcpyAsync(status, dst.get(), 4 * (size_t)nt,
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -139,6 +169,55 @@ RbfInterpolateResult rbf_interpolate(int_t spaceDimension,
   return result;
 }
 
+FeInterpolateResult fe_interpolate(int_t spaceDimension,
+                                   unsigned char execSpace, method_t method,
+                                   coordinates_t *sourcePoints,
+                                   int_t nSourcePoints, fp_t *sourceValues,
+                                   int_t *connVal, int_t connValSize,
+                                   offset_t *connOff, int_t connOffSize,
+                                   cell_t *cellTypes,
+                                   coordinates_t *targetPoints,
+                                   int_t nTargetPoints, bool fortranIndexing) {
+  (void)connVal;
+  (void)connValSize;
+  (void)connOff;
+  (void)connOffSize;
+  (void)cellTypes;
+  (void)fortranIndexing;
+  // src/interface.cpp:101-107
+  if (sourcePoints == nullptr && nSourcePoints > 0)
+    throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
+  if (targetPoints == nullptr && nTargetPoints > 0)
+    throw std::invalid_argument("targetPoints is null but nTargetPoints > 0");
+  if (sourceValues == nullptr && nSourcePoints > 0)
+    throw std::invalid_argument("sourceValues is null but nSourcePoints > 0");
+  if (spaceDimension < 1 || spaceDimension > 3)
+    throw std::runtime_error(
+        "The dimension of the points can only be: 1, 2 or 3.\n");
+  const char *name = space_name(execSpace);
+  if (name == nullptr)
+    throw std::invalid_argument(
+        "execspace: Serial, OpenMP, Threads, Cuda, HIP, SYCL");
+  if (execSpace != ExecSpaces::HIP)
+    throw std::invalid_argument(std::string("execution space ") + name +
+                                " is not enabled in this build (available: HIP)");
+  if (method != (method_t)TransferMethods::NEAREST_NEAREST)
+    throw std::invalid_argument("FE method " + std::to_string(method) +
+                                " is not enabled in this build (available: "
+                                "NEAREST_NEAREST)");
+  if (nSourcePoints < 0 || nTargetPoints < 0)
+    throw std::invalid_argument("negative number of points");
+  FeInterpolateResult result;
+  result.targetValues.assign((size_t)nTargetPoints, 0.0);
+  result.targetStatus.assign((size_t)nTargetPoints, 0);
+  if (nTargetPoints == 0)
+    return result;
+  run_host_nearest(spaceDimension, sourcePoints, nSourcePoints, sourceValues,
+                   targetPoints, nTargetPoints, result.targetValues.data(),
+                   result.targetStatus.data());
+  return result;
+}
+
 } // namespace PACMAN
 
 // ---------------------------------------------------------------------------
@@ -180,6 +259,76 @@ int pacman_rbf_interpolate_c(int spaceDimension, int execSpace, int rbfFunction,
   } catch (...) {
     set_error("pacman_rbf_interpolate_c", "unknown exception");
     std::cerr << "pacman_rbf_interpolate_c: unknown exception\n";
+    return -2;
+  }
+}
+
+int pacman_fe_interpolate_c(int spaceDimension, int execSpace, int method,
+                            const double *sourcePoints, int nSourcePoints,
+                            const double *sourceValues, const int *connVal,
+                            int connValSize, const int *connOff,
+                            int connOffSize, const int *cellTypes,
+                            const double *targetPoints, int nTargetPoints,
+                            double *targetValues, int *targetStatus) {
+  try {
+    if ((targetValues == nullptr || targetStatus == nullptr) &&
+        nTargetPoints > 0)
+      throw std::invalid_argument(
+          "targetValues / targetStatus is null but nTargetPoints > 0");
+    PACMAN::FeInterpolateResult r = PACMAN::fe_interpolate(
+        spaceDimension, (unsigned char)execSpace, method,
+        const_cast<double *>(sourcePoints), nSourcePoints,
+        const_cast<double *>(sourceValues), const_cast<int *>(connVal),
+        connValSize, const_cast<int *>(connOff), connOffSize,
+        const_cast<int *>(cellTypes), const_cast<double *>(targetPoints),
+        nTargetPoints);
+    for (int i = 0; i < nTargetPoints; ++i) {
+      targetValues[i] = r.targetValues[(size_t)i];
+      targetStatus[i] = r.targetStatus[(size_t)i];
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_fe_interpolate_c", e.what());
+    std::cerr << "pacman_fe_interpolate_c: " << e.what() << "\n";
+    return -1;
+  } catch (...) {
+    set_error("pacman_fe_interpolate_c", "unknown exception");
+    std::cerr << "pacman_fe_interpolate_c: unknown exception\n";
+    return -2;
+  }
+}
+
+int pacman_fe_nearest_device(int spaceDimension, const double *sourcePoints,
+                             long long nSourcePoints,
+                             const double *sourceValues,
+                             const double *targetPoints,
+                             long long nTargetPoints, double *targetValues,
+                             int *targetStatus, void *stream) {
+  try {
+    if (spaceDimension < 1 || spaceDimension > 3)
+      throw std::runtime_error(
+          "The dimension of the points can only be: 1, 2 or 3.");
+    if (nSourcePoints < 0 || nTargetPoints < 0)
+      throw std::invalid_argument("negative number of points");
+    if (nSourcePoints > INT_MAX / 2 || nTargetPoints > INT_MAX / 2)
+      throw std::invalid_argument("point counts above 2^30 are not supported");
+    if ((sourcePoints == nullptr || sourceValues == nullptr) &&
+        nSourcePoints > 0)
+      throw std::invalid_argument("null source pointer");
+    if ((targetPoints == nullptr || targetValues == nullptr ||
+         targetStatus == nullptr) &&
+        nTargetPoints > 0)
+      throw std::invalid_argument("null target pointer");
+    pacman_amd::fe_nearest_transfer(spaceDimension, sourcePoints,
+                                    nSourcePoints, sourceValues, targetPoints,
+                                    nTargetPoints, targetValues, targetStatus,
+                                    (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_fe_nearest_device", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_fe_nearest_device", "unknown exception");
     return -2;
   }
 }

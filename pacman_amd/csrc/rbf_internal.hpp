@@ -175,6 +175,13 @@ struct BlendArgs {
 };
 void launch_blend(const BlendArgs &a, hipStream_t s);
 
+// Nearest-source transfer (FE NEAREST_NEAREST): out[t] = vals[nearest source
+// of target t] (ties: lowest source index), status[t] = 0 (OUTSIDE);
+// device pointers, points row-major [n x dim].
+void fe_nearest_transfer(int dim, const double *dSrc, int64_t N,
+                         const double *dVals, const double *dTgt, int64_t M,
+                         double *dOut, int *dStatus, hipStream_t s);
+
 void launch_gather_values(const double *vals, const int *perm, int64_t N,
                           int F, double *out, hipStream_t s);
 

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     a.status[patch] = spivot;
 
   // evaluation at the member targets: tiles of 64 pairs, thread = (pair,
-  // slice); slices of ceil(n / 8) sources, two at a time into two partial
+  // slice); slices of ceil(n / 8) sources, four at a time into four partial
   // sums, added in a fixed order. The next tile's targets are loaded while
   // the current tile is summed.
   {
@@ -518,42 +518,47 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
         yy = a.ty[tp];
         yz = a.tz[tp];
       }
-      double acc0[kFMax], acc1[kFMax];
+      double acc0[kFMax], acc1[kFMax], acc2[kFMax], acc3[kFMax];
 #pragma unroll
       for (int f = 0; f < kFMax; ++f) {
         acc0[f] = 0.0;
         acc1[f] = 0.0;
+        acc2[f] = 0.0;
+        acc3[f] = 0.0;
       }
       if (mem) {
         with_kind([&](auto kc) __attribute__((always_inline)) {
           constexpr int K = decltype(kc)::value;
-          int l = l0;
-          for (; l + 1 < l1; l += 2) {
-            const double phi0 = wendland<K, double, true>(
+          auto phi = [&](int l) __attribute__((always_inline)) {
+            return wendland<K, double, true>(
                 dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
-            const double phi1 = wendland<K, double, true>(
-                dist_fast(qx, qy, qz, px[l + 1], py[l + 1], pz[l + 1]), dinv);
+          };
+          int l = l0;
+          for (; l + 3 < l1; l += 4) {
+            const double p0 = phi(l), p1 = phi(l + 1), p2 = phi(l + 2),
+                         p3 = phi(l + 3);
 #pragma unroll
             for (int f = 0; f < kFMax; ++f)
               if (f < F) {
-                acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
-                acc1[f] = fma(phi1, rhs[f][l + 1], acc1[f]);
+                acc0[f] = fma(p0, rhs[f][l], acc0[f]);
+                acc1[f] = fma(p1, rhs[f][l + 1], acc1[f]);
+                acc2[f] = fma(p2, rhs[f][l + 2], acc2[f]);
+                acc3[f] = fma(p3, rhs[f][l + 3], acc3[f]);
               }
           }
-          if (l < l1) {
-            const double phi0 = wendland<K, double, true>(
-                dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
+          for (; l < l1; ++l) {
+            const double p0 = phi(l);
 #pragma unroll
             for (int f = 0; f < kFMax; ++f)
               if (f < F)
-                acc0[f] = fma(phi0, rhs[f][l], acc0[f]);
+                acc0[f] = fma(p0, rhs[f][l], acc0[f]);
           }
         });
       }
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
         if (f < F)
-          epart[sl][f][lane] = acc0[f] + acc1[f];
+          epart[sl][f][lane] = (acc0[f] + acc1[f]) + (acc2[f] + acc3[f]);
       __syncthreads();
       if (sl == 0 && mem) {
         const int slot = a.pairSlot[p0 + pi];

@@ -121,3 +121,42 @@ def test_plan_create_validation():
     assert lib.pacman_rbf_plan_create(ctypes.byref(h), 0, 0x11, 50, 0.15, 0.1,
                                       None, 0, None, 0, 0, None) == -1
     assert not h.value
+
+
+def test_fe_constants_and_validation():
+    # src/common/types.hpp:29-45, src/fortran_interface.h:51-59
+    m = pacman.fe.methods
+    assert (m.NEAREST_NEAREST, m.INTERP_CLAMP, m.INTERP_NEAREST,
+            m.INTERP_ZEROFILL, m.INTERP_EXTRAP) == (0xF0, 0xF1, 0xF2, 0xF3,
+                                                    0xF4)
+    hdr = open(os.path.join(INCLUDE, "fortran_interface.h")).read()
+    assert re.search(r"PACMAN_FE_NEAREST_NEAREST\s*=\s*0xF0", hdr)
+    assert re.search(r"PACMAN_MLS\s*=\s*0xF6", hdr)
+    with pytest.raises(ValueError):   # methods other than NEAREST_NEAREST
+        pacman.fe.interpolate(3, 4, m.INTERP_CLAMP, P3, V4, None, None, None,
+                              P3)
+    with pytest.raises(ValueError):   # dimension mismatch
+        pacman.fe.interpolate(3, 4, m.NEAREST_NEAREST, P3, V4, None, None,
+                              None, P2)
+    with pytest.raises(RuntimeError):
+        pacman.fe.interpolate(4, 4, m.NEAREST_NEAREST, np.zeros((4, 4)), V4,
+                              None, None, None, np.zeros((4, 4)))
+    lib = _lib.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    p = P3.ctypes.data_as(dp)
+    v = V4.ctypes.data_as(dp)
+    out = np.zeros(4)
+    st = np.zeros(4, np.int32)
+    o = out.ctypes.data_as(dp)
+    s = st.ctypes.data_as(ip)
+    assert lib.pacman_fe_interpolate_c(3, 4, 0xF1, p, 4, v, None, 0, None, 0,
+                                       None, p, 4, o, s) == -1
+    assert "not enabled" in _lib.last_error()
+    assert lib.pacman_fe_interpolate_c(3, 1, 0xF0, p, 4, v, None, 0, None, 0,
+                                       None, p, 4, o, s) == -1
+    assert lib.pacman_fe_interpolate_c(3, 4, 0xF0, None, 4, v, None, 0, None,
+                                       0, None, p, 4, o, s) == -1
+    assert "sourcePoints is null" in _lib.last_error()
+    assert lib.pacman_fe_interpolate_c(3, 4, 0xF0, p, 4, v, None, 0, None, 0,
+                                       None, p, 0, o, s) == 0

@@ -322,3 +322,33 @@ def test_coplanar_patch_drops_normal_axis(oracle_mod):
     out, pv = o.apply(S[:, 0] + S[:, 1], return_pv=True)
     assert np.all(pv == 3)
     assert np.allclose(out, T[:, 0] + T[:, 1], atol=1e-10)
+
+
+# ---- FE NEAREST_NEAREST ------------------------------------------------------
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_fe_nearest_matches_brute_force(oracle_mod, dim):
+    """Grid search == numpy brute force (lowest index wins ties), status 0."""
+    rng = np.random.default_rng(40 + dim)
+    S = rng.uniform(0, 1, (700, dim))
+    S[350:360] = S[10:20]               # duplicated sources: exact ties
+    T = np.concatenate([rng.uniform(-0.3, 1.3, (400, dim)), S[5:25]])
+    v = rng.normal(size=len(S))
+    out, st = oracle_mod.fe_nearest(dim, S, v, T)
+    d2 = ((T[:, None, :] - S[None]) ** 2).sum(-1)
+    ref = v[np.argmin(d2, axis=1)]      # argmin returns the first minimum
+    np.testing.assert_array_equal(out, ref)
+    assert not st.any()
+    out_b, _ = oracle_mod.fe_nearest(dim, S, v, T, brute=True)
+    np.testing.assert_array_equal(out, out_b)
+
+
+def test_fe_nearest_same_points_and_empty(oracle_mod):
+    """Same source and target cloud copies the values; no sources -> 0."""
+    rng = np.random.default_rng(7)
+    S = rng.uniform(0, 1, (300, 3))
+    v = rng.normal(size=300)
+    out, _ = oracle_mod.fe_nearest(3, S, v, S)
+    np.testing.assert_array_equal(out, v)
+    out0, st0 = oracle_mod.fe_nearest(3, np.zeros((0, 3)), np.zeros(0), S)
+    assert not out0.any() and not st0.any()
