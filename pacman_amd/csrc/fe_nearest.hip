@@ -10,6 +10,9 @@
 // Here the sources are cell-sorted once (the grid of rbf_geometry.hip) and
 // every target thread searches shells of cells around its own cell until
 // the best squared distance is below the distance to the unsearched region.
+// Targets are cell-sorted too and processed in that order, so the threads of
+// a wave search neighbouring cells (coherent loops, L2 reuse) and scatter
+// their results back to the original order.
 // The minimum is over (squared distance, source index): ties go to the
 // lowest index, so the answer is unique and equals brute force. Compiled
 // with -ffp-contract=off: distances round exactly like the oracle's.
@@ -40,14 +43,12 @@ __device__ inline double outside_gap(const GridView &g, int a, double v,
 }
 
 __global__ __launch_bounds__(kNearestBlock) void nearest_value_kernel(
-    GridView g, const double *tgt, int64_t M, int dim, const double *vals,
-    double *out, int *status) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= M)
+    GridView g, GridView tg, const double *vals, double *out, int *status) {
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p0 >= tg.n)
     return;
-  double q[3] = {0.0, 0.0, 0.0};
-  for (int a = 0; a < dim; ++a)
-    q[a] = tgt[t * dim + a];
+  const int64_t t = tg.perm[p0];  // original target index
+  const double q[3] = {tg.x[p0], tg.y[p0], tg.z[p0]};
   status[t] = 0;  // TransferStatus::OUTSIDE
   if (g.n == 0) {
     out[t] = 0.0;
@@ -117,12 +118,14 @@ void fe_nearest_transfer(int dim, const double *dSrc, int64_t N,
                          double *dOut, int *dStatus, hipStream_t s) {
   if (M <= 0)
     return;
-  double lo[3], hi[3];
+  double lo[3], hi[3], tlo[3], thi[3];
   bbox_device(dSrc, N, dim, lo, hi, s);
-  DeviceGrid g;
+  bbox_device(dTgt, M, dim, tlo, thi, s);
+  DeviceGrid g, tg;
   build_grid(dSrc, N, dim, lo, hi, 2.0, g, s);
+  build_grid(dTgt, M, dim, tlo, thi, 4.0, tg, s);
   const unsigned blocks = (unsigned)((M + kNearestBlock - 1) / kNearestBlock);
-  nearest_value_kernel<<<blocks, kNearestBlock, 0, s>>>(g.view, dTgt, M, dim,
+  nearest_value_kernel<<<blocks, kNearestBlock, 0, s>>>(g.view, tg.view,
                                                          dVals, dOut, dStatus);
   PACMAN_HIP_CHECK(hipGetLastError());
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
