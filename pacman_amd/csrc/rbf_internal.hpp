@@ -149,9 +149,9 @@ struct SolveArgs {
 int solve_fields_per_launch();
 // Polynomial fit of every patch (one wave per patch): coef[K][F][4], pvOut.
 void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s);
-// Size bin (16-row tile blocks R) of a patch of n sources: room for the
-// patch and the right-hand-side rows of one launch.
-int solve_bin_rows(int n);
+// Size bin (16-row tile blocks R) of a patch of n sources solved for F
+// fields in one launch: room for the patch and the F right-hand-side rows.
+int solve_bin_rows(int n, int F);
 // Launches the FP64 per-patch solve for the patches of bin R.
 void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
                       hipStream_t s);

@@ -149,30 +149,18 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
                                   hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   maxN_ = 0;
-  std::vector<std::vector<int>> bins(solve_bin_rows(kMaxPatch) + 1);
-  for (int64_t kk = 0; kk < K_; ++kk) {
-    const int n = (int)(off[kk + 1] - off[kk]);
-    maxN_ = std::max(maxN_, n);
-    if (n > kMaxPatch)
-      continue;
-    bins[solve_bin_rows(n)].push_back((int)kk);
-  }
+  for (int64_t kk = 0; kk < K_; ++kk)
+    maxN_ = std::max(maxN_, (int)(off[kk + 1] - off[kk]));
   if (maxN_ > kMaxPatch)
     throw std::runtime_error(
         "a patch holds " + std::to_string(maxN_) +
         " source points; this build solves patches of up to " +
         std::to_string(kMaxPatch) + " points (lower nodesPerCluster)");
-  for (int R = 0; R < (int)bins.size(); ++R) {
-    if (bins[R].empty())
-      continue;
-    DevBuf<int> d(bins[R].size());
-    PACMAN_HIP_CHECK(hipMemcpyAsync(d.get(), bins[R].data(),
-                                    4 * bins[R].size(), hipMemcpyHostToDevice,
-                                    s));
-    binDev_.push_back(std::move(d));
-    binPatches_.push_back(std::move(bins[R]));
-    binR_.push_back(R);
-  }
+  patchN_.resize(K_);
+  for (int64_t kk = 0; kk < K_; ++kk)
+    patchN_[kk] = (int)(off[kk + 1] - off[kk]);
+  bins_.resize(solve_fields_per_launch());
+  bins_for(1, s);
   pv_.alloc(std::max<int64_t>(K_, 1));
   status_.alloc(std::max<int64_t>(K_, 1));
   for (auto &e : ev_)
@@ -180,6 +168,30 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   setupMs_[3] = ms_since(tPhase);
   setupMs_[4] = ms_since(tSetup);
+}
+
+// Patches grouped by tile blocks R = solve_bin_rows(n, F), one launch each.
+const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
+  Bins &B = bins_[F - 1];
+  if (B.built)
+    return B;
+  std::vector<std::vector<int>> bins(solve_bin_rows(kMaxPatch, F) + 1);
+  for (int64_t kk = 0; kk < K_; ++kk)
+    bins[solve_bin_rows(patchN_[kk], F)].push_back((int)kk);
+  for (int R = 0; R < (int)bins.size(); ++R) {
+    if (bins[R].empty())
+      continue;
+    B.host.push_back(std::move(bins[R]));
+    const std::vector<int> &h = B.host.back();
+    DevBuf<int> d(h.size());
+    PACMAN_HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), 4 * h.size(),
+                                    hipMemcpyHostToDevice, s));
+    B.dev.push_back(std::move(d));
+    B.count.push_back((int)h.size());
+    B.R.push_back(R);
+  }
+  B.built = true;
+  return B;
 }
 
 RbfPumPlan::~RbfPumPlan() {
@@ -287,10 +299,11 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
     launch_poly(a, K_, coef_.get(), s);
     if (f0 == 0)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
-    for (size_t b = 0; b < binR_.size(); ++b) {
-      a.patches = binDev_[b].get();
-      a.count = (int)binPatches_[b].size();
-      launch_solve_f64(binR_[b], prm_.kind, a, coef_.get(), s);
+    const Bins &B = bins_for(nf, s);
+    for (size_t b = 0; b < B.R.size(); ++b) {
+      a.patches = B.dev[b].get();
+      a.count = B.count[b];
+      launch_solve_f64(B.R[b], prm_.kind, a, coef_.get(), s);
     }
     BlendArgs bl{};
     bl.toff = pairs_.toff.get();

@@ -70,9 +70,17 @@ private:
   DevBuf<int64_t> srcOff_;
   DevBuf<int> srcIdx_;
   PairSet pairs_;
-  std::vector<std::vector<int>> binPatches_;  // by register tile R
-  std::vector<DevBuf<int>> binDev_;
-  std::vector<int> binR_;
+  // Patches binned by tile blocks R, one binning per fields-per-launch
+  // count F (index F - 1).
+  struct Bins {
+    bool built = false;
+    std::vector<int> R, count;
+    std::vector<std::vector<int>> host;  // kept alive for the async copies
+    std::vector<DevBuf<int>> dev;
+  };
+  std::vector<Bins> bins_;
+  std::vector<int> patchN_;  // sources per patch (host)
+  const Bins &bins_for(int F, hipStream_t s);
   int maxN_ = 0;
   DevBuf<int> pv_, status_;
   DevBuf<double> valsSorted_, contrib_, coef_;

@@ -424,19 +424,19 @@ void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s) {
   if (K <= 0)
     return;
   if (a.F < 1 || a.F > kFMax)
-    throw std::runtime_error("poly: 1 <= fields <= 4 per launch");
+    throw std::runtime_error("poly: 1 <= fields <= 8 per launch");
   poly_kernel<<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, coef);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
-int solve_bin_rows(int n) { return (n + kFMax + 15) / 16; }
+int solve_bin_rows(int n, int F) { return (n + F + 15) / 16; }
 
 void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
                       hipStream_t s) {
   if (a.count <= 0)
     return;
   if (a.F < 1 || a.F > kFMax)
-    throw std::runtime_error("solve: 1 <= fields <= 4 per launch");
+    throw std::runtime_error("solve: 1 <= fields <= 8 per launch");
   if (kind < 0 || kind > 4)
     throw std::runtime_error("solve: unknown Wendland kind");
   switch (R) {

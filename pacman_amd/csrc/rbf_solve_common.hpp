@@ -10,7 +10,7 @@
 namespace pacman_amd {
 namespace solve_detail {
 
-constexpr int kFMax = 4;   // fields per launch
+constexpr int kFMax = 8;   // fields per launch
 constexpr int KGRP = 2;    // columns factored per barrier
 
 // Compile-time loop: f(integral_constant<int, I>) for I in [B, E). Keeps

@@ -67,7 +67,9 @@ template <int R> struct TileShape {
   static constexpr int LDW = NMAX + 4;      // panel column stride (doubles)
   static constexpr int PANEL = 2 * 16 * LDW;
   static constexpr int EVAL = NW * kFMax * 64;
-  static constexpr int SBUF = PANEL > EVAL ? PANEL : EVAL;
+  static constexpr int BACK = (R - 1) * 16 * 17 + R * kFMax * 16;
+  static constexpr int SBUF0 = PANEL > EVAL ? PANEL : EVAL;
+  static constexpr int SBUF = SBUF0 > BACK ? SBUF0 : BACK;
 };
 
 template <int R>
@@ -89,7 +91,6 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   __shared__ double linvAll[R][120];         // strictly-lower Linv, packed
   __shared__ double dtile[16][17];           // diagonal tile hand-over
   __shared__ double dpan[2][16], dinvp[2][16];
-  __shared__ double sacc[R][kFMax][16];      // back-substitution sums
   __shared__ double xs[16];
   __shared__ double scoef[kFMax][4];
   __shared__ int spivot;
@@ -288,6 +289,8 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     // soon as their alpha was known) and the sub-diagonal tile is staged in
     // LDS, so the chain from one block to the next stays in this wave.
     double(*sub)[16][17] = reinterpret_cast<double(*)[16][17]>(sbuf);
+    double(*sacc)[kFMax][16] = reinterpret_cast<double(*)[kFMax][16]>(
+        sbuf + (R - 1) * 16 * 17);
     for (int B = P - 1; B >= 0; --B) {
       const bool valid = 16 * B + lc < n;
       for (int f = 0; f < F; ++f) {
@@ -441,8 +444,11 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
         }
       }
     });
-    // stage the sub-diagonal tiles for the control wave
+    // stage the sub-diagonal tiles for the control wave; the back-
+    // substitution sums S_J follow them in the same (now free) panel buffer
     double(*sub)[16][17] = reinterpret_cast<double(*)[16][17]>(sbuf);
+    double(*sacc)[kFMax][16] = reinterpret_cast<double(*)[kFMax][16]>(
+        sbuf + (R - 1) * 16 * 17);
     static_for<0, TPW>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
       const int I = tI[s], J = tJ[s];
