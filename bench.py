@@ -45,9 +45,14 @@ def main() -> int:
     ap.add_argument("--fields", type=int, default=1,
                     help="fields per transfer (franke_3d plus seeded trig "
                          "fields)")
-    ap.add_argument("--mode", default="rbf", choices=["rbf", "fe-nearest"],
-                    help="rbf: RBF-PUM setup+apply (default); fe-nearest: "
-                         "the nearest-source transfer on the same clouds")
+    ap.add_argument("--mode", default="rbf",
+                    choices=["rbf", "apply", "fe-nearest"],
+                    help="rbf: RBF-PUM setup+apply (default); apply: one "
+                         "setup, then --steps applies (repeated coupling "
+                         "transfer); fe-nearest: the nearest-source transfer "
+                         "on the same clouds")
+    ap.add_argument("--graph", action="store_true",
+                    help="apply mode: replay the apply as a HIP graph")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB "
                          "(calibrates HBM byte counters) and exit")
@@ -79,6 +84,8 @@ def main() -> int:
     S, T, npc, support = W.config(args.config)
     if args.mode == "fe-nearest":
         return bench_fe_nearest(args, S, T)
+    if args.mode == "apply":
+        return bench_apply(args, S, T, npc, support)
     v = W.franke_3d(*S.T)
     if args.fields > 1:
         v = np.concatenate([v[None], W.trig_fields(S, args.fields - 1,
@@ -198,6 +205,47 @@ def main() -> int:
             PEAK_FP64_TFLOPS,
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_apply(args, S, T, npc, support) -> int:
+    """Repeated apply on one plan (coupling loop): fields change per step."""
+    import numpy as np
+    import torch
+
+    import pacman_amd as pacman
+    from pacman_amd import workloads as W
+
+    F = args.fields
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(S).to(dev)
+    dT = torch.from_numpy(T).to(dev)
+    vals = torch.from_numpy(W.trig_fields(S, F, seed=11)).to(dev)
+    out = torch.empty((F, len(T)), dtype=torch.float64, device=dev)
+    plan = pacman.rbf.Plan(3, args.rbf, dS, dT, npc, 0.15, support)
+    for _ in range(max(args.warmup, 1)):
+        plan.apply(vals, out=out, graph=args.graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        vals.mul_(-1.0)   # new field contents every step, same buffer
+        plan.apply(vals, out=out, graph=args.graph)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    M = len(T)
+    res = {
+        "metric": "RBF-PUM repeated apply Mpts*fields/s (FP64)",
+        "value": M * F / (ms / 1e3) / 1e6, "unit": "Mpts*fields/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {len(S)} -> {M}, one setup, "
+                               f"{F} trig fields per apply, graph="
+                               f"{bool(args.graph)}",
+                   "fields": F, "graph": bool(args.graph)},
+        "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)
     return 0

@@ -39,6 +39,14 @@ int pacman_rbf_plan_apply(pacman_rbf_plan plan, const double *sourceValues,
                           int nFields, double *targetValues, int onDevice,
                           void *stream);
 
+/* The same apply replayed as a HIP graph, for repeated transfers: the launch
+ * sequence is captured on the first call for these (device) pointers and
+ * nFields and replayed on later calls (new contents of the same buffers are
+ * picked up). Asynchronous on `stream`, ordered with it through events. */
+int pacman_rbf_plan_apply_graph(pacman_rbf_plan plan,
+                                const double *sourceValues, int nFields,
+                                double *targetValues, void *stream);
+
 int pacman_rbf_plan_destroy(pacman_rbf_plan plan);
 
 /* Introspection (host outputs). */

@@ -427,6 +427,28 @@ int pacman_rbf_plan_apply(pacman_rbf_plan plan, const double *sourceValues,
   }
 }
 
+int pacman_rbf_plan_apply_graph(pacman_rbf_plan plan,
+                                const double *sourceValues, int nFields,
+                                double *targetValues, void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    RbfPumPlan &P = *plan->plan;
+    if (nFields < 1)
+      throw std::invalid_argument("nFields must be >= 1");
+    if (sourceValues == nullptr || (targetValues == nullptr && P.numTargets() > 0))
+      throw std::invalid_argument("null values pointer");
+    P.applyGraph(sourceValues, nFields, targetValues, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_apply_graph", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_apply_graph", "unknown exception");
+    return -2;
+  }
+}
+
 int pacman_rbf_plan_destroy(pacman_rbf_plan plan) {
   delete plan;
   return 0;

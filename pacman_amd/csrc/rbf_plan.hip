@@ -195,6 +195,12 @@ const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
 }
 
 RbfPumPlan::~RbfPumPlan() {
+  releaseGraph();
+  if (gstream_)
+    (void)hipStreamDestroy(gstream_);
+  for (auto &e : gev_)
+    if (e)
+      (void)hipEventDestroy(e);
   for (auto &e : ev_)
     if (e)
       (void)hipEventDestroy(e);
@@ -251,25 +257,94 @@ void RbfPumPlan::work(double *w) const {
   w[6] = (double)maxN_;
 }
 
-void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
-                       hipStream_t s) {
+// Buffers and size bins an apply of F fields needs (allocations happen here,
+// never inside enqueue, so enqueue can be captured into a graph).
+void RbfPumPlan::prepare(int F, hipStream_t s) {
   if (F < 1)
     throw std::invalid_argument("apply needs at least one field");
-  if (M_ == 0)
-    return;
   const int64_t P = std::max<int64_t>(pairs_.P, 1);
   if (valsSorted_.n < (size_t)(F * N_))
     valsSorted_.alloc((size_t)F * N_);
-  PACMAN_HIP_CHECK(hipEventRecord(ev_[0], s));
-  launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
-  PACMAN_HIP_CHECK(hipEventRecord(ev_[1], s));
-  std::vector<BlendArgs> blends;
   const int perLaunch = solve_fields_per_launch();
   const int Fc = std::min(F, perLaunch);
   if (contrib_.n < (size_t)(P * Fc))
     contrib_.alloc((size_t)P * Fc);
   if (coef_.n < (size_t)(std::max<int64_t>(K_, 1) * Fc * 4))
     coef_.alloc((size_t)std::max<int64_t>(K_, 1) * Fc * 4);
+  for (int f0 = 0; f0 < F; f0 += perLaunch)
+    bins_for(std::min(perLaunch, F - f0), s);
+}
+
+void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
+                       hipStream_t s) {
+  if (M_ == 0)
+    return;
+  prepare(F, s);
+  enqueue(dVals, F, dOut, s, true);
+  applied_ = true;
+}
+
+// The apply sequence as a HIP graph, captured on first use for these
+// pointers and field count and replayed afterwards on a private stream
+// ordered after `s` (before and after) through events. No allocation,
+// host synchronisation or memset happens inside the captured sequence.
+void RbfPumPlan::applyGraph(const double *dVals, int F, double *dOut,
+                            hipStream_t s) {
+  if (M_ == 0)
+    return;
+  prepare(F, s);
+  if (!gstream_) {
+    PACMAN_HIP_CHECK(hipStreamCreateWithFlags(&gstream_, hipStreamNonBlocking));
+    for (auto &e : gev_)
+      PACMAN_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (!gexec_ || gVals_ != dVals || gOut_ != dOut || gF_ != F) {
+    releaseGraph();
+    PACMAN_HIP_CHECK(
+        hipStreamBeginCapture(gstream_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue(dVals, F, dOut, gstream_, false);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(gstream_, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+      throw;
+    }
+    PACMAN_HIP_CHECK(hipStreamEndCapture(gstream_, &graph_));
+    PACMAN_HIP_CHECK(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+    gVals_ = dVals;
+    gOut_ = dOut;
+    gF_ = F;
+  }
+  PACMAN_HIP_CHECK(hipEventRecord(gev_[0], s));
+  PACMAN_HIP_CHECK(hipStreamWaitEvent(gstream_, gev_[0], 0));
+  PACMAN_HIP_CHECK(hipGraphLaunch(gexec_, gstream_));
+  PACMAN_HIP_CHECK(hipEventRecord(gev_[1], gstream_));
+  PACMAN_HIP_CHECK(hipStreamWaitEvent(s, gev_[1], 0));
+}
+
+void RbfPumPlan::releaseGraph() {
+  if (gexec_)
+    (void)hipGraphExecDestroy(gexec_);
+  if (graph_)
+    (void)hipGraphDestroy(graph_);
+  gexec_ = nullptr;
+  graph_ = nullptr;
+  gVals_ = nullptr;
+  gOut_ = nullptr;
+  gF_ = 0;
+}
+
+void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
+                         hipStream_t s, bool events) {
+  const int perLaunch = solve_fields_per_launch();
+  if (events)
+    PACMAN_HIP_CHECK(hipEventRecord(ev_[0], s));
+  launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
+  if (events)
+    PACMAN_HIP_CHECK(hipEventRecord(ev_[1], s));
+  std::vector<BlendArgs> blends;
   for (int f0 = 0; f0 < F; f0 += perLaunch) {
     const int nf = std::min(perLaunch, F - f0);
     SolveArgs a{};
@@ -297,9 +372,9 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
     a.pvOut = pv_.get();
     a.status = status_.get();
     launch_poly(a, K_, coef_.get(), s);
-    if (f0 == 0)
+    if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
-    const Bins &B = bins_for(nf, s);
+    const Bins &B = bins_[nf - 1];
     for (size_t b = 0; b < B.R.size(); ++b) {
       a.patches = B.dev[b].get();
       a.count = B.count[b];
@@ -327,11 +402,12 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
       launch_blend(bl, s);
     }
   }
-  PACMAN_HIP_CHECK(hipEventRecord(ev_[2], s));
+  if (events)
+    PACMAN_HIP_CHECK(hipEventRecord(ev_[2], s));
   for (const BlendArgs &bl : blends)
     launch_blend(bl, s);
-  PACMAN_HIP_CHECK(hipEventRecord(ev_[3], s));
-  applied_ = true;
+  if (events)
+    PACMAN_HIP_CHECK(hipEventRecord(ev_[3], s));
 }
 
 void RbfPumPlan::centers(double *xyz, int64_t *candidate, hipStream_t s) const {

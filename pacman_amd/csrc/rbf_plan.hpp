@@ -27,6 +27,9 @@ public:
 
   // dVals: F x N (field-major, original point order), dOut: F x M.
   void apply(const double *dVals, int F, double *dOut, hipStream_t s);
+  // The same as a HIP graph replay (captured on the first call for these
+  // pointers and F; asynchronous on s). For repeated transfers.
+  void applyGraph(const double *dVals, int F, double *dOut, hipStream_t s);
 
   const PlanParams &params() const { return prm_; }
   int64_t numSources() const { return N_; }
@@ -81,6 +84,17 @@ private:
   std::vector<Bins> bins_;
   std::vector<int> patchN_;  // sources per patch (host)
   const Bins &bins_for(int F, hipStream_t s);
+  void prepare(int F, hipStream_t s);
+  void enqueue(const double *dVals, int F, double *dOut, hipStream_t s,
+               bool events);
+  void releaseGraph();
+  hipStream_t gstream_ = nullptr;
+  hipEvent_t gev_[2] = {nullptr, nullptr};
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+  const double *gVals_ = nullptr;
+  double *gOut_ = nullptr;
+  int gF_ = 0;
   int maxN_ = 0;
   DevBuf<int> pv_, status_;
   DevBuf<double> valsSorted_, contrib_, coef_;

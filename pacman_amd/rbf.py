@@ -147,8 +147,13 @@ class Plan:
             self._h = ctypes.c_void_p()
 
     # -- apply -------------------------------------------------------------
-    def apply(self, source_values, out=None, stream=None):
-        """Transfers F fields: values (F, N) or (N,) -> (F, M) or (M,)."""
+    def apply(self, source_values, out=None, stream=None, graph=False):
+        """Transfers F fields: values (F, N) or (N,) -> (F, M) or (M,).
+
+        graph=True (device tensors only) replays the launch sequence as a HIP
+        graph captured on the first call for these buffers: for repeated
+        transfers that reuse the same value and output tensors.
+        """
         v = source_values
         squeeze = len(v.shape) == 1
         F = 1 if squeeze else int(v.shape[0])
@@ -166,8 +171,14 @@ class Plan:
         po, devo = _ptr(out)
         if dev != devo:
             raise ValueError("values and output must live in the same place")
-        rc = self._lib.pacman_rbf_plan_apply(self._h, pv, F, po, dev,
-                                             ctypes.c_void_p(stream or 0))
+        if graph:
+            if not dev:
+                raise ValueError("graph replay needs device tensors")
+            rc = self._lib.pacman_rbf_plan_apply_graph(
+                self._h, pv, F, po, ctypes.c_void_p(stream or 0))
+        else:
+            rc = self._lib.pacman_rbf_plan_apply(self._h, pv, F, po, dev,
+                                                 ctypes.c_void_p(stream or 0))
         if rc != 0:
             raise RuntimeError(_lib.last_error())
         return out.reshape(-1) if squeeze else out
