@@ -39,6 +39,15 @@ int pacman_rbf_plan_apply(pacman_rbf_plan plan, const double *sourceValues,
                           int nFields, double *targetValues, int onDevice,
                           void *stream);
 
+/* Transpose of pacman_rbf_plan_apply (no reference counterpart; for the
+ * gradient of a loss with respect to the source values):
+ * sourceGrad[f * nSource + :] = Op^T targetGrad[f * nTarget + :], Op the
+ * linear map source values -> target values of this plan. */
+int pacman_rbf_plan_apply_transpose(pacman_rbf_plan plan,
+                                    const double *targetGrad, int nFields,
+                                    double *sourceGrad, int onDevice,
+                                    void *stream);
+
 /* The same apply replayed as a HIP graph, for repeated transfers: the launch
  * sequence is captured on the first call for these (device) pointers and
  * nFields and replayed on later calls (new contents of the same buffers are

@@ -4,7 +4,9 @@ Drop-in for the reference module ``pacman`` on its RBF-PUM path
 (src/pybindings/bindings.cpp:17-93): ``initialize``, ``finalize``,
 ``execspaces`` and ``rbf`` (``interpolate``, ``functions``), so
 ``import pacman_amd as pacman`` keeps callers of that path working. ``fe``
-serves the nearest-source FE method (NEAREST_NEAREST) on the same search. The
+serves the nearest-source FE method (NEAREST_NEAREST) on the same search;
+``autograd.transfer`` makes a plan's transfer differentiable with respect to
+the source values. The
 compute runs in hand-written HIP kernels for gfx950 inside
 ``libpacman_rbf.so``; there is no CPU fallback.
 """
@@ -14,7 +16,12 @@ from . import fe
 from . import rbf
 from . import _lib
 
-__all__ = ["execspaces", "fe", "rbf", "initialize", "finalize"]
+try:  # the differentiable transfer needs torch
+    from . import autograd
+except ImportError:  # pragma: no cover
+    autograd = None
+
+__all__ = ["autograd", "execspaces", "fe", "rbf", "initialize", "finalize"]
 
 
 def initialize() -> None:

@@ -37,6 +37,8 @@ SIGNATURES = {
         ctypes.c_longlong, ctypes.c_int, _vp]),
     "pacman_rbf_plan_apply": (ctypes.c_int, [
         _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
+    "pacman_rbf_plan_apply_transpose": (ctypes.c_int, [
+        _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
     "pacman_rbf_plan_apply_graph": (ctypes.c_int, [
         _vp, _vp, ctypes.c_int, _vp, _vp]),
     "pacman_rbf_plan_destroy": (ctypes.c_int, [_vp]),

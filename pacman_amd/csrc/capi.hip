@@ -427,6 +427,46 @@ int pacman_rbf_plan_apply(pacman_rbf_plan plan, const double *sourceValues,
   }
 }
 
+int pacman_rbf_plan_apply_transpose(pacman_rbf_plan plan,
+                                    const double *targetGrad, int nFields,
+                                    double *sourceGrad, int onDevice,
+                                    void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    RbfPumPlan &P = *plan->plan;
+    hipStream_t s = (hipStream_t)stream;
+    if (nFields < 1)
+      throw std::invalid_argument("nFields must be >= 1");
+    if ((targetGrad == nullptr && P.numTargets() > 0) ||
+        (sourceGrad == nullptr && P.numSources() > 0))
+      throw std::invalid_argument("null gradient pointer");
+    if (onDevice) {
+      P.applyTranspose(targetGrad, nFields, sourceGrad, s);
+    } else {
+      DevBuf<double> dg((size_t)nFields * std::max<int64_t>(P.numTargets(), 1)),
+          dout((size_t)nFields * std::max<int64_t>(P.numSources(), 1));
+      if (P.numTargets() > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(dg.get(), targetGrad,
+                                        8 * (size_t)nFields * P.numTargets(),
+                                        hipMemcpyHostToDevice, s));
+      P.applyTranspose(dg.get(), nFields, dout.get(), s);
+      if (P.numSources() > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(sourceGrad, dout.get(),
+                                        8 * (size_t)nFields * P.numSources(),
+                                        hipMemcpyDeviceToHost, s));
+      PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_apply_transpose", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_apply_transpose", "unknown exception");
+    return -2;
+  }
+}
+
 int pacman_rbf_plan_apply_graph(pacman_rbf_plan plan,
                                 const double *sourceValues, int nFields,
                                 double *targetValues, void *stream) {

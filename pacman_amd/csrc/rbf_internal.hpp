@@ -144,6 +144,11 @@ struct SolveArgs {
   int dim;
   int *pvOut;               // K (optional) active polynomial columns
   int *status;              // K (optional) nonzero on a zero pivot
+  // transpose (adjoint) apply only
+  const double *wslot;      // P: Shepard weight of each member slot, else 0
+  const double *gSorted;    // F x M upstream values, target cell order
+  int64_t M;
+  double *gammaOut;         // (sum n) x F: A^-1 E^T (w g) per patch source
 };
 // Fields handled by one poly/solve launch.
 int solve_fields_per_launch();
@@ -159,6 +164,25 @@ void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
 template <int R>
 void launch_solve_tile(int kind, const SolveArgs &a, const double *coef,
                        hipStream_t s);
+// Transpose of the per-patch chain for the bin with R blocks: right-hand
+// side E^T (w g) at the member targets, gammaOut = A^-1 of it.
+void launch_solve_f64_adjoint(int R, int kind, const SolveArgs &a,
+                              hipStream_t s);
+template <int R>
+void launch_solve_tile_adjoint(int kind, const SolveArgs &a, hipStream_t s);
+// Adjoint of the polynomial part (one wave per patch): contribS[e * F + f] =
+// gamma + q(x)^T G^-1 (V^T w g - Q^T gamma) at every patch source entry e.
+void launch_adjoint_poly(const SolveArgs &a, int64_t K, const double *gamma,
+                         double *contribS, hipStream_t s);
+// Source-major transpose of the patch source lists: srcTOff[N + 1] offsets,
+// srcTEnt entry indices (into srcIdx) per source in ascending patch order.
+void build_source_transpose(const int *srcIdx, int64_t total, int64_t N,
+                            int64_t *srcTOff, int *srcTEnt, hipStream_t s);
+// out[f * N + perm[p]] = sum of contribS[e * F + f] over the entries e of
+// source position p, in order.
+void launch_source_reduce(const int64_t *srcTOff, const int *srcTEnt,
+                          int64_t N, int F, const double *contribS,
+                          const int *perm, double *out, hipStream_t s);
 
 struct BlendArgs {
   const int64_t *toff;      // M + 1
@@ -174,6 +198,8 @@ struct BlendArgs {
   double *out;              // F x M, original order
 };
 void launch_blend(const BlendArgs &a, hipStream_t s);
+// Shepard weight of every member slot (0 for the others).
+void launch_slot_weights(const BlendArgs &a, double *wslot, hipStream_t s);
 
 // Nearest-source transfer (FE NEAREST_NEAREST): out[t] = vals[nearest source
 // of target t] (ties: lowest source index), status[t] = 0 (OUTSIDE);

@@ -336,6 +336,103 @@ void RbfPumPlan::releaseGraph() {
   gF_ = 0;
 }
 
+SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
+  SolveArgs a{};
+  a.srcOff = srcOff_.get();
+  a.srcIdx = srcIdx_.get();
+  a.sx = gs_.view.x;
+  a.sy = gs_.view.y;
+  a.sz = gs_.view.z;
+  a.vals = vals;
+  a.N = N_;
+  a.F = F;
+  a.pairOff = pairs_.off.get();
+  a.pairTgt = pairs_.tgt.get();
+  a.pairSlot = pairs_.slot.get();
+  a.tx = gt_.view.x;
+  a.ty = gt_.view.y;
+  a.tz = gt_.view.z;
+  a.contrib = contrib_.get();
+  a.cx = cx_.get();
+  a.cy = cy_.get();
+  a.cz = cz_.get();
+  a.rinv = 1.0 / r_;
+  a.dinv = 1.0 / delta_;
+  a.dim = prm_.dim;
+  a.pvOut = pv_.get();
+  a.status = status_.get();
+  a.M = M_;
+  return a;
+}
+
+BlendArgs RbfPumPlan::blendArgs(int F) const {
+  BlendArgs bl{};
+  bl.toff = pairs_.toff.get();
+  bl.slotPatch = pairs_.slotPatch.get();
+  bl.slotMember = pairs_.slotMember.get();
+  bl.contrib = contrib_.get();
+  bl.tx = gt_.view.x;
+  bl.ty = gt_.view.y;
+  bl.tz = gt_.view.z;
+  bl.tperm = gt_.view.perm;
+  bl.cx = cx_.get();
+  bl.cy = cy_.get();
+  bl.cz = cz_.get();
+  bl.M = M_;
+  bl.F = F;
+  bl.rinv = 1.0 / r_;  // interpolator.hpp:91
+  return bl;
+}
+
+// Transpose of apply: grad_f = sum_k S_k^T [gamma_k + Q_k G_k^-1 (V_k^T h_k -
+// Q_k^T gamma_k)], gamma_k = A_k^-1 E_k^T h_k, h_k = w_k g at the member
+// targets of patch k (the slot weights of the blend). Deterministic: every
+// source sums its patch entries in ascending patch order.
+void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
+                                hipStream_t s) {
+  if (F < 1)
+    throw std::invalid_argument("transpose apply needs at least one field");
+  if (N_ == 0)
+    return;
+  const int64_t total = std::max<int64_t>(csrSize(0), 1);
+  const int perLaunch = solve_fields_per_launch();
+  const int Fc = std::min(F, perLaunch);
+  if (!adjReady_) {
+    wslot_.alloc(std::max<int64_t>(pairs_.P, 1));
+    launch_slot_weights(blendArgs(1), wslot_.get(), s);
+    srcTOff_.alloc(N_ + 1);
+    srcTEnt_.alloc(total);
+    build_source_transpose(srcIdx_.get(), csrSize(0), N_, srcTOff_.get(),
+                           srcTEnt_.get(), s);
+    adjReady_ = true;
+  }
+  if (gSorted_.n < (size_t)(Fc * std::max<int64_t>(M_, 1)))
+    gSorted_.alloc((size_t)Fc * std::max<int64_t>(M_, 1));
+  if (gamma_.n < (size_t)(total * Fc))
+    gamma_.alloc((size_t)total * Fc);
+  if (contribS_.n < (size_t)(total * Fc))
+    contribS_.alloc((size_t)total * Fc);
+  for (int f0 = 0; f0 < F; f0 += perLaunch) {
+    const int nf = std::min(perLaunch, F - f0);
+    const Bins &B = bins_for(nf, s);
+    launch_gather_values(dG + (int64_t)f0 * M_, gt_.view.perm, M_, nf,
+                         gSorted_.get(), s);
+    SolveArgs a = solveArgs(nf, nullptr);
+    a.wslot = wslot_.get();
+    a.gSorted = gSorted_.get();
+    a.gammaOut = gamma_.get();
+    for (size_t b = 0; b < B.R.size(); ++b) {
+      a.patches = B.dev[b].get();
+      a.count = B.count[b];
+      launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
+    }
+    launch_adjoint_poly(a, K_, gamma_.get(), contribS_.get(), s);
+    launch_source_reduce(srcTOff_.get(), srcTEnt_.get(), N_, nf,
+                         contribS_.get(), gs_.view.perm,
+                         dGrad + (int64_t)f0 * N_, s);
+  }
+}
+
 void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
                          hipStream_t s, bool events) {
   const int perLaunch = solve_fields_per_launch();
@@ -347,30 +444,7 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
   std::vector<BlendArgs> blends;
   for (int f0 = 0; f0 < F; f0 += perLaunch) {
     const int nf = std::min(perLaunch, F - f0);
-    SolveArgs a{};
-    a.srcOff = srcOff_.get();
-    a.srcIdx = srcIdx_.get();
-    a.sx = gs_.view.x;
-    a.sy = gs_.view.y;
-    a.sz = gs_.view.z;
-    a.vals = valsSorted_.get() + (int64_t)f0 * N_;
-    a.N = N_;
-    a.F = nf;
-    a.pairOff = pairs_.off.get();
-    a.pairTgt = pairs_.tgt.get();
-    a.pairSlot = pairs_.slot.get();
-    a.tx = gt_.view.x;
-    a.ty = gt_.view.y;
-    a.tz = gt_.view.z;
-    a.contrib = contrib_.get();
-    a.cx = cx_.get();
-    a.cy = cy_.get();
-    a.cz = cz_.get();
-    a.rinv = 1.0 / r_;
-    a.dinv = 1.0 / delta_;
-    a.dim = prm_.dim;
-    a.pvOut = pv_.get();
-    a.status = status_.get();
+    SolveArgs a = solveArgs(nf, valsSorted_.get() + (int64_t)f0 * N_);
     launch_poly(a, K_, coef_.get(), s);
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
@@ -380,21 +454,7 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
       a.count = B.count[b];
       launch_solve_f64(B.R[b], prm_.kind, a, coef_.get(), s);
     }
-    BlendArgs bl{};
-    bl.toff = pairs_.toff.get();
-    bl.slotPatch = pairs_.slotPatch.get();
-    bl.slotMember = pairs_.slotMember.get();
-    bl.contrib = contrib_.get();
-    bl.tx = gt_.view.x;
-    bl.ty = gt_.view.y;
-    bl.tz = gt_.view.z;
-    bl.tperm = gt_.view.perm;
-    bl.cx = cx_.get();
-    bl.cy = cy_.get();
-    bl.cz = cz_.get();
-    bl.M = M_;
-    bl.F = nf;
-    bl.rinv = 1.0 / r_;  // interpolator.hpp:91
+    BlendArgs bl = blendArgs(nf);
     bl.out = dOut + (int64_t)f0 * M_;
     if (F <= perLaunch) {
       blends.push_back(bl);  // blend after all solves: phases stay separable

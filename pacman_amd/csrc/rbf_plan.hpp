@@ -30,6 +30,9 @@ public:
   // The same as a HIP graph replay (captured on the first call for these
   // pointers and F; asynchronous on s). For repeated transfers.
   void applyGraph(const double *dVals, int F, double *dOut, hipStream_t s);
+  // Transpose of apply (the gradient with respect to the source values):
+  // dG: F x M (original target order), dGrad: F x N (original source order).
+  void applyTranspose(const double *dG, int F, double *dGrad, hipStream_t s);
 
   const PlanParams &params() const { return prm_; }
   int64_t numSources() const { return N_; }
@@ -88,6 +91,12 @@ private:
   void enqueue(const double *dVals, int F, double *dOut, hipStream_t s,
                bool events);
   void releaseGraph();
+  SolveArgs solveArgs(int F, const double *vals) const;
+  BlendArgs blendArgs(int F) const;
+  bool adjReady_ = false;
+  DevBuf<double> wslot_, gSorted_, gamma_, contribS_;
+  DevBuf<int64_t> srcTOff_;
+  DevBuf<int> srcTEnt_;
   hipStream_t gstream_ = nullptr;
   hipEvent_t gev_[2] = {nullptr, nullptr};
   hipGraph_t graph_ = nullptr;

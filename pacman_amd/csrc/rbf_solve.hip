@@ -223,18 +223,16 @@ __device__ inline bool ill_conditioned(const double (&L)[4][4], int pv,
   return S[0] / fmax(smin, kEps) > kCondMax;
 }
 
-// One wave per patch; coef[(patch * F + f) * 4 + k], pvOut[patch].
-__global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
-                                                   double *coef) {
-  const int patch = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (patch >= K)
-    return;
-  const int lane = threadIdx.x & 63;
+// Active polynomial columns of a patch and the Cholesky factor L of their
+// scaled Gram matrix (one wave, all lanes get the same result): columns
+// ca[0..pv), ca[0] = -1 for the constant, else the axis; the conditioning
+// loop of SerialFillQ (serial_ops.hpp:108-139).
+__device__ inline void poly_basis(const SolveArgs &a, int patch, int lane,
+                                  const double (&c)[3], double sc, int &pv,
+                                  int (&ca)[4], double (&L)[4][4]) {
   const int64_t s0 = a.srcOff[patch];
   const int n = (int)(a.srcOff[patch + 1] - s0);
-  const int F = a.F, dim = a.dim;
-  const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
-  const double sc = a.rinv;
+  const int dim = a.dim;
   double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
   double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
   double G[4][4];
@@ -271,9 +269,7 @@ __global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
       G[j][i] = G[i][j];
     }
   bool active[3] = {dim > 0, dim > 1, dim > 2};
-  int ca[4];
-  int pv = assign_columns(active, ca);
-  double L[4][4];
+  pv = assign_columns(active, ca);
   for (;;) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -302,6 +298,23 @@ __global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
         active[ax] = false;
     pv = assign_columns(active, ca);
   }
+}
+
+// One wave per patch; coef[(patch * F + f) * 4 + k], pvOut[patch].
+__global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
+                                                   double *coef) {
+  const int patch = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (patch >= K)
+    return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int F = a.F;
+  const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
+  const double sc = a.rinv;
+  int pv, ca[4];
+  double L[4][4];
+  poly_basis(a, patch, lane, c, sc, pv, ca, L);
   // beta'' = G^-1 Q''^T b on the active columns (normal equations of the
   // well-conditioned scaled basis); coefficients returned around the centre.
   for (int f = 0; f < F; ++f) {
@@ -416,6 +429,180 @@ __global__ void gather_values_kernel(const double *vals, const int *perm,
     out[(int64_t)f * N + p] = vals[(int64_t)f * N + i];
 }
 
+// Adjoint of the polynomial fit (one wave per patch). The forward transfer
+// of a patch is t = E A^-1 (I - Q G^-1 Q^T) b + V G^-1 Q^T b (G = Q^T Q on
+// the active columns of the scaled basis); its transpose applied to
+// h = w g (member targets) is, with gamma = A^-1 E^T h,
+//   gamma + Q G^-1 (V^T h - Q^T gamma).
+__global__ __launch_bounds__(256) void adjoint_poly_kernel(
+    SolveArgs a, int K, const double *gamma, double *contribS) {
+  const int patch = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (patch >= K)
+    return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int64_t p0 = a.pairOff[patch];
+  const int mall = (int)(a.pairOff[patch + 1] - p0);
+  const int F = a.F;
+  const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
+  const double sc = a.rinv;
+  int pv, ca[4];
+  double L[4][4];
+  poly_basis(a, patch, lane, c, sc, pv, ca, L);
+  for (int f = 0; f < F; ++f) {
+    double r4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int e = lane; e < mall; e += 64) {
+      const int pe = a.pairTgt[p0 + e];
+      if (((unsigned)pe >> 31) == 0)
+        continue;
+      const int tp = pe & 0x7fffffff;
+      const double h = a.wslot[a.pairSlot[p0 + e]] * a.gSorted[f * a.M + tp];
+      r4[0] += h;
+      r4[1] += (a.tx[tp] - c[0]) * sc * h;
+      r4[2] += (a.ty[tp] - c[1]) * sc * h;
+      r4[3] += (a.tz[tp] - c[2]) * sc * h;
+    }
+    for (int t = lane; t < n; t += 64) {
+      const int p = a.srcIdx[s0 + t];
+      const double gm = gamma[(s0 + t) * F + f];
+      r4[0] -= gm;
+      r4[1] -= (a.sx[p] - c[0]) * sc * gm;
+      r4[2] -= (a.sy[p] - c[1]) * sc * gm;
+      r4[3] -= (a.sz[p] - c[2]) * sc * gm;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      r4[k] = wave_sum(r4[k]);
+    // z = G^-1 r on the active columns (L L^T z = r)
+    double y[4], z[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double hv = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        hv = (ca[i] + 1 == k) ? r4[k] : hv;
+      y[i] = hv;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < pv) {
+        double t = y[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k < i)
+            t -= L[i][k] * y[k];
+        y[i] = L[i][i] > 0.0 ? t / L[i][i] : 0.0;
+      }
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+      z[i] = 0.0;
+      if (i < pv) {
+        double t = y[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k > i && k < pv)
+            t -= L[k][i] * z[k];
+        z[i] = L[i][i] > 0.0 ? t / L[i][i] : 0.0;
+      }
+    }
+    double cu[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double v = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        v = (i < pv && ca[i] + 1 == k) ? z[i] : v;
+      cu[k] = v;
+    }
+    for (int t = lane; t < n; t += 64) {
+      const int p = a.srcIdx[s0 + t];
+      const double q = cu[0] + cu[1] * ((a.sx[p] - c[0]) * sc) +
+                       cu[2] * ((a.sy[p] - c[1]) * sc) +
+                       cu[3] * ((a.sz[p] - c[2]) * sc);
+      contribS[(s0 + t) * F + f] = gamma[(s0 + t) * F + f] + q;
+    }
+  }
+}
+
+// Shepard weight of each member slot of a target (cell order), as the blend
+// applies it; 0 for the slots that are not members.
+__global__ __launch_bounds__(256) void slot_weights_kernel(BlendArgs a,
+                                                           double *wslot) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.M)
+    return;
+  const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
+  const double yx = a.tx[t], yy = a.ty[t], yz = a.tz[t];
+  double sum = 0.0;
+  for (int64_t s = s0; s < s1; ++s) {
+    const int k = a.slotPatch[s];
+    const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
+    sum += wendland<1, double, true>(d, a.rinv);
+  }
+  for (int64_t s = s0; s < s1; ++s) {
+    const int k = a.slotPatch[s];
+    const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
+    wslot[s] = a.slotMember[s] ? wendland<1, double, true>(d, a.rinv) / sum
+                               : 0.0;
+  }
+}
+
+__global__ void src_count_kernel(const int *srcIdx, int64_t total, int *cnt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total)
+    atomicAdd(&cnt[srcIdx[e]], 1);
+}
+
+__global__ void src_fill_kernel(const int *srcIdx, int64_t total,
+                                const int64_t *off, int *cursor, int *ent) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total)
+    return;
+  const int p = srcIdx[e];
+  const int k = atomicAdd(&cursor[p], 1);
+  ent[off[p] + k] = (int)e;
+}
+
+// Entries of each source in ascending order (= ascending patch).
+__global__ void src_sort_kernel(const int64_t *off, int64_t N, int *ent) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N)
+    return;
+  const int64_t b = off[p], e = off[p + 1];
+  for (int64_t i = b + 1; i < e; ++i) {
+    const int v = ent[i];
+    int64_t j = i - 1;
+    while (j >= b && ent[j] > v) {
+      ent[j + 1] = ent[j];
+      --j;
+    }
+    ent[j + 1] = v;
+  }
+}
+
+__global__ void src_reduce_kernel(const int64_t *off, const int *ent,
+                                  int64_t N, int F, const double *contribS,
+                                  const int *perm, double *out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N)
+    return;
+  const int64_t b = off[p], e = off[p + 1];
+  const int64_t o = perm[p];
+  for (int f = 0; f < F; ++f) {
+    double acc = 0.0;
+    for (int64_t i = b; i < e; ++i)
+      acc += contribS[(int64_t)ent[i] * F + f];
+    out[(int64_t)f * N + o] = acc;
+  }
+}
+
+__global__ void zero_i32_kernel(int *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = 0;
+}
+
 } // namespace
 
 int solve_fields_per_launch() { return kFMax; }
@@ -456,6 +643,84 @@ void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
   default:
     throw std::runtime_error("solve: unsupported size bin");
   }
+}
+
+void launch_solve_f64_adjoint(int R, int kind, const SolveArgs &a,
+                              hipStream_t s) {
+  if (a.count <= 0)
+    return;
+  if (a.F < 1 || a.F > kFMax)
+    throw std::runtime_error("solve: 1 <= fields <= 8 per launch");
+  if (kind < 0 || kind > 4)
+    throw std::runtime_error("solve: unknown Wendland kind");
+  switch (R) {
+  case 1: launch_solve_tile_adjoint<1>(kind, a, s); break;
+  case 2: launch_solve_tile_adjoint<2>(kind, a, s); break;
+  case 3: launch_solve_tile_adjoint<3>(kind, a, s); break;
+  case 4: launch_solve_tile_adjoint<4>(kind, a, s); break;
+  case 5: launch_solve_tile_adjoint<5>(kind, a, s); break;
+  case 6: launch_solve_tile_adjoint<6>(kind, a, s); break;
+  case 7: launch_solve_tile_adjoint<7>(kind, a, s); break;
+  case 8: launch_solve_tile_adjoint<8>(kind, a, s); break;
+  case 9: launch_solve_tile_adjoint<9>(kind, a, s); break;
+  case 10: launch_solve_tile_adjoint<10>(kind, a, s); break;
+  case 11: launch_solve_tile_adjoint<11>(kind, a, s); break;
+  case 12: launch_solve_tile_adjoint<12>(kind, a, s); break;
+  case 13: launch_solve_tile_adjoint<13>(kind, a, s); break;
+  default:
+    throw std::runtime_error("solve: unsupported size bin");
+  }
+}
+
+void launch_adjoint_poly(const SolveArgs &a, int64_t K, const double *gamma,
+                         double *contribS, hipStream_t s) {
+  if (K <= 0)
+    return;
+  adjoint_poly_kernel<<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, gamma,
+                                                               contribS);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void launch_slot_weights(const BlendArgs &a, double *wslot, hipStream_t s) {
+  if (a.M <= 0)
+    return;
+  slot_weights_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a, wslot);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void build_source_transpose(const int *srcIdx, int64_t total, int64_t N,
+                            int64_t *srcTOff, int *srcTEnt, hipStream_t s) {
+  DevBuf<int> cnt(std::max<int64_t>(N, 1)), cur(std::max<int64_t>(N, 1));
+  const unsigned bn = (unsigned)((std::max<int64_t>(N, 1) + 255) / 256);
+  zero_i32_kernel<<<bn, 256, 0, s>>>(cnt.get(), N);
+  zero_i32_kernel<<<bn, 256, 0, s>>>(cur.get(), N);
+  PACMAN_HIP_CHECK(hipGetLastError());
+  if (total > 0) {
+    src_count_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+        srcIdx, total, cnt.get());
+    PACMAN_HIP_CHECK(hipGetLastError());
+  }
+  exclusive_scan_i32_to_i64(cnt.get(), srcTOff, N, s);
+  if (total > 0) {
+    src_fill_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+        srcIdx, total, srcTOff, cur.get(), srcTEnt);
+    PACMAN_HIP_CHECK(hipGetLastError());
+  }
+  if (N > 0) {
+    src_sort_kernel<<<bn, 256, 0, s>>>(srcTOff, N, srcTEnt);
+    PACMAN_HIP_CHECK(hipGetLastError());
+  }
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void launch_source_reduce(const int64_t *srcTOff, const int *srcTEnt,
+                          int64_t N, int F, const double *contribS,
+                          const int *perm, double *out, hipStream_t s) {
+  if (N <= 0)
+    return;
+  src_reduce_kernel<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(
+      srcTOff, srcTEnt, N, F, contribS, perm, out);
+  PACMAN_HIP_CHECK(hipGetLastError());
 }
 
 void launch_blend(const BlendArgs &a, hipStream_t s) {

@@ -72,7 +72,10 @@ template <int R> struct TileShape {
   static constexpr int SBUF = SBUF0 > BACK ? SBUF0 : BACK;
 };
 
-template <int R>
+// ADJ: the transpose chain. The right-hand side is E^T h with h = w g at
+// the member targets (instead of the residual b - Q beta), and the kernel
+// stores gamma = A^-1 E^T h per patch source instead of evaluating.
+template <int R, bool ADJ>
 __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
                                                             const double *coef,
                                                             int kind) {
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
   const double dinv = a.dinv;
 
-  if (tid < kFMax * 4)
+  if (!ADJ && tid < kFMax * 4)
     scoef[tid >> 2][tid & 3] =
         (tid >> 2) < F ? coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)]
                        : 0.0;
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     pz[t] = z;
     for (int f = 0; f < F; ++f) {
       double v = 0.0;
-      if (t < n)
+      if (!ADJ && t < n)
         v = a.vals[(int64_t)f * a.N + p] -
             (scoef[f][0] + scoef[f][1] * (x - cx) + scoef[f][2] * (y - cy) +
              scoef[f][3] * (z - cz));
@@ -216,6 +219,77 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
       t4[v] *= sc;
     return t4;
   };
+
+  if constexpr (ADJ) {
+    // right-hand side E^T h, h = w g at the member targets: tiles of 64
+    // pairs staged in LDS; thread (j, half) sums half of each tile for
+    // source j; the halves are added in a fixed order.
+    double *sty = sbuf;                        // [3][64] target coordinates
+    double *sth = sbuf + 3 * 64;               // [kFMax][64] h
+    double *sred = sbuf + (3 + kFMax) * 64;    // [kFMax][256] second halves
+    const int64_t p0 = a.pairOff[patch];
+    const int mall = (int)(a.pairOff[patch + 1] - p0);
+    const int j = tid & 255, half = tid >> 8;
+    double acc[kFMax];
+#pragma unroll
+    for (int f = 0; f < kFMax; ++f)
+      acc[f] = 0.0;
+    const int jc = j < n ? j : 0;
+    const double xj = px[jc], yj = py[jc], zj = pz[jc];
+    for (int tile = 0; tile < mall; tile += 64) {
+      if (tid < 64) {
+        const int e = tile + tid;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0, w = 0.0;
+        int tp = 0;
+        if (e < mall) {
+          const int pe = a.pairTgt[p0 + e];
+          tp = pe & 0x7fffffff;
+          y0 = a.tx[tp];
+          y1 = a.ty[tp];
+          y2 = a.tz[tp];
+          w = a.wslot[a.pairSlot[p0 + e]];  // 0 unless a member slot
+        }
+        sty[tid] = y0;
+        sty[64 + tid] = y1;
+        sty[128 + tid] = y2;
+#pragma unroll
+        for (int f = 0; f < kFMax; ++f)
+          sth[f * 64 + tid] = (f < F && w != 0.0) ? w * a.gSorted[f * a.M + tp]
+                                                  : 0.0;
+      }
+      __syncthreads();
+      if (j < n) {
+        const int q0 = half * 32, q1 = min(q0 + 32, mall - tile);
+        with_kind([&](auto kc) __attribute__((always_inline)) {
+          constexpr int K = decltype(kc)::value;
+          for (int q = q0; q < q1; ++q) {
+            const double ph = wendland<K, double, true>(
+                dist_fast(xj, yj, zj, sty[q], sty[64 + q], sty[128 + q]),
+                dinv);
+#pragma unroll
+            for (int f = 0; f < kFMax; ++f)
+              if (f < F)
+                acc[f] = fma(ph, sth[f * 64 + q], acc[f]);
+          }
+        });
+      }
+      __syncthreads();
+    }
+    if (half == 1 && j < n) {
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f)
+        if (f < F)
+          sred[f * 256 + j] = acc[f];
+    }
+    __syncthreads();
+    if (half == 0 && j < NMAX) {
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f)
+        if (f < F)
+          rhs[f][j] = j < n ? acc[f] + sred[f * 256 + j] : 0.0;
+    }
+    __syncthreads();
+  }
 
   if (wave == 0) {
     // ---------------- control wave: eliminations -------------------------
@@ -489,6 +563,13 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   if (tid == 0 && a.status)
     a.status[patch] = spivot;
 
+  if constexpr (ADJ) {
+    for (int t = tid; t < n; t += NT)
+      for (int f = 0; f < F; ++f)
+        a.gammaOut[(s0 + t) * F + f] = rhs[f][t];
+    return;
+  }
+
   // evaluation at the member targets: tiles of 64 pairs, thread = (pair,
   // slice); slices of ceil(n / 8) sources, four at a time into four partial
   // sums, added in a fixed order. The next tile's targets are loaded while
@@ -591,8 +672,16 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 template <int R>
 void launch_solve_tile(int kind, const SolveArgs &a, const double *coef,
                        hipStream_t s) {
-  solve_tile_kernel<R><<<dim3((unsigned)a.count), dim3(TileShape<R>::NT), 0,
-                         s>>>(a, coef, kind);
+  solve_tile_kernel<R, false><<<dim3((unsigned)a.count),
+                                dim3(TileShape<R>::NT), 0, s>>>(a, coef, kind);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+template <int R>
+void launch_solve_tile_adjoint(int kind, const SolveArgs &a, hipStream_t s) {
+  solve_tile_kernel<R, true><<<dim3((unsigned)a.count),
+                               dim3(TileShape<R>::NT), 0, s>>>(a, nullptr,
+                                                               kind);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 

@@ -14,4 +14,16 @@ template void launch_solve_tile<5>(int, const SolveArgs &, const double *,
                                      hipStream_t);
 template void launch_solve_tile<6>(int, const SolveArgs &, const double *,
                                      hipStream_t);
+template void launch_solve_tile_adjoint<1>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<2>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<3>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<4>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<5>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<6>(int, const SolveArgs &,
+                                             hipStream_t);
 } // namespace pacman_amd

@@ -8,4 +8,10 @@ template void launch_solve_tile<8>(int, const SolveArgs &, const double *,
                                      hipStream_t);
 template void launch_solve_tile<9>(int, const SolveArgs &, const double *,
                                      hipStream_t);
+template void launch_solve_tile_adjoint<7>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<8>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<9>(int, const SolveArgs &,
+                                             hipStream_t);
 } // namespace pacman_amd

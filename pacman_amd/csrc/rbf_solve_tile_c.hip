@@ -6,4 +6,8 @@ template void launch_solve_tile<10>(int, const SolveArgs &, const double *,
                                      hipStream_t);
 template void launch_solve_tile<11>(int, const SolveArgs &, const double *,
                                      hipStream_t);
+template void launch_solve_tile_adjoint<10>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<11>(int, const SolveArgs &,
+                                             hipStream_t);
 } // namespace pacman_amd

@@ -6,4 +6,8 @@ template void launch_solve_tile<12>(int, const SolveArgs &, const double *,
                                      hipStream_t);
 template void launch_solve_tile<13>(int, const SolveArgs &, const double *,
                                      hipStream_t);
+template void launch_solve_tile_adjoint<12>(int, const SolveArgs &,
+                                             hipStream_t);
+template void launch_solve_tile_adjoint<13>(int, const SolveArgs &,
+                                             hipStream_t);
 } // namespace pacman_amd

@@ -183,6 +183,35 @@ class Plan:
             raise RuntimeError(_lib.last_error())
         return out.reshape(-1) if squeeze else out
 
+    def apply_transpose(self, target_grad, out=None, stream=None):
+        """Transpose of apply: (F, M) or (M,) -> (F, N) or (N,).
+
+        The gradient of a loss with respect to the source values, given its
+        gradient with respect to the target values.
+        """
+        g = target_grad
+        squeeze = len(g.shape) == 1
+        F = 1 if squeeze else int(g.shape[0])
+        if hasattr(g, "data_ptr"):
+            import torch
+            g = g.contiguous().to(torch.float64)
+            if out is None:
+                out = torch.empty((F, self.num_sources), dtype=torch.float64,
+                                  device=g.device)
+        else:
+            g = _as_f64(g)
+            if out is None:
+                out = np.zeros((F, self.num_sources))
+        pg, dev = _ptr(g)
+        po, devo = _ptr(out)
+        if dev != devo:
+            raise ValueError("gradients must live in the same place")
+        rc = self._lib.pacman_rbf_plan_apply_transpose(
+            self._h, pg, F, po, dev, ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return out.reshape(-1) if squeeze else out
+
     # -- introspection -----------------------------------------------------
     @property
     def radius(self) -> float:
