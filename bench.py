@@ -46,11 +46,13 @@ def main() -> int:
                     help="fields per transfer (franke_3d plus seeded trig "
                          "fields)")
     ap.add_argument("--mode", default="rbf",
-                    choices=["rbf", "apply", "fe-nearest"],
+                    choices=["rbf", "apply", "adjoint", "fe-nearest"],
                     help="rbf: RBF-PUM setup+apply (default); apply: one "
                          "setup, then --steps applies (repeated coupling "
-                         "transfer); fe-nearest: the nearest-source transfer "
-                         "on the same clouds")
+                         "transfer); adjoint: one setup, then forward + "
+                         "backward (gradient of 0.5 |u - g|^2 with respect to "
+                         "the source values) per step; fe-nearest: the "
+                         "nearest-source transfer on the same clouds")
     ap.add_argument("--graph", action="store_true",
                     help="apply mode: replay the apply as a HIP graph")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -86,6 +88,8 @@ def main() -> int:
         return bench_fe_nearest(args, S, T)
     if args.mode == "apply":
         return bench_apply(args, S, T, npc, support)
+    if args.mode == "adjoint":
+        return bench_adjoint(args, S, T, npc, support)
     v = W.franke_3d(*S.T)
     if args.fields > 1:
         v = np.concatenate([v[None], W.trig_fields(S, args.fields - 1,
@@ -245,6 +249,51 @@ def bench_apply(args, S, T, npc, support) -> int:
                                f"{F} trig fields per apply, graph="
                                f"{bool(args.graph)}",
                    "fields": F, "graph": bool(args.graph)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_adjoint(args, S, T, npc, support) -> int:
+    """Forward + backward through pacman_amd.autograd on one plan."""
+    import numpy as np
+    import torch
+
+    import pacman_amd as pacman
+    from pacman_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(S).to(dev)
+    dT = torch.from_numpy(T).to(dev)
+    f = torch.from_numpy(W.franke_3d(*S.T)).to(dev).requires_grad_()
+    g = torch.from_numpy(
+        np.random.default_rng(8).normal(size=len(T))).to(dev)
+    plan = pacman.rbf.Plan(3, args.rbf, dS, dT, npc, 0.15, support)
+
+    def step():
+        f.grad = None
+        u = pacman.autograd.transfer(plan, f)
+        (0.5 * ((u - g) ** 2).sum()).backward()
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    M = len(T)
+    res = {
+        "metric": "RBF-PUM forward+backward Mpts/s (FP64)",
+        "value": M / (ms / 1e3) / 1e6, "unit": "Mpts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {len(S)} -> {M}, one setup, "
+                               "loss 0.5|u - g|^2, gradient with respect to "
+                               "the source values"},
         "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)
