@@ -50,8 +50,9 @@ def main() -> int:
                     help="rbf: RBF-PUM setup+apply (default); apply: one "
                          "setup, then --steps applies (repeated coupling "
                          "transfer); adjoint: one setup, then forward + "
-                         "backward (gradient of 0.5 |u - g|^2 with respect to "
-                         "the source values) per step; fe-nearest: the "
+                         "backward (gradients of 0.5 |u - g|^2 with respect "
+                         "to the source values and the target coordinates) "
+                         "per step; fe-nearest: the "
                          "nearest-source transfer on the same clouds")
     ap.add_argument("--graph", action="store_true",
                     help="apply mode: replay the apply as a HIP graph")
@@ -270,10 +271,12 @@ def bench_adjoint(args, S, T, npc, support) -> int:
     g = torch.from_numpy(
         np.random.default_rng(8).normal(size=len(T))).to(dev)
     plan = pacman.rbf.Plan(3, args.rbf, dS, dT, npc, 0.15, support)
+    dT.requires_grad_()
 
     def step():
         f.grad = None
-        u = pacman.autograd.transfer(plan, f)
+        dT.grad = None
+        u = pacman.autograd.transfer(plan, f, dT)
         (0.5 * ((u - g) ** 2).sum()).backward()
 
     for _ in range(max(args.warmup, 1)):
@@ -292,8 +295,9 @@ def bench_adjoint(args, S, T, npc, support) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: {len(S)} -> {M}, one setup, "
-                               "loss 0.5|u - g|^2, gradient with respect to "
-                               "the source values"},
+                               "loss 0.5|u - g|^2, gradients with respect to "
+                               "the source values and the target "
+                               "coordinates"},
         "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)

@@ -56,6 +56,28 @@ int pacman_rbf_plan_apply_graph(pacman_rbf_plan plan,
                                 const double *sourceValues, int nFields,
                                 double *targetValues, void *stream);
 
+/* Target-coordinate gradient (no reference counterpart). Doubles of solver
+ * state that pacman_rbf_plan_apply_state saves for nFields fields (the
+ * coefficients of every patch), or -1 on error. */
+long long pacman_rbf_plan_state_size(pacman_rbf_plan plan, int nFields);
+
+/* pacman_rbf_plan_apply (device pointers, asynchronous on `stream`) that also
+ * writes the solver state into `state` (device, state_size doubles). */
+int pacman_rbf_plan_apply_state(pacman_rbf_plan plan,
+                                const double *sourceValues, int nFields,
+                                double *targetValues, double *state,
+                                void *stream);
+
+/* coordGrad[i * spaceDimension + a] = d/dy_ia of
+ * sum_f sum_j targetGrad[f * nTarget + j] u_f(y_j), u the transfer at the
+ * plan's targets with the patch structure held fixed (device pointers,
+ * asynchronous). `state` comes from pacman_rbf_plan_apply_state with the
+ * same nFields and the source values whose transfer is differentiated. */
+int pacman_rbf_plan_target_coord_grad(pacman_rbf_plan plan,
+                                      const double *state, int nFields,
+                                      const double *targetGrad,
+                                      double *coordGrad, void *stream);
+
 int pacman_rbf_plan_destroy(pacman_rbf_plan plan);
 
 /* Introspection (host outputs). */

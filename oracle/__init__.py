@@ -78,6 +78,8 @@ def lib():
             ctypes.c_double, _dp, ctypes.c_int64, _dp, _dp, ctypes.c_int64,
             _dp]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_setup_target_grad.argtypes = [ctypes.c_void_p, _dp,
+                                               ctypes.c_int, _dp, _dp]
         L.oracle_fe_nearest.argtypes = [
             ctypes.c_int, _dp, ctypes.c_int64, _dp, _dp, ctypes.c_int64,
             ctypes.c_int, _dp, _ip]
@@ -210,6 +212,19 @@ class Setup:
                                  _ptr(out), _ptr(pv, _ip))
         out = out[0] if F == 1 and np.ndim(values) == 1 else out
         return (out, pv) if return_pv else out
+
+    def target_grad(self, values, target_grad):
+        """d/dy of sum_f <target_grad_f, u_f> at every target: (nt, dim)."""
+        v = _f64(values)
+        F = 1 if v.ndim == 1 else v.shape[0]
+        v = v.reshape(F, self.ns)
+        g = _f64(target_grad).reshape(F, self.nt)
+        out = np.zeros((self.nt, self.dim))
+        rc = lib().oracle_setup_target_grad(self._h, _ptr(v), F, _ptr(g),
+                                            _ptr(out))
+        if rc != 0:
+            raise ValueError("oracle_setup_target_grad failed")
+        return out
 
 
 def rbf_interpolate(dim, rbf_function, source_points, source_values,

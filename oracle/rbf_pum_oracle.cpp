@@ -100,6 +100,46 @@ inline double wendland(int kind, double r, double rinv) {
   return mask * v;
 }
 
+// d phi / d p of wendland(kind, p, 1) on [0, 1) by the product rule on the
+// closed forms above (0 outside); the kink of C0 at p = 0 is given slope 0.
+// Used only by the target-coordinate gradient, an extension the reference
+// does not have.
+inline double wendlandDp(int kind, double p) {
+  if (!(p >= 0.0 && p < 1.0))
+    return 0.0;
+  const double q = 1.0 - p;
+  switch (kind) {
+  case 0:
+    return p > 0.0 ? -2.0 * q : 0.0;
+  case 1:
+    return -4.0 * std::pow(q, 3) * (4.0 * p + 1.0) + 4.0 * std::pow(q, 4);
+  case 2:
+    return -6.0 * std::pow(q, 5) * (35.0 * p * p + 18.0 * p + 3.0) +
+           std::pow(q, 6) * (70.0 * p + 18.0);
+  case 3:
+    return -8.0 * std::pow(q, 7) *
+               (32.0 * p * p * p + 25.0 * p * p + 8.0 * p + 1.0) +
+           std::pow(q, 8) * (96.0 * p * p + 50.0 * p + 8.0);
+  case 4:
+    return -10.0 * std::pow(q, 9) *
+               (1287.0 * std::pow(p, 4) + 1350.0 * std::pow(p, 3) +
+                630.0 * p * p + 150.0 * p + 15.0) +
+           std::pow(q, 10) *
+               (5148.0 * std::pow(p, 3) + 4050.0 * p * p + 1260.0 * p + 150.0);
+  default:
+    return std::numeric_limits<double>::quiet_NaN();
+  }
+}
+
+// grad_y of wendland(kind, |y - x|, rinv): phi'(p) rinv (y - x) / |y - x|.
+inline void wendlandGrad(int kind, const double *y, const double *x,
+                         double rinv, double *g) {
+  const double d = dist(y, x);
+  const double c = d > 0.0 ? wendlandDp(kind, d * rinv) * rinv / d : 0.0;
+  for (int a = 0; a < 3; ++a)
+    g[a] = c * (y[a] - x[a]);
+}
+
 // ---------------------------------------------------------------------------
 // Uniform cell grid used only to accelerate exact sphere / nearest queries.
 // Every query visits all points whose cells overlap the query cube (padded),
@@ -665,6 +705,8 @@ void runSetup(const Params &prm, const std::vector<double> &S, i64 ns,
 struct PatchOut {
   std::vector<double> local; // tgtIdx.size() x F
   std::vector<int> pv;       // active polynomial columns per patch
+  bool wantGrad = false;     // also the gradient of each local value
+  std::vector<double> grad;  // tgtIdx.size() x F x 3 (d/dy of local)
 };
 
 void solvePatches(const Params &prm, const std::vector<double> &S,
@@ -674,6 +716,8 @@ void solvePatches(const Params &prm, const std::vector<double> &S,
   const int D = prm.dim;
   const i64 K = (i64)st.centers.size() / 3;
   po.local.assign(st.tgtIdx.size() * (size_t)F, 0.0);
+  if (po.wantGrad)
+    po.grad.assign(st.tgtIdx.size() * (size_t)F * 3, 0.0);
   po.pv.assign(K, 0);
   const double dinv = 1.0 / delta;
 #pragma omp parallel for schedule(dynamic, 1)
@@ -770,6 +814,21 @@ void solvePatches(const Params &prm, const std::vector<double> &S,
           if (active[a])
             pvv += y[a] * beta[c++];
         po.local[(size_t)(t0 + j) * F + f] = t + pvv;
+        if (po.wantGrad) {
+          // d/dy of E(y) alpha + V(y) beta with the raw basis [1, y_a]
+          double gr[3] = {0.0, 0.0, 0.0}, gi[3];
+          for (i64 i = 0; i < n; ++i) {
+            wendlandGrad(prm.kind, y, &S[3 * src[i]], dinv, gi);
+            for (int a = 0; a < 3; ++a)
+              gr[a] += gi[a] * b[i];
+          }
+          int cc = 1;
+          for (int a = 0; a < D; ++a)
+            if (active[a])
+              gr[a] += beta[cc++];
+          for (int a = 0; a < 3; ++a)
+            po.grad[((size_t)(t0 + j) * F + f) * 3 + a] = gr[a];
+        }
       }
     }
   }
@@ -798,6 +857,52 @@ void blend(const Setup &st, const PatchOut &po, i64 nt, int F, double *out) {
       }
       out[(size_t)f * nt + t] = acc;
     }
+  }
+}
+
+// Target-coordinate gradient of sum_f sum_t g_f(t) u_f(t) for a fixed
+// setup: u(y) = sum_k w_k(y) t_k(y) with w_k = psi_k / S and S the sum of
+// psi over every centre within r of y (solver.hpp:128-169, psi = Wendland C2
+// over the patch radius), the sum over k over the patches holding y as a
+// member (solver.hpp:275-285). out: nt x dim.
+void blendGrad(const Setup &st, const PatchOut &po, const std::vector<double> &T,
+               i64 nt, int F, const double *g, int dim, double *out) {
+  const i64 K = (i64)st.centers.size() / 3;
+  const double rinv = 1.0 / st.radius;
+  std::vector<std::vector<std::pair<i64, i64>>> byTarget(nt);
+  for (i64 k = 0; k < K; ++k)
+    for (i64 j = st.tgtOff[k]; j < st.tgtOff[k + 1]; ++j)
+      byTarget[st.tgtIdx[j]].push_back({k, j});
+#pragma omp parallel for schedule(static)
+  for (i64 t = 0; t < nt; ++t) {
+    const double *y = &T[3 * t];
+    double Ssum = 0.0, dS[3] = {0.0, 0.0, 0.0}, gi[3];
+    for (i64 e = st.wOff[t]; e < st.wOff[t + 1]; ++e) {
+      const double *c = &st.centers[3 * st.wIdx[e]];
+      Ssum += wendland(1, dist(y, c), rinv);
+      wendlandGrad(1, y, c, rinv, gi);
+      for (int a = 0; a < 3; ++a)
+        dS[a] += gi[a];
+    }
+    auto &lst = byTarget[t];
+    std::sort(lst.begin(), lst.end());
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (auto &kj : lst) {
+      const double *c = &st.centers[3 * kj.first];
+      const double w = wendland(1, dist(y, c), rinv) / Ssum;
+      wendlandGrad(1, y, c, rinv, gi);
+      for (int f = 0; f < F; ++f) {
+        const double gf = g[(size_t)f * nt + t];
+        const double loc = po.local[(size_t)kj.second * F + f];
+        for (int a = 0; a < 3; ++a) {
+          const double dw = (gi[a] - w * dS[a]) / Ssum;
+          acc[a] += gf * (dw * loc +
+                          w * po.grad[((size_t)kj.second * F + f) * 3 + a]);
+        }
+      }
+    }
+    for (int a = 0; a < dim; ++a)
+      out[(size_t)t * dim + a] = acc[a];
   }
 }
 
@@ -956,6 +1061,22 @@ int oracle_setup_apply(void *h, const double *vals, int F, int64_t kBegin,
   blend(o->st, po, o->nt, F, out);
   if (pvOut)
     std::copy(po.pv.begin(), po.pv.end(), pvOut);
+  return 0;
+}
+
+// Gradient with respect to the target coordinates (nt x dim) of
+// sum_f <g_f, u_f>, u = the transfer of vals (F x ns), g: F x nt.
+int oracle_setup_target_grad(void *h, const double *vals, int F,
+                             const double *g, double *out) {
+  auto *o = static_cast<OracleSetup *>(h);
+  if (F < 1)
+    return -1;
+  const i64 K = (i64)o->st.centers.size() / 3;
+  const double delta = o->prm.support > 0 ? o->prm.support : o->st.radius;
+  PatchOut po;
+  po.wantGrad = true;
+  solvePatches(o->prm, o->S, vals, o->ns, F, o->T, o->st, delta, 0, K, po);
+  blendGrad(o->st, po, o->T, o->nt, F, g, o->prm.dim, out);
   return 0;
 }
 

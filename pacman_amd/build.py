@@ -8,8 +8,8 @@ contraction policy:
   ``-ffp-contract=off`` - distances, lattice coordinates and probe points
   must round exactly as written so the patch membership and nearest
   neighbours are bit-identical to the CPU restatement;
-* rbf_solve.hip and the size-bin units rbf_solve_tile_a.hip ...
-  rbf_solve_tile_d.hip: FMA contraction allowed (dense arithmetic).
+* rbf_solve.hip, rbf_grad.hip and the size-bin units rbf_solve_tile_a.hip
+  ... rbf_solve_tile_d.hip: FMA contraction allowed (dense arithmetic).
 
 All HIP code is compiled with the packed-FP32 instructions disabled.
 """
@@ -40,6 +40,7 @@ UNITS = {
     "capi.hip": ["-ffp-contract=off"],
     "fe_nearest.hip": ["-ffp-contract=off"],
     "rbf_solve.hip": [],
+    "rbf_grad.hip": [],
     "rbf_solve_tile_a.hip": [],
     "rbf_solve_tile_b.hip": [],
     "rbf_solve_tile_c.hip": [],

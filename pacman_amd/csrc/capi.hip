@@ -489,6 +489,64 @@ int pacman_rbf_plan_apply_graph(pacman_rbf_plan plan,
   }
 }
 
+long long pacman_rbf_plan_state_size(pacman_rbf_plan plan, int nFields) {
+  if (!plan || !plan->plan || nFields < 1) {
+    set_error("pacman_rbf_plan_state_size", "null plan or nFields < 1");
+    return -1;
+  }
+  return plan->plan->stateSize(nFields);
+}
+
+int pacman_rbf_plan_apply_state(pacman_rbf_plan plan,
+                                const double *sourceValues, int nFields,
+                                double *targetValues, double *state,
+                                void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    RbfPumPlan &P = *plan->plan;
+    if (nFields < 1)
+      throw std::invalid_argument("nFields must be >= 1");
+    if (sourceValues == nullptr ||
+        (targetValues == nullptr && P.numTargets() > 0))
+      throw std::invalid_argument("null values pointer");
+    P.applyState(sourceValues, nFields, targetValues, state,
+                 (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_apply_state", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_apply_state", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_rbf_plan_target_coord_grad(pacman_rbf_plan plan,
+                                      const double *state, int nFields,
+                                      const double *targetGrad,
+                                      double *coordGrad, void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    RbfPumPlan &P = *plan->plan;
+    if (nFields < 1)
+      throw std::invalid_argument("nFields must be >= 1");
+    if (P.numTargets() > 0 &&
+        (state == nullptr || targetGrad == nullptr || coordGrad == nullptr))
+      throw std::invalid_argument("null state or gradient pointer");
+    P.targetCoordGrad(state, nFields, targetGrad, coordGrad,
+                      (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_target_coord_grad", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_target_coord_grad", "unknown exception");
+    return -2;
+  }
+}
+
 int pacman_rbf_plan_destroy(pacman_rbf_plan plan) {
   delete plan;
   return 0;

@@ -22,6 +22,9 @@
 
 namespace pacman_amd {
 
+// Largest patch (sources) the register-resident solve takes.
+constexpr int kMaxPatch = 192;
+
 // Owning device buffer (hipMalloc / hipFree).
 template <typename T> struct DevBuf {
   T *p = nullptr;
@@ -144,6 +147,7 @@ struct SolveArgs {
   int dim;
   int *pvOut;               // K (optional) active polynomial columns
   int *status;              // K (optional) nonzero on a zero pivot
+  double *alphaOut;         // (sum n) x F (optional): alpha per patch source
   // transpose (adjoint) apply only
   const double *wslot;      // P: Shepard weight of each member slot, else 0
   const double *gSorted;    // F x M upstream values, target cell order
@@ -210,5 +214,19 @@ void fe_nearest_transfer(int dim, const double *dSrc, int64_t N,
 
 void launch_gather_values(const double *vals, const int *perm, int64_t N,
                           int F, double *out, hipStream_t s);
+
+// Gradient with respect to the target coordinates (rbf_grad.hip).
+// Per member pair of patch k (one workgroup per patch), with alpha/coef the
+// saved solution of the forward apply and gSorted the upstream values
+// (F x M, target cell order): gslot[4 s + 0] = sum_f g_f t_kf(y),
+// gslot[4 s + 1 + a] = sum_f g_f d t_kf / d y_a at the pair's slot s;
+// accumulate adds to gslot instead of overwriting it.
+void launch_target_grad_eval(const SolveArgs &a, int64_t K, int kind,
+                             const double *alpha, const double *coef,
+                             double *gslot, bool accumulate, hipStream_t s);
+// Per target: d/dy of sum_k w_k(y) (slot values), with the Shepard weights'
+// derivative; coordGrad[o * dim + a] in original target order.
+void launch_target_grad_blend(const BlendArgs &a, const double *gslot,
+                              int dim, double *coordGrad, hipStream_t s);
 
 } // namespace pacman_amd

@@ -15,8 +15,6 @@
 namespace pacman_amd {
 
 namespace {
-// Largest patch the register-resident solve takes.
-constexpr int kMaxPatch = 192;
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(
@@ -159,6 +157,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   patchN_.resize(K_);
   for (int64_t kk = 0; kk < K_; ++kk)
     patchN_[kk] = (int)(off[kk + 1] - off[kk]);
+  totalSrc_ = off[K_];
   bins_.resize(solve_fields_per_launch());
   bins_for(1, s);
   pv_.alloc(std::max<int64_t>(K_, 1));
@@ -433,8 +432,54 @@ void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
   }
 }
 
+int64_t RbfPumPlan::stateSize(int F) const {
+  return (int64_t)std::max(F, 0) * (totalSrc_ + 4 * K_);
+}
+
+void RbfPumPlan::applyState(const double *dVals, int F, double *dOut,
+                            double *dState, hipStream_t s) {
+  if (M_ == 0)
+    return;
+  if (dState == nullptr && stateSize(F) > 0)
+    throw std::invalid_argument("null state buffer");
+  prepare(F, s);
+  enqueue(dVals, F, dOut, s, true, dState);
+  applied_ = true;
+}
+
+// Per chunk of nf fields (as the apply solved them): slot sums of g times the
+// local value and of g times its gradient, accumulated over the chunks, then
+// one blend with the derivative of the Shepard weights.
+void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
+                                 double *dCoordGrad, hipStream_t s) {
+  if (F < 1)
+    throw std::invalid_argument("target gradient needs at least one field");
+  if (M_ == 0)
+    return;
+  const int64_t total = totalSrc_;
+  const int perLaunch = solve_fields_per_launch();
+  const int Fc = std::min(F, perLaunch);
+  if (gSorted_.n < (size_t)(Fc * M_))
+    gSorted_.alloc((size_t)Fc * M_);
+  if (gslot_.n < (size_t)(4 * std::max<int64_t>(pairs_.P, 1)))
+    gslot_.alloc((size_t)4 * std::max<int64_t>(pairs_.P, 1));
+  for (int f0 = 0; f0 < F; f0 += perLaunch) {
+    const int nf = std::min(perLaunch, F - f0);
+    launch_gather_values(dG + (int64_t)f0 * M_, gt_.view.perm, M_, nf,
+                         gSorted_.get(), s);
+    SolveArgs a = solveArgs(nf, nullptr);
+    a.gSorted = gSorted_.get();
+    launch_target_grad_eval(a, K_, prm_.kind, dState + (int64_t)f0 * total,
+                            dState + (int64_t)F * total + (int64_t)f0 * K_ * 4,
+                            gslot_.get(), f0 > 0, s);
+  }
+  launch_target_grad_blend(blendArgs(1), gslot_.get(), prm_.dim, dCoordGrad,
+                           s);
+}
+
 void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
-                         hipStream_t s, bool events) {
+                         hipStream_t s, bool events, double *dState) {
+  const int64_t total = totalSrc_;
   const int perLaunch = solve_fields_per_launch();
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[0], s));
@@ -448,6 +493,13 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     launch_poly(a, K_, coef_.get(), s);
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
+    if (dState) {
+      a.alphaOut = dState + (int64_t)f0 * total;
+      if (K_ > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(
+            dState + (int64_t)F * total + (int64_t)f0 * K_ * 4, coef_.get(),
+            sizeof(double) * (size_t)K_ * nf * 4, hipMemcpyDeviceToDevice, s));
+    }
     const Bins &B = bins_[nf - 1];
     for (size_t b = 0; b < B.R.size(); ++b) {
       a.patches = B.dev[b].get();

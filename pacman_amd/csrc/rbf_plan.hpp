@@ -33,6 +33,18 @@ public:
   // Transpose of apply (the gradient with respect to the source values):
   // dG: F x M (original target order), dGrad: F x N (original source order).
   void applyTranspose(const double *dG, int F, double *dGrad, hipStream_t s);
+  // Solver state an apply saves for the target-coordinate gradient: per
+  // chunk of fields solved together, the coefficients alpha of every patch
+  // source entry, then the polynomial coefficients of every patch.
+  int64_t stateSize(int F) const;
+  // apply that also writes the state (device buffer of stateSize(F)).
+  void applyState(const double *dVals, int F, double *dOut, double *dState,
+                  hipStream_t s);
+  // Gradient of sum_f <dG_f, u_f> with respect to the target coordinates at
+  // the plan's targets, the patch structure held fixed: dCoordGrad is M x dim
+  // (original order, row-major); dState from applyState with the same F.
+  void targetCoordGrad(const double *dState, int F, const double *dG,
+                       double *dCoordGrad, hipStream_t s);
 
   const PlanParams &params() const { return prm_; }
   int64_t numSources() const { return N_; }
@@ -89,12 +101,12 @@ private:
   const Bins &bins_for(int F, hipStream_t s);
   void prepare(int F, hipStream_t s);
   void enqueue(const double *dVals, int F, double *dOut, hipStream_t s,
-               bool events);
+               bool events, double *dState = nullptr);
   void releaseGraph();
   SolveArgs solveArgs(int F, const double *vals) const;
   BlendArgs blendArgs(int F) const;
   bool adjReady_ = false;
-  DevBuf<double> wslot_, gSorted_, gamma_, contribS_;
+  DevBuf<double> wslot_, gSorted_, gamma_, contribS_, gslot_;
   DevBuf<int64_t> srcTOff_;
   DevBuf<int> srcTEnt_;
   hipStream_t gstream_ = nullptr;
@@ -105,6 +117,7 @@ private:
   double *gOut_ = nullptr;
   int gF_ = 0;
   int maxN_ = 0;
+  int64_t totalSrc_ = 0;  // sum over patches of the source counts
   DevBuf<int> pv_, status_;
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};

@@ -568,6 +568,11 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
       for (int f = 0; f < F; ++f)
         a.gammaOut[(s0 + t) * F + f] = rhs[f][t];
     return;
+  } else {
+    if (a.alphaOut)  // saved for the target-coordinate gradient
+      for (int t = tid; t < n; t += NT)
+        for (int f = 0; f < F; ++f)
+          a.alphaOut[(s0 + t) * F + f] = rhs[f][t];
   }
 
   // evaluation at the member targets: tiles of 64 pairs, thread = (pair,

@@ -212,6 +212,52 @@ class Plan:
             raise RuntimeError(_lib.last_error())
         return out.reshape(-1) if squeeze else out
 
+    def apply_with_state(self, source_values, stream=None):
+        """apply() that also returns the solver state the target-coordinate
+        gradient needs: (values (F, M) or (M,), state). Device tensors only
+        (the state is a float64 tensor on the same device)."""
+        import torch
+        v = source_values
+        if not (hasattr(v, "data_ptr") and v.is_cuda):
+            raise ValueError("apply_with_state needs device tensors")
+        squeeze = v.dim() == 1
+        F = 1 if squeeze else int(v.shape[0])
+        v = v.contiguous().to(torch.float64)
+        out = torch.empty((F, self.num_targets), dtype=torch.float64,
+                          device=v.device)
+        n = int(self._lib.pacman_rbf_plan_state_size(self._h, F))
+        if n < 0:
+            raise RuntimeError(_lib.last_error())
+        state = torch.empty(max(n, 1), dtype=torch.float64, device=v.device)
+        rc = self._lib.pacman_rbf_plan_apply_state(
+            self._h, v.data_ptr(), F, out.data_ptr(), state.data_ptr(),
+            ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return (out.reshape(-1) if squeeze else out), state
+
+    def target_coord_grad(self, state, target_grad, stream=None):
+        """Gradient of sum <target_grad, u> with respect to the target
+        coordinates (M, dim), the patch structure held fixed; `state` from
+        apply_with_state with the same field count. Device tensors only."""
+        import torch
+        g = target_grad
+        if not (hasattr(g, "data_ptr") and g.is_cuda):
+            raise ValueError("target_coord_grad needs device tensors")
+        F = 1 if g.dim() == 1 else int(g.shape[0])
+        g = g.contiguous().to(torch.float64)
+        n = int(self._lib.pacman_rbf_plan_state_size(self._h, F))
+        if state.numel() < n:
+            raise ValueError("state is smaller than this field count needs")
+        out = torch.empty((self.num_targets, self.dim), dtype=torch.float64,
+                          device=g.device)
+        rc = self._lib.pacman_rbf_plan_target_coord_grad(
+            self._h, state.data_ptr(), F, g.data_ptr(), out.data_ptr(),
+            ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return out
+
     # -- introspection -----------------------------------------------------
     @property
     def radius(self) -> float:

@@ -352,3 +352,52 @@ def test_fe_nearest_same_points_and_empty(oracle_mod):
     np.testing.assert_array_equal(out, v)
     out0, st0 = oracle_mod.fe_nearest(3, np.zeros((0, 3)), np.zeros(0), S)
     assert not out0.any() and not st0.any()
+
+
+@pytest.mark.parametrize("rbf,dim", [(0x10, 3), (0x11, 3), (0x12, 3),
+                                     (0x13, 2), (0x14, 3)])
+def test_target_grad_matches_finite_differences(oracle_mod, rbf, dim):
+    """The oracle's analytic target-coordinate gradient against central
+    differences of its own forward transfer (a new setup per perturbed
+    target set, step 1e-6): pins the Wendland derivatives, the polynomial
+    part and the derivative of the Shepard weights."""
+    rng = np.random.default_rng(40 + rbf + dim)
+    S = rng.uniform(0, 1, (500, dim))
+    T = rng.uniform(0.02, 0.98, (400, dim))
+    f = np.stack([np.sin(3 * S[:, 0]) + S[:, -1] ** 2,
+                  np.cos(2 * S.sum(1))])
+    g = rng.normal(size=(2, len(T)))
+    args = (40, 0.15, -1.0)
+    base = oracle_mod.Setup(dim, rbf, S, T, *args)
+    grad = base.target_grad(f, g)
+    h = 1e-6
+    picks = rng.choice(len(T) * dim, 40, replace=False)
+    fd = np.empty(len(picks))
+    for i, e in enumerate(picks):
+        t, a = divmod(int(e), dim)
+        val = []
+        for sgn in (1.0, -1.0):
+            Tp = T.copy()
+            Tp[t, a] += sgn * h
+            o = oracle_mod.Setup(dim, rbf, S, Tp, *args)
+            assert o.num_patches == base.num_patches
+            val.append(float(np.sum(g[:, t] * o.apply(f)[:, t])))
+        fd[i] = (val[0] - val[1]) / (2 * h)
+    ref = grad.reshape(-1)[picks]
+    err = np.abs(fd - ref).max() / np.abs(ref).max()
+    assert err < 1e-6, err
+
+
+def test_target_grad_of_linear_field_is_its_slope(oracle_mod):
+    """A linear field is reproduced exactly, so d/dy of <g, u> is g times the
+    slope at every target covered by a patch."""
+    rng = np.random.default_rng(9)
+    S = rng.uniform(0, 1, (600, 3))
+    T = rng.uniform(0.05, 0.95, (300, 3))
+    slope = np.array([0.7, -1.3, 2.1])
+    f = 0.4 + S @ slope
+    g = rng.normal(size=len(T))
+    o = oracle_mod.Setup(3, 0x11, S, T, 40, 0.15, -1.0)
+    grad = o.target_grad(f, g)
+    assert np.allclose(grad, g[:, None] * slope[None, :], rtol=0,
+                       atol=1e-9 * np.abs(g).max() * 3)
