@@ -46,13 +46,14 @@ def main() -> int:
                     help="fields per transfer (franke_3d plus seeded trig "
                          "fields)")
     ap.add_argument("--mode", default="rbf",
-                    choices=["rbf", "apply", "adjoint", "fe-nearest"],
+                    choices=["rbf", "apply", "adjoint", "fe-nearest", "mls"],
                     help="rbf: RBF-PUM setup+apply (default); apply: one "
                          "setup, then --steps applies (repeated coupling "
                          "transfer); adjoint: one setup, then forward + "
                          "backward (gradients of 0.5 |u - g|^2 with respect "
                          "to the source values and the target coordinates) "
-                         "per step; fe-nearest: the "
+                         "per step; mls: the moving-least-squares transfer "
+                         "(setup + apply) on the same clouds; fe-nearest: the "
                          "nearest-source transfer on the same clouds")
     ap.add_argument("--graph", action="store_true",
                     help="apply mode: replay the apply as a HIP graph")
@@ -91,6 +92,8 @@ def main() -> int:
         return bench_apply(args, S, T, npc, support)
     if args.mode == "adjoint":
         return bench_adjoint(args, S, T, npc, support)
+    if args.mode == "mls":
+        return bench_mls(args, S, T)
     v = W.franke_3d(*S.T)
     if args.fields > 1:
         v = np.concatenate([v[None], W.trig_fields(S, args.fields - 1,
@@ -298,6 +301,70 @@ def bench_adjoint(args, S, T, npc, support) -> int:
                                "loss 0.5|u - g|^2, gradients with respect to "
                                "the source values and the target "
                                "coordinates"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_mls(args, S, T) -> int:
+    """Moving-least-squares transfer, inputs in HBM.
+
+    A step = plan setup (grids, k = 10 nearest sources and the QR weights of
+    every target) + one apply. Compulsory HBM bytes: sources 24 B read + the
+    sorted copy (24 B + 4 B) written, value 8 B read; targets 24 B read, the
+    k indices and weights (12 B each) written and read back, 8 B out.
+    """
+    import torch
+
+    import pacman_amd as pacman
+    from pacman_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    dS = torch.from_numpy(S).to(dev)
+    dT = torch.from_numpy(T).to(dev)
+    dv = torch.from_numpy(W.franke_3d(*S.T)).to(dev)
+    N, M = len(S), len(T)
+
+    def step():
+        plan = pacman.mls.Plan(3, dS, dT)
+        return plan, plan.apply(dv)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan, out = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # apply alone (the plan kept)
+    torch.cuda.synchronize()
+    a0 = time.perf_counter()
+    for _ in range(10):
+        plan.apply(dv)
+    torch.cuda.synchronize()
+    apply_ms = (time.perf_counter() - a0) * 1e3 / 10
+    ref = W.franke_3d(*T.T)
+    u = out.cpu().numpy()
+    byts = N * (24 + 28 + 8) + M * (24 + 2 * 10 * 12 + 8)
+    gbs = byts / (ms / 1e3) / 1e9
+    res = {
+        "metric": "MLS transfer setup+apply Mpts/s (FP64)",
+        "value": M / (ms / 1e3) / 1e6, "unit": "Mpts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config} clouds: {N} jittered-grid "
+                               f"sources -> {M} uniform targets, degree-2 "
+                               "MLS over the 10 nearest sources",
+                   "apply_ms": apply_ms,
+                   "rel_l2_vs_franke_3d": float(
+                       ((u - ref) ** 2).sum() ** 0.5 / (ref ** 2).sum() ** 0.5)},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                     "traffic": None, "kernel": "grid build + mls_coef_kernel "
+                     "+ mls_apply_kernel, compulsory bytes"},
         "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)

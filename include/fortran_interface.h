@@ -6,7 +6,7 @@
  * Fortran module (src/pacman_fortran.f90:97-113, 215-235) and any C caller
  * bind to this library unchanged. Of the FE entry point only the method
  * NEAREST_NEAREST is served (it shares the RBF path's point search); the
- * other FE methods and MLS return an error code.
+ * other FE methods return an error code. The MLS entry point is served.
  */
 #ifndef PACMAN_AMD_FORTRAN_INTERFACE_H
 #define PACMAN_AMD_FORTRAN_INTERFACE_H
@@ -87,6 +87,18 @@ int pacman_fe_interpolate_c(int spaceDimension, int execSpace, int method,
                             int connOffSize, const int *cellTypes,
                             const double *targetPoints, int nTargetPoints,
                             double *targetValues, int *targetStatus);
+
+/* Replaces pacman_mls_interpolate_c (src/fortran_interface.h:175-179,
+ * src/fortran_interface.cpp:135-164): moving least squares with the
+ * reference's defaults (degree-2 basis, the basis-size nearest sources,
+ * Wendland C0 weights over 1.1 times the farthest neighbour's distance).
+ * Caller-allocated output of length nTargetPoints. Returns 0, -1 on an error
+ * (message in pacman_rbf_last_error), -2 on an unknown failure. */
+int pacman_mls_interpolate_c(int spaceDimension, int execSpace,
+                             const double *sourcePoints, int nSourcePoints,
+                             const double *sourceValues,
+                             const double *targetPoints, int nTargetPoints,
+                             double *targetValues);
 
 #ifdef __cplusplus
 }

@@ -3,7 +3,8 @@
 // Same namespace, names, constants and result types as the reference's flat
 // C++ API for this path (src/interface.hpp:25-40, 76-81, 107-110, 132-136;
 // src/common/execspaces.hpp:17-24; src/common/types.hpp:29-45). The RBF-PUM
-// entry is complete; fe_interpolate serves the method NEAREST_NEAREST.
+// and MLS entries are complete; fe_interpolate serves the method
+// NEAREST_NEAREST.
 #pragma once
 
 #include <cstdint>
@@ -92,5 +93,22 @@ FeInterpolateResult fe_interpolate(int_t spaceDimension,
                                    coordinates_t *targetPoints,
                                    int_t nTargetPoints,
                                    bool fortranIndexing = false);
+
+/// Replaces PACMAN::MLSInterpolateResult (src/interface.hpp:143-146).
+struct MLSInterpolateResult {
+  std::vector<fp_t> targetValues;  ///< Interpolated value per target point.
+};
+
+/// Replaces PACMAN::MLS_interpolate (src/interface.hpp:166-169,
+/// src/mls_interface.cpp:77-106): moving least squares with the reference's
+/// defaults (degree-2 basis, the basis-size nearest sources, Wendland C0
+/// weights). Same null-pointer and dimension errors; an execution space other
+/// than HIP throws std::invalid_argument instead of calling exit(1).
+MLSInterpolateResult MLS_interpolate(int_t spaceDimension,
+                                     unsigned char execSpace,
+                                     coordinates_t *sourcePoints,
+                                     int_t nSourcePoints, fp_t *sourceValues,
+                                     coordinates_t *targetPoints,
+                                     int_t nTargetPoints);
 
 } // namespace PACMAN

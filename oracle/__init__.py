@@ -4,8 +4,9 @@ This package is the checker. Only ``tests/``, ``__graft_entry__.smoke()`` and
 the ``cpu_baseline`` leg of ``bench.py`` may import it. The product library
 (``pacman_amd``) never loads it.
 
-The C++ restatement lives in ``oracle/rbf_pum_oracle.cpp`` and follows
-``src/rbf_pum/**`` of the reference (file:line citations in that file). This
+The C++ restatements live in ``oracle/rbf_pum_oracle.cpp`` (follows
+``src/rbf_pum/**`` of the reference, file:line citations in that file) and
+``oracle/mls_oracle.cpp`` (the MLS transfer of ``src/mls/``). This
 module is a thin ctypes wrapper plus the reference's analytic test field.
 """
 
@@ -78,6 +79,9 @@ def lib():
             ctypes.c_double, _dp, ctypes.c_int64, _dp, _dp, ctypes.c_int64,
             _dp]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_mls_coefficients.argtypes = [
+            ctypes.c_int, _dp, ctypes.c_int64, _dp, ctypes.c_int64, _i64p,
+            _dp]
         L.oracle_setup_target_grad.argtypes = [ctypes.c_void_p, _dp,
                                                ctypes.c_int, _dp, _dp]
         L.oracle_fe_nearest.argtypes = [
@@ -243,6 +247,29 @@ def rbf_interpolate(dim, rbf_function, source_points, source_values,
     return out
 
 
+def mls_coefficients(dim, source_points, target_points):
+    """k nearest sources of every target and their MLS coefficients:
+    (idx (nt, k) int64, coef (nt, k))."""
+    S = _f64(source_points).reshape(-1, dim)
+    T = _f64(target_points).reshape(-1, dim)
+    k = (dim + 1) * (dim + 2) // 2
+    idx = np.zeros((len(T), k), np.int64)
+    coef = np.zeros((len(T), k))
+    rc = lib().oracle_mls_coefficients(int(dim), _ptr(S), len(S), _ptr(T),
+                                       len(T), _ptr(idx, _i64p), _ptr(coef))
+    if rc != k:
+        raise ValueError("oracle_mls_coefficients: bad dimension or fewer "
+                         "sources than the basis size")
+    return idx, coef
+
+
+def mls_interpolate(dim, source_points, source_values, target_points):
+    """Oracle counterpart of pacman.MLS.interpolate (one or more fields)."""
+    idx, coef = mls_coefficients(dim, source_points, target_points)
+    v = _f64(source_values)
+    return np.sum(coef * v[..., idx], axis=-1)
+
+
 def fe_nearest(dim, source_points, source_values, target_points,
                brute=False):
     """Oracle of the FE NEAREST_NEAREST transfer: (values, status)."""
@@ -257,6 +284,26 @@ def fe_nearest(dim, source_points, source_values, target_points,
     if rc != 0:
         raise ValueError("oracle rejected the parameters")
     return out, st
+
+
+def franke_1d(x):
+    """Franke-1D test field (tests/franke_functions.py:29-69; that file's
+    second definition of franke_1d, the one in effect)."""
+    x = np.asarray(x, dtype=np.float64)
+    return (0.75 * np.exp(-((9 * x - 2) ** 2) / 4)
+            + 0.75 * np.exp(-((9 * x + 1) ** 2) / 49)
+            + 0.5 * np.exp(-((9 * x - 7) ** 2) / 4)
+            - 0.2 * np.exp(-((9 * x - 4) ** 2)))
+
+
+def franke_2d(x, y):
+    """Franke-2D test field (tests/franke_functions.py:71-116)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    return (0.75 * np.exp(-((9 * x - 2) ** 2 + (9 * y - 2) ** 2) / 4)
+            + 0.75 * np.exp(-((9 * x + 1) ** 2) / 49 - (9 * y + 1) / 10)
+            + 0.5 * np.exp(-((9 * x - 7) ** 2 + (9 * y - 3) ** 2) / 4)
+            - 0.2 * np.exp(-((9 * x - 4) ** 2) - (9 * y - 7) ** 2))
 
 
 def franke_3d(x, y, z):

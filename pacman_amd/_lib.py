@@ -62,6 +62,17 @@ SIGNATURES = {
     "pacman_fe_nearest_device": (ctypes.c_int, [
         ctypes.c_int, _vp, ctypes.c_longlong, _vp, _vp, ctypes.c_longlong,
         _vp, _vp, _vp]),
+    "pacman_mls_interpolate_c": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp,
+        ctypes.c_int, _vp]),
+    "pacman_mls_plan_create": (ctypes.c_int, [
+        _vp, ctypes.c_int, _vp, ctypes.c_longlong, _vp, ctypes.c_longlong,
+        ctypes.c_int, _vp]),
+    "pacman_mls_plan_apply": (ctypes.c_int, [
+        _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
+    "pacman_mls_plan_k": (ctypes.c_int, [_vp]),
+    "pacman_mls_plan_neighbours": (ctypes.c_int, [_vp, _vp, _vp]),
+    "pacman_mls_plan_destroy": (ctypes.c_int, [_vp]),
     "pacman_rbf_last_error": (ctypes.c_char_p, []),
 }
 

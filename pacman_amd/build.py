@@ -4,7 +4,7 @@ Run as ``python -m pacman_amd.build`` or through ``__graft_entry__.build()``.
 Each translation unit is compiled by hipcc with its own floating-point
 contraction policy:
 
-* rbf_geometry.hip, rbf_plan.hip, capi.hip, fe_nearest.hip:
+* rbf_geometry.hip, rbf_plan.hip, capi.hip, fe_nearest.hip, mls.hip:
   ``-ffp-contract=off`` - distances, lattice coordinates and probe points
   must round exactly as written so the patch membership and nearest
   neighbours are bit-identical to the CPU restatement;
@@ -39,6 +39,7 @@ UNITS = {
     "rbf_plan.hip": ["-ffp-contract=off"],
     "capi.hip": ["-ffp-contract=off"],
     "fe_nearest.hip": ["-ffp-contract=off"],
+    "mls.hip": ["-ffp-contract=off"],
     "rbf_solve.hip": [],
     "rbf_grad.hip": [],
     "rbf_solve_tile_a.hip": [],

@@ -6,7 +6,8 @@
 // this build does not provide is an error, not exit(1).
 // pacman_fe_interpolate_c / PACMAN::fe_interpolate replace
 // src/fortran_interface.cpp and src/interface.cpp:93-125 for the method
-// NEAREST_NEAREST.
+// NEAREST_NEAREST. pacman_mls_interpolate_c / PACMAN::MLS_interpolate
+// replace src/fortran_interface.cpp:135-164 and src/mls_interface.cpp:77-106.
 
 #include <hip/hip_runtime.h>
 
@@ -21,7 +22,9 @@
 
 #include "../../include/fortran_interface.h"
 #include "../../include/interface.hpp"
+#include "../../include/pacman_mls_plan.h"
 #include "../../include/pacman_rbf_plan.h"
+#include "mls.hpp"
 #include "rbf_plan.hpp"
 
 using pacman_amd::DevBuf;
@@ -30,6 +33,10 @@ using pacman_amd::RbfPumPlan;
 
 struct pacman_rbf_plan_s {
   std::unique_ptr<RbfPumPlan> plan;
+};
+
+struct pacman_mls_plan_s {
+  std::unique_ptr<pacman_amd::MlsPlan> plan;
 };
 
 namespace {
@@ -118,6 +125,29 @@ void run_host_nearest(int dim, const double *src, int64_t ns,
   PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
                                   hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipMemcpyAsync(status, dst.get(), 4 * (size_t)nt,
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// MLS transfer on host arrays.
+void run_host_mls(int dim, const double *src, int64_t ns, const double *vals,
+                  const double *tgt, int64_t nt, double *out) {
+  hipStream_t s = nullptr;
+  DevBuf<double> dsrc((size_t)std::max<int64_t>(ns * dim, 1)),
+      dvals((size_t)std::max<int64_t>(ns, 1)),
+      dtgt((size_t)std::max<int64_t>(nt * dim, 1)),
+      dout((size_t)std::max<int64_t>(nt, 1));
+  if (ns > 0) {
+    PACMAN_HIP_CHECK(hipMemcpyAsync(dsrc.get(), src, 8 * (size_t)ns * dim,
+                                    hipMemcpyHostToDevice, s));
+    PACMAN_HIP_CHECK(hipMemcpyAsync(dvals.get(), vals, 8 * (size_t)ns,
+                                    hipMemcpyHostToDevice, s));
+  }
+  PACMAN_HIP_CHECK(hipMemcpyAsync(dtgt.get(), tgt, 8 * (size_t)nt * dim,
+                                  hipMemcpyHostToDevice, s));
+  pacman_amd::MlsPlan plan(dim, dsrc.get(), ns, dtgt.get(), nt, s);
+  plan.apply(dvals.get(), 1, dout.get(), s);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
                                   hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -215,6 +245,40 @@ FeInterpolateResult fe_interpolate(int_t spaceDimension,
   run_host_nearest(spaceDimension, sourcePoints, nSourcePoints, sourceValues,
                    targetPoints, nTargetPoints, result.targetValues.data(),
                    result.targetStatus.data());
+  return result;
+}
+
+MLSInterpolateResult MLS_interpolate(int_t spaceDimension,
+                                     unsigned char execSpace,
+                                     coordinates_t *sourcePoints,
+                                     int_t nSourcePoints, fp_t *sourceValues,
+                                     coordinates_t *targetPoints,
+                                     int_t nTargetPoints) {
+  // src/mls_interface.cpp:83-105: null pointers, dimension, exec space
+  if (sourcePoints == nullptr && nSourcePoints > 0)
+    throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
+  if (targetPoints == nullptr && nTargetPoints > 0)
+    throw std::invalid_argument("targetPoints is null but nTargetPoints > 0");
+  if (sourceValues == nullptr && nSourcePoints > 0)
+    throw std::invalid_argument("sourceValues is null but nSourcePoints > 0");
+  if (spaceDimension < 1 || spaceDimension > 3)
+    throw std::runtime_error(
+        "The dimension of the points can only be: 1, 2 or 3.\n");
+  const char *name = space_name(execSpace);
+  if (name == nullptr)
+    throw std::invalid_argument(
+        "execspace: Serial, OpenMP, Threads, Cuda, HIP, SYCL");
+  if (execSpace != ExecSpaces::HIP)
+    throw std::invalid_argument(std::string("execution space ") + name +
+                                " is not enabled in this build (available: HIP)");
+  if (nSourcePoints < 0 || nTargetPoints < 0)
+    throw std::invalid_argument("negative number of points");
+  MLSInterpolateResult result;
+  result.targetValues.assign((size_t)nTargetPoints, 0.0);
+  if (nTargetPoints == 0)
+    return result;
+  run_host_mls(spaceDimension, sourcePoints, nSourcePoints, sourceValues,
+               targetPoints, nTargetPoints, result.targetValues.data());
   return result;
 }
 
@@ -545,6 +609,144 @@ int pacman_rbf_plan_target_coord_grad(pacman_rbf_plan plan,
     set_error("pacman_rbf_plan_target_coord_grad", "unknown exception");
     return -2;
   }
+}
+
+int pacman_mls_interpolate_c(int spaceDimension, int execSpace,
+                             const double *sourcePoints, int nSourcePoints,
+                             const double *sourceValues,
+                             const double *targetPoints, int nTargetPoints,
+                             double *targetValues) {
+  try {
+    if (targetValues == nullptr && nTargetPoints > 0)
+      throw std::invalid_argument("targetValues is null but nTargetPoints > 0");
+    PACMAN::MLSInterpolateResult r = PACMAN::MLS_interpolate(
+        spaceDimension, (unsigned char)execSpace,
+        const_cast<double *>(sourcePoints), nSourcePoints,
+        const_cast<double *>(sourceValues), const_cast<double *>(targetPoints),
+        nTargetPoints);
+    for (int i = 0; i < nTargetPoints; ++i)
+      targetValues[i] = r.targetValues[(size_t)i];
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_mls_interpolate_c", e.what());
+    std::cerr << "pacman_mls_interpolate_c: " << e.what() << "\n";
+    return -1;
+  } catch (...) {
+    set_error("pacman_mls_interpolate_c", "unknown exception");
+    std::cerr << "pacman_mls_interpolate_c: unknown exception\n";
+    return -2;
+  }
+}
+
+int pacman_mls_plan_create(pacman_mls_plan *plan, int spaceDimension,
+                           const double *sourcePoints, long long nSourcePoints,
+                           const double *targetPoints, long long nTargetPoints,
+                           int onDevice, void *stream) {
+  try {
+    if (plan == nullptr)
+      throw std::invalid_argument("plan is null");
+    *plan = nullptr;
+    if (spaceDimension < 1 || spaceDimension > 3)
+      throw std::runtime_error(
+          "The dimension of the points can only be: 1, 2 or 3.\n");
+    if (nSourcePoints < 0 || nTargetPoints < 0)
+      throw std::invalid_argument("negative number of points");
+    if ((sourcePoints == nullptr && nSourcePoints > 0) ||
+        (targetPoints == nullptr && nTargetPoints > 0))
+      throw std::invalid_argument("null points pointer");
+    hipStream_t s = (hipStream_t)stream;
+    auto h = std::make_unique<pacman_mls_plan_s>();
+    if (onDevice) {
+      h->plan = std::make_unique<pacman_amd::MlsPlan>(
+          spaceDimension, sourcePoints, nSourcePoints, targetPoints,
+          nTargetPoints, s);
+    } else {
+      DevBuf<double> ds((size_t)std::max<long long>(nSourcePoints, 1) *
+                        spaceDimension),
+          dt((size_t)std::max<long long>(nTargetPoints, 1) * spaceDimension);
+      if (nSourcePoints > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(
+            ds.get(), sourcePoints, 8 * (size_t)nSourcePoints * spaceDimension,
+            hipMemcpyHostToDevice, s));
+      if (nTargetPoints > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(
+            dt.get(), targetPoints, 8 * (size_t)nTargetPoints * spaceDimension,
+            hipMemcpyHostToDevice, s));
+      h->plan = std::make_unique<pacman_amd::MlsPlan>(
+          spaceDimension, ds.get(), nSourcePoints, dt.get(), nTargetPoints, s);
+    }
+    *plan = h.release();
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_mls_plan_create", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_mls_plan_create", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_mls_plan_apply(pacman_mls_plan plan, const double *sourceValues,
+                          int nFields, double *targetValues, int onDevice,
+                          void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    const pacman_amd::MlsPlan &P = *plan->plan;
+    hipStream_t s = (hipStream_t)stream;
+    if (nFields < 1)
+      throw std::invalid_argument("nFields must be >= 1");
+    if (sourceValues == nullptr ||
+        (targetValues == nullptr && P.numTargets() > 0))
+      throw std::invalid_argument("null values pointer");
+    if (onDevice) {
+      P.apply(sourceValues, nFields, targetValues, s);
+    } else {
+      DevBuf<double> dv((size_t)nFields * P.numSources()),
+          dout((size_t)nFields * std::max<int64_t>(P.numTargets(), 1));
+      PACMAN_HIP_CHECK(hipMemcpyAsync(dv.get(), sourceValues, 8 * dv.n,
+                                      hipMemcpyHostToDevice, s));
+      P.apply(dv.get(), nFields, dout.get(), s);
+      if (P.numTargets() > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(
+            targetValues, dout.get(), 8 * (size_t)nFields * P.numTargets(),
+            hipMemcpyDeviceToHost, s));
+      PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_mls_plan_apply", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_mls_plan_apply", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_mls_plan_k(pacman_mls_plan plan) {
+  return plan && plan->plan ? plan->plan->k() : -1;
+}
+
+int pacman_mls_plan_neighbours(pacman_mls_plan plan, int *idx, double *coef) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    if ((idx == nullptr || coef == nullptr) && plan->plan->numTargets() > 0)
+      throw std::invalid_argument("null output pointer");
+    plan->plan->neighbours(idx, coef, nullptr);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_mls_plan_neighbours", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_mls_plan_neighbours", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_mls_plan_destroy(pacman_mls_plan plan) {
+  delete plan;
+  return 0;
 }
 
 int pacman_rbf_plan_destroy(pacman_rbf_plan plan) {

@@ -1,0 +1,109 @@
+"""MLS transfer on the GPU against the oracle and the reference's pins.
+
+* neighbour lists (k nearest sources, ties to the lowest index) equal the
+  oracle's bit for bit, on random clouds, jittered grids and regular grids;
+* values against the oracle (pinned by an independent numpy least-squares
+  solve in tests/test_oracle.py): relative 1e-10;
+* the reference's known-answer test through the drop-in API: a 20^D grid
+  onto itself, rel-L2 < 1e-4 (tests/test_mls.py:86-94);
+* quadratic fields reproduced at 1M -> 1M points, multi-field applies equal
+  single-field ones, host and device plans agree, and the errors of the
+  drop-in call.
+"""
+
+import numpy as np
+import pytest
+
+import pacman_amd as pacman
+
+pytestmark = pytest.mark.gpu
+
+HIP = pacman.execspaces.HIP
+
+
+def _basis(s, dim):
+    cols = ([np.ones(len(s))] + [s[:, a] for a in range(dim)]
+            + [s[:, a] * s[:, b] for a in range(dim) for b in range(a, dim)])
+    return np.stack(cols, 1)
+
+
+def _grid(n, dim):
+    ax = [np.linspace(0.0, 1.0, n)] * dim
+    return np.stack([g.ravel() for g in np.meshgrid(*ax, indexing="ij")], 1)
+
+
+def _clouds(kind, dim, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "random":
+        return rng.uniform(0, 1, (20000, dim)), rng.uniform(0, 1, (6000, dim))
+    if kind == "jittered":
+        g = {1: 4000, 2: 90, 3: 25}[dim]
+        S = _grid(g, dim)
+        S = S + rng.uniform(-0.3, 0.3, S.shape) / (g - 1)
+        return S, rng.uniform(0, 1, (6000, dim))
+    G = _grid({1: 200, 2: 40, 3: 15}[dim], dim)
+    return G, np.concatenate([G, rng.uniform(0, 1, (2000, dim))])
+
+
+@pytest.mark.parametrize("kind", ["random", "jittered", "grid"])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_neighbours_and_values_equal_oracle(oracle_mod, kind, dim):
+    S, T = _clouds(kind, dim, 100 + dim)
+    f = np.sin(3 * S[:, 0]) + np.cos(2 * S[:, -1]) + S.sum(1) ** 3
+    ridx, rcoef = oracle_mod.mls_coefficients(dim, S, T)
+    plan = pacman.mls.Plan(dim, S, T)
+    idx, coef = plan.neighbours()
+    assert np.array_equal(idx, ridx)
+    ref = np.sum(rcoef * f[ridx], 1)
+    got = plan.apply(f)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_reference_same_grid_drop_in(oracle_mod, dim):
+    G = _grid(20, dim)
+    fr = {1: oracle_mod.franke_1d, 2: oracle_mod.franke_2d,
+          3: oracle_mod.franke_3d}[dim]
+    f = fr(*G.T)
+    got = pacman.MLS.interpolate(dim, HIP, G, f, G)
+    assert np.linalg.norm(got - f) / np.linalg.norm(f) < 1e-4
+
+
+def test_quadratic_reproduction_at_scale():
+    import torch
+    from pacman_amd import workloads as W
+    S = W.jittered_grid(100, 6)
+    T = W.uniform(1_000_000, 7)
+    c = np.random.default_rng(3).normal(size=10)
+    plan = pacman.mls.Plan(3, torch.from_numpy(S).cuda(),
+                           torch.from_numpy(T).cuda())
+    q = lambda X: _basis(X, 3) @ c
+    v = torch.from_numpy(np.stack([q(S), np.sin(S[:, 0])])).cuda()
+    out = plan.apply(v).cpu().numpy()
+    assert np.abs(out[0] - q(T)).max() < 1e-9 * np.abs(q(T)).max()
+    one = plan.apply(v[1].contiguous()).cpu().numpy()
+    assert np.array_equal(out[1], one)
+
+
+def test_host_and_device_plans_agree():
+    import torch
+    rng = np.random.default_rng(4)
+    S = rng.uniform(0, 1, (5000, 2))
+    T = rng.uniform(0, 1, (3000, 2))
+    f = rng.normal(size=(3, len(S)))
+    a = pacman.mls.Plan(2, S, T).apply(f)
+    b = pacman.mls.Plan(2, torch.from_numpy(S).cuda(),
+                        torch.from_numpy(T).cuda())
+    assert np.array_equal(a, b.apply(torch.from_numpy(f).cuda()).cpu().numpy())
+
+
+def test_drop_in_errors():
+    rng = np.random.default_rng(5)
+    S = rng.uniform(0, 1, (9, 3))
+    T = rng.uniform(0, 1, (4, 3))
+    with pytest.raises(RuntimeError, match="at least 10"):
+        pacman.MLS.interpolate(3, HIP, S, np.zeros(9), T)
+    with pytest.raises(ValueError):
+        pacman.MLS.interpolate(3, pacman.execspaces.SERIAL, S, np.zeros(9), T)
+    assert pacman.MLS.interpolate(3, HIP, rng.uniform(0, 1, (20, 3)),
+                                  np.ones(20), T[:0]).shape == (0,)

@@ -401,3 +401,72 @@ def test_target_grad_of_linear_field_is_its_slope(oracle_mod):
     grad = o.target_grad(f, g)
     assert np.allclose(grad, g[:, None] * slope[None, :], rtol=0,
                        atol=1e-9 * np.abs(g).max() * 3)
+
+
+# ---- MLS -------------------------------------------------------------------
+
+def _mls_basis(s, dim):
+    cols = ([np.ones(len(s))] + [s[:, a] for a in range(dim)]
+            + [s[:, a] * s[:, b] for a in range(dim) for b in range(a, dim)])
+    return np.stack(cols, 1)
+
+
+def _mls_numpy(S, T, f, dim):
+    """Independent MLS: brute-force k nearest (lexicographic (d^2, index)),
+    weighted least squares by numpy.linalg.lstsq."""
+    k = (dim + 1) * (dim + 2) // 2
+    idx = np.empty((len(T), k), np.int64)
+    out = np.empty(len(T))
+    for t in range(len(T)):
+        d2 = ((S - T[t]) ** 2).sum(1)
+        order = np.lexsort((np.arange(len(S)), d2))[:k]
+        idx[t] = order
+        s = S[order] - T[t]
+        d = np.sqrt((s ** 2).sum(1))
+        rho = 1.1 * max(d.max(), np.finfo(float).eps)
+        w = 1 - d / rho
+        B = w[:, None] * _mls_basis(s / rho, dim)
+        out[t] = np.linalg.lstsq(B, w * f[order], rcond=None)[0][0]
+    return idx, out
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_mls_matches_numpy_least_squares(oracle_mod, dim):
+    rng = np.random.default_rng(90 + dim)
+    S = rng.uniform(0, 1, (3000, dim))
+    T = rng.uniform(0, 1, (400, dim))
+    f = np.sin(3 * S[:, 0]) + np.cos(2 * S[:, -1]) + S.sum(1) ** 3
+    idx, coef = oracle_mod.mls_coefficients(dim, S, T)
+    ridx, ref = _mls_numpy(S, T, f, dim)
+    assert np.array_equal(idx, ridx)
+    got = np.sum(coef * f[idx], 1)
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_mls_reproduces_quadratics(oracle_mod, dim):
+    rng = np.random.default_rng(95 + dim)
+    S = rng.uniform(0, 1, (5000, dim))
+    T = rng.uniform(0, 1, (1000, dim))
+    c = rng.normal(size=(dim + 1) * (dim + 2) // 2)
+    q = lambda X: _mls_basis(X, dim) @ c
+    got = oracle_mod.mls_interpolate(dim, S, q(S), T)
+    assert np.abs(got - q(T)).max() < 1e-9
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_mls_reference_same_grid(oracle_mod, dim):
+    """The reference's known-answer test: a 20^D grid onto itself, rel-L2
+    below 1e-4 against the Franke field (tests/test_mls.py:86-94,
+    tests/test_cpp_mls_interface.cpp:15,156-166)."""
+    ax = [np.linspace(0.0, 1.0, 20)] * dim
+    G = np.stack([g.ravel() for g in np.meshgrid(*ax, indexing="ij")], 1)
+    fr = {1: oracle_mod.franke_1d, 2: oracle_mod.franke_2d,
+          3: oracle_mod.franke_3d}[dim]
+    f = fr(*G.T)
+    got = oracle_mod.mls_interpolate(dim, G, f, G)
+    assert np.linalg.norm(got - f) / np.linalg.norm(f) < 1e-4
+    # and the independent least squares agrees on the grid's rank-deficient
+    # neighbourhoods
+    _, ref = _mls_numpy(G, G[::7], f, dim)
+    assert np.abs(got[::7] - ref).max() < 1e-10
