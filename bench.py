@@ -151,6 +151,17 @@ def main() -> int:
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_apply")
 
+    # Host buffers in and out (the drop-in boundary hands over host arrays):
+    # the same transfer with the PCIe copies, outside the timed region.
+    del plan
+    torch.cuda.synchronize()
+    h0 = time.perf_counter()
+    hplan = pacman.rbf.Plan(3, args.rbf, S, T, npc, 0.15, support)
+    hplan.apply(v)
+    h1 = time.perf_counter()
+    del hplan
+    host_ms = (h1 - h0) * 1e3
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
@@ -197,6 +208,8 @@ def main() -> int:
             "max_patch_size": int(work["max_n"]),
             "phase_ms": {k: round(x, 3) for k, x in ph.items()},
             "rel_l2_vs_franke_3d": rel_l2,
+            "host_buffers_ms": round(host_ms, 3),
+            "host_buffers_mpts_s": M / (host_ms / 1e3) / 1e6,
         },
         "roofline": {
             "bound": "mfma",

@@ -80,6 +80,14 @@ int pacman_rbf_plan_target_coord_grad(pacman_rbf_plan plan,
 
 int pacman_rbf_plan_destroy(pacman_rbf_plan plan);
 
+/* Device memory of destroyed plans is cached for reuse by later plans of
+ * the same sizes (no reference counterpart). This returns the cached blocks
+ * to the HIP runtime (synchronises the device); 0 on success. Setting
+ * PACMAN_AMD_NO_MEMORY_CACHE=1 in the environment disables the cache. */
+int pacman_release_device_cache(void);
+/* Bytes of device memory the cache holds. */
+long long pacman_device_cache_bytes(void);
+
 /* Introspection (host outputs). */
 double pacman_rbf_plan_radius(pacman_rbf_plan plan);
 double pacman_rbf_plan_support(pacman_rbf_plan plan);

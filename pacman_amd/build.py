@@ -40,6 +40,7 @@ UNITS = {
     "capi.hip": ["-ffp-contract=off"],
     "fe_nearest.hip": ["-ffp-contract=off"],
     "mls.hip": ["-ffp-contract=off"],
+    "dev_alloc.hip": [],
     "rbf_solve.hip": [],
     "rbf_grad.hip": [],
     "rbf_solve_tile_a.hip": [],

@@ -749,6 +749,15 @@ int pacman_mls_plan_destroy(pacman_mls_plan plan) {
   return 0;
 }
 
+int pacman_release_device_cache(void) {
+  pacman_amd::dev_release_cache();
+  return 0;
+}
+
+long long pacman_device_cache_bytes(void) {
+  return (long long)pacman_amd::dev_cached_bytes();
+}
+
 int pacman_rbf_plan_destroy(pacman_rbf_plan plan) {
   delete plan;
   return 0;

@@ -25,7 +25,15 @@ namespace pacman_amd {
 // Largest patch (sources) the register-resident solve takes.
 constexpr int kMaxPatch = 192;
 
-// Owning device buffer (hipMalloc / hipFree).
+// Device memory through the library's cache (dev_alloc.hip): blocks freed
+// by one plan are reused by the next allocation of the same rounded size.
+void *dev_malloc(size_t bytes);
+void dev_free(void *p, size_t bytes);
+// Returns every cached block to the runtime (synchronises the device).
+void dev_release_cache();
+size_t dev_cached_bytes();
+
+// Owning device buffer (dev_malloc / dev_free).
 template <typename T> struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
@@ -50,13 +58,13 @@ template <typename T> struct DevBuf {
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
-    n = count;
     if (count)
-      PACMAN_HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+      p = static_cast<T *>(dev_malloc(count * sizeof(T)));
+    n = count;
   }
   void release() {
     if (p)
-      (void)hipFree(p);
+      dev_free(p, n * sizeof(T));
     p = nullptr;
     n = 0;
   }

@@ -42,3 +42,26 @@ def test_graph_replay_matches_eager(nfields):
     plan.apply(dV, out=out, graph=True)
     torch.cuda.synchronize()
     assert torch.equal(out, eager2)
+
+
+def test_device_cache_reuse_gives_identical_results():
+    """Plans built after others were destroyed reuse their cached device
+    blocks (stale contents) and must give bitwise the same transfer."""
+    import gc
+    from pacman_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(31)
+    S = rng.uniform(0, 1, (30000, 3))
+    T = rng.uniform(0, 1, (20000, 3))
+    f = rng.normal(size=(2, len(S)))
+    first = pacman.rbf.Plan(3, 0x11, S, T, 80, 0.15, -1.0).apply(f)
+    gc.collect()
+    assert lib.pacman_device_cache_bytes() > 0
+    for _ in range(3):
+        again = pacman.rbf.Plan(3, 0x11, S, T, 80, 0.15, -1.0).apply(f)
+        assert np.array_equal(first, again)
+    gc.collect()
+    assert lib.pacman_release_device_cache() == 0
+    assert lib.pacman_device_cache_bytes() == 0
+    fresh = pacman.rbf.Plan(3, 0x11, S, T, 80, 0.15, -1.0).apply(f)
+    assert np.array_equal(first, fresh)
