@@ -575,41 +575,32 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
           a.alphaOut[(s0 + t) * F + f] = rhs[f][t];
   }
 
-  // evaluation at the member targets: tiles of 64 pairs, thread = (pair,
-  // slice); slices of ceil(n / 8) sources, four at a time into four partial
-  // sums, added in a fixed order. The next tile's targets are loaded while
-  // the current tile is summed.
+  // evaluation at the member targets: thread = (pair, slice) over chunks of
+  // up to NT pairs. With c pairs in a chunk, S = NT / c slices of
+  // ceil(n / S) sources each (four at a time into four partial sums); the
+  // slices' sums are added in slice order. S depends only on n and the pair
+  // count, so the result does not depend on the number of fields.
   {
-    double(*epart)[kFMax][64] =
-        reinterpret_cast<double(*)[kFMax][64]>(sbuf);
-    constexpr int TP = 64;
-    constexpr int ES = NT / TP;
+    double *epart = sbuf;  // [S][F][c], S * c <= NT
     const int64_t p0 = a.pairOff[patch], p1 = a.pairOff[patch + 1];
     const int mall = (int)(p1 - p0);
-    const int sl = tid / TP;
-    const int qn = (n + ES - 1) / ES;
-    const int l0 = min(n, sl * qn), l1 = min(n, l0 + qn);
-    int e = 0;
-    double yx = 0.0, yy = 0.0, yz = 0.0;
-    if (lane < mall) {
-      e = a.pairTgt[p0 + lane];
-      const int tp = e & 0x7fffffff;
-      yx = a.tx[tp];
-      yy = a.ty[tp];
-      yz = a.tz[tp];
-    }
-    for (int tile = 0; tile < mall; tile += TP) {
-      const int pi = tile + lane;
-      const bool mem = pi < mall && ((unsigned)e >> 31) != 0;
-      const double qx = yx, qy = yy, qz = yz;
-      const int pn = pi + TP;
-      if (pn < mall) {
-        e = a.pairTgt[p0 + pn];
+    for (int c0 = 0; c0 < mall; c0 += NT) {
+      const int c = min(NT, mall - c0);
+      const int S = NT / c;
+      const int pi = tid % c, sl = tid / c;
+      const bool act = sl < S;
+      const int qn = (n + S - 1) / S;
+      const int l0 = min(n, sl * qn), l1 = min(n, l0 + qn);
+      int e = 0;
+      double qx = 0.0, qy = 0.0, qz = 0.0;
+      if (act) {
+        e = a.pairTgt[p0 + c0 + pi];
         const int tp = e & 0x7fffffff;
-        yx = a.tx[tp];
-        yy = a.ty[tp];
-        yz = a.tz[tp];
+        qx = a.tx[tp];
+        qy = a.ty[tp];
+        qz = a.tz[tp];
       }
+      const bool mem = act && ((unsigned)e >> 31) != 0;
       double acc0[kFMax], acc1[kFMax], acc2[kFMax], acc3[kFMax];
 #pragma unroll
       for (int f = 0; f < kFMax; ++f) {
@@ -627,45 +618,53 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
           };
           int l = l0;
           for (; l + 3 < l1; l += 4) {
-            const double p0 = phi(l), p1 = phi(l + 1), p2 = phi(l + 2),
-                         p3 = phi(l + 3);
+            const double f0 = phi(l), f1 = phi(l + 1), f2 = phi(l + 2),
+                         f3 = phi(l + 3);
 #pragma unroll
             for (int f = 0; f < kFMax; ++f)
               if (f < F) {
-                acc0[f] = fma(p0, rhs[f][l], acc0[f]);
-                acc1[f] = fma(p1, rhs[f][l + 1], acc1[f]);
-                acc2[f] = fma(p2, rhs[f][l + 2], acc2[f]);
-                acc3[f] = fma(p3, rhs[f][l + 3], acc3[f]);
+                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
+                acc1[f] = fma(f1, rhs[f][l + 1], acc1[f]);
+                acc2[f] = fma(f2, rhs[f][l + 2], acc2[f]);
+                acc3[f] = fma(f3, rhs[f][l + 3], acc3[f]);
               }
           }
           for (; l < l1; ++l) {
-            const double p0 = phi(l);
+            const double f0 = phi(l);
 #pragma unroll
             for (int f = 0; f < kFMax; ++f)
               if (f < F)
-                acc0[f] = fma(p0, rhs[f][l], acc0[f]);
+                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
           }
         });
       }
-#pragma unroll
-      for (int f = 0; f < kFMax; ++f)
-        if (f < F)
-          epart[sl][f][lane] = (acc0[f] + acc1[f]) + (acc2[f] + acc3[f]);
-      __syncthreads();
-      if (sl == 0 && mem) {
-        const int slot = a.pairSlot[p0 + pi];
+      if (act) {
 #pragma unroll
         for (int f = 0; f < kFMax; ++f)
-          if (f < F) {
-            double t = epart[0][f][lane];
+          if (f < F)
+            epart[(sl * F + f) * c + pi] =
+                (acc0[f] + acc1[f]) + (acc2[f] + acc3[f]);
+      }
+      __syncthreads();
+      if (tid < c) {
+        const int ee = a.pairTgt[p0 + c0 + tid];
+        if (((unsigned)ee >> 31) != 0) {
+          const int tp = ee & 0x7fffffff;
+          const double yx = a.tx[tp], yy = a.ty[tp], yz = a.tz[tp];
+          const int slot = a.pairSlot[p0 + c0 + tid];
 #pragma unroll
-            for (int q = 1; q < ES; ++q)
-              t += epart[q][f][lane];
-            // V beta (solver.hpp:272), around the centre
-            const double vb = scoef[f][0] + scoef[f][1] * (qx - cx) +
-                              scoef[f][2] * (qy - cy) + scoef[f][3] * (qz - cz);
-            a.contrib[(int64_t)slot * F + f] = t + vb;
-          }
+          for (int f = 0; f < kFMax; ++f)
+            if (f < F) {
+              double t = epart[f * c + tid];
+              for (int q = 1; q < S; ++q)
+                t += epart[(q * F + f) * c + tid];
+              // V beta (solver.hpp:272), around the centre
+              const double vb = scoef[f][0] + scoef[f][1] * (yx - cx) +
+                                scoef[f][2] * (yy - cy) +
+                                scoef[f][3] * (yz - cz);
+              a.contrib[(int64_t)slot * F + f] = t + vb;
+            }
+        }
       }
       __syncthreads();
     }
