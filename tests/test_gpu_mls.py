@@ -107,3 +107,18 @@ def test_drop_in_errors():
         pacman.MLS.interpolate(3, pacman.execspaces.SERIAL, S, np.zeros(9), T)
     assert pacman.MLS.interpolate(3, HIP, rng.uniform(0, 1, (20, 3)),
                                   np.ones(20), T[:0]).shape == (0,)
+
+
+def test_reference_same_grid_fortran():
+    """tests/test_fortran_mls_interface.f90 restated against the Fortran
+    module (MLS and FE NEAREST_NEAREST through pacman_mod; binary built by
+    __graft_entry__.build() when amdflang exists)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran",
+                       "test_fortran_mls_interface")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran test binary not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
