@@ -602,52 +602,80 @@ __device__ inline double dist3(double ax, double ay, double az, double bx,
   return sqrt(sqdiff3(ax, ay, az, bx, by, bz));
 }
 
-__global__ __launch_bounds__(kBlock) void patch_count_kernel(
-    GridView g, const double *cx, const double *cy, const double *cz,
-    int64_t K, double rad, int *cnt) {
-  const int64_t k =
-      (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (k >= K)
-    return;
-  const double x = cx[k], y = cy[k], z = cz[k];
-  int c = 0;
+// Membership in one geometric pass: the members of patch k are written to
+// its fixed-capacity row scratch[k * cap ...] (first cap members, in scan
+// order) and counted; member_compact_kernel then moves them to their CSR
+// place. `value(p, d)` gives the stored entry of point p at distance d.
+template <class V>
+__device__ inline void capped_fill(const GridView &g, double x, double y,
+                                   double z, double rad, int cap, int *row,
+                                   int *count, V &&value) {
+  int w = 0;
   wave_box_scan(g, grid_box(g, x, y, z, rad), [&](int p, bool valid) {
-    if (valid && dist3(x, y, z, g.x[p], g.y[p], g.z[p]) <= rad)
-      ++c;
-  });
-  c = wave_sum_int(c);
-  if (threadIdx.x % kWave == 0)
-    cnt[k] = c;
-}
-
-__global__ __launch_bounds__(kBlock) void patch_src_fill_kernel(
-    GridView g, const double *cx, const double *cy, const double *cz,
-    int64_t K, double rad, const int64_t *off, int *idx) {
-  const int64_t k =
-      (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (k >= K)
-    return;
-  const double x = cx[k], y = cy[k], z = cz[k];
-  int64_t w = off[k];
-  wave_box_scan(g, grid_box(g, x, y, z, rad), [&](int p, bool valid) {
-    const bool hit = valid && dist3(x, y, z, g.x[p], g.y[p], g.z[p]) <= rad;
+    double d = 0.0;
+    if (valid)
+      d = dist3(x, y, z, g.x[p], g.y[p], g.z[p]);
+    const bool hit = valid && d <= rad;
     const uint64_t m = __ballot(hit);
-    if (hit)
-      idx[w + lane_prefix(m)] = p;
+    const int o = w + lane_prefix(m);
+    if (hit && o < cap)
+      row[o] = value(p, d);
     w += __popcll(m);
   });
+  if (threadIdx.x % kWave == 0)
+    *count = w;
 }
 
-__global__ __launch_bounds__(kBlock) void patch_tgt_fill_kernel(
+__global__ __launch_bounds__(kBlock) void patch_src_capped_kernel(
     GridView g, const double *cx, const double *cy, const double *cz,
-    int64_t K, double rad, double radMember, const int64_t *off, int *pairTgt,
+    int64_t K, double rad, int cap, int *scratch, int *cnt) {
+  const int64_t k =
+      (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (k >= K)
+    return;
+  capped_fill(g, cx[k], cy[k], cz[k], rad, cap, scratch + k * cap, cnt + k,
+              [](int p, double) { return p; });
+}
+
+__global__ __launch_bounds__(kBlock) void patch_tgt_capped_kernel(
+    GridView g, const double *cx, const double *cy, const double *cz,
+    int64_t K, double rad, double radMember, int cap, int *scratch, int *cnt) {
+  const int64_t k =
+      (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (k >= K)
+    return;
+  capped_fill(g, cx[k], cy[k], cz[k], rad, cap, scratch + k * cap, cnt + k,
+              [&](int p, double d) {
+                const unsigned member = d <= radMember ? 1u : 0u;
+                return (int)((unsigned)p | (member << 31));
+              });
+}
+
+// CSR from the capped rows: a wave per patch copies its row; a patch with
+// more members than the row holds is listed again from the geometry (the
+// same scan order, so the same list). pairPatch (optional) gets k.
+__global__ __launch_bounds__(kBlock) void member_compact_kernel(
+    GridView g, const double *cx, const double *cy, const double *cz,
+    int64_t K, double rad, double radMember, bool targets, int cap,
+    const int *scratch, const int *cnt, const int64_t *off, int *out,
     int *pairPatch) {
   const int64_t k =
       (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (k >= K)
     return;
+  const int lane = threadIdx.x % kWave;
+  const int c = cnt[k];
+  const int64_t w0 = off[k];
+  if (c <= cap) {
+    for (int i = lane; i < c; i += kWave) {
+      out[w0 + i] = scratch[k * cap + i];
+      if (pairPatch)
+        pairPatch[w0 + i] = (int)k;
+    }
+    return;
+  }
   const double x = cx[k], y = cy[k], z = cz[k];
-  int64_t w = off[k];
+  int64_t w = w0;
   wave_box_scan(g, grid_box(g, x, y, z, rad), [&](int p, bool valid) {
     double d = 0.0;
     if (valid)
@@ -656,9 +684,14 @@ __global__ __launch_bounds__(kBlock) void patch_tgt_fill_kernel(
     const uint64_t m = __ballot(hit);
     if (hit) {
       const int64_t o = w + lane_prefix(m);
-      const unsigned member = d <= radMember ? 1u : 0u;
-      pairTgt[o] = (int)((unsigned)p | (member << 31));
-      pairPatch[o] = (int)k;
+      if (targets) {
+        const unsigned member = d <= radMember ? 1u : 0u;
+        out[o] = (int)((unsigned)p | (member << 31));
+      } else {
+        out[o] = p;
+      }
+      if (pairPatch)
+        pairPatch[o] = (int)k;
     }
     w += __popcll(m);
   });
@@ -903,14 +936,21 @@ int64_t make_centers(const Lattice &Lh, const GridView &src,
   return K;
 }
 
+// Rows of the one-pass membership scratch: wide enough for the patches the
+// solve takes; larger patches are listed again from the geometry.
+constexpr int kMemberCap = 256;
+
 void patch_sources(const GridView &src, const double *cx, const double *cy,
                    const double *cz, int64_t K, double r,
                    DevBuf<int64_t> &off, DevBuf<int> &idx, hipStream_t s) {
   DevBuf<int> cnt(std::max<int64_t>(K, 1));
+  DevBuf<int> scratch((size_t)std::max<int64_t>(K, 1) * kMemberCap);
   off.alloc(K + 1);
   const unsigned wb = blocks_for(K, kBlock / kWave);
   if (K > 0) {
-    patch_count_kernel<<<wb, kBlock, 0, s>>>(src, cx, cy, cz, K, r, cnt.get());
+    patch_src_capped_kernel<<<wb, kBlock, 0, s>>>(src, cx, cy, cz, K, r,
+                                                  kMemberCap, scratch.get(),
+                                                  cnt.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   const int64_t tot = exclusive_scan_i32_to_i64(cnt.get(), off.get(), K, s);
@@ -918,8 +958,9 @@ void patch_sources(const GridView &src, const double *cx, const double *cy,
     throw std::runtime_error("patch source lists exceed 2^31 entries");
   idx.alloc(std::max<int64_t>(tot, 1));
   if (K > 0) {
-    patch_src_fill_kernel<<<wb, kBlock, 0, s>>>(src, cx, cy, cz, K, r,
-                                                off.get(), idx.get());
+    member_compact_kernel<<<wb, kBlock, 0, s>>>(
+        src, cx, cy, cz, K, r, r, false, kMemberCap, scratch.get(), cnt.get(),
+        off.get(), idx.get(), nullptr);
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
@@ -930,10 +971,14 @@ void patch_targets(const GridView &tgt, const double *cx, const double *cy,
                    hipStream_t s) {
   const int64_t M = tgt.n;
   DevBuf<int> cnt(std::max<int64_t>(K, 1));
+  DevBuf<int> scratch((size_t)std::max<int64_t>(K, 1) * kMemberCap);
   ps.off.alloc(K + 1);
   const unsigned wb = blocks_for(K, kBlock / kWave);
   if (K > 0) {
-    patch_count_kernel<<<wb, kBlock, 0, s>>>(tgt, cx, cy, cz, K, r, cnt.get());
+    // solver.hpp:62: targets use the radius r - epsilon
+    patch_tgt_capped_kernel<<<wb, kBlock, 0, s>>>(tgt, cx, cy, cz, K, r,
+                                                  r - kEps, kMemberCap,
+                                                  scratch.get(), cnt.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   const int64_t P = exclusive_scan_i32_to_i64(cnt.get(), ps.off.get(), K, s);
@@ -947,10 +992,9 @@ void patch_targets(const GridView &tgt, const double *cx, const double *cy,
   ps.slotMember.alloc(P1);
   DevBuf<int> pairPatch(P1), tlist(P1);
   if (K > 0) {
-    // solver.hpp:62: targets use the radius r - epsilon
-    patch_tgt_fill_kernel<<<wb, kBlock, 0, s>>>(tgt, cx, cy, cz, K, r,
-                                                r - kEps, ps.off.get(),
-                                                ps.tgt.get(), pairPatch.get());
+    member_compact_kernel<<<wb, kBlock, 0, s>>>(
+        tgt, cx, cy, cz, K, r, r - kEps, true, kMemberCap, scratch.get(),
+        cnt.get(), ps.off.get(), ps.tgt.get(), pairPatch.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   DevBuf<int> tcnt(std::max<int64_t>(M, 1)), tfill(std::max<int64_t>(M, 1));
