@@ -57,13 +57,73 @@ __device__ __forceinline__ double fast_rcp(double d) {
   return fma(r, e, r);
 }
 
+// Owners of the lower-triangle 16x16 tiles among the 7 tile waves. Tile
+// (J+1, J) and the diagonal tile (J+1, J+1) go to the same wave: that wave
+// computes W_{J+1} during panel J and applies it to the diagonal tile at
+// once, so the control wave gets that tile without a further barrier. Pairs
+// are dealt first, then the tile (0, 0), then the other tiles, each to the
+// least-loaded wave; every wave lists its tiles column by column.
+struct TileTable {
+  int I[7][16], J[7][16], cnt[7];
+};
+
+constexpr TileTable make_tile_table(int R) {
+  TileTable t{};
+  int load[7] = {0, 0, 0, 0, 0, 0, 0};
+  auto pick = [&]() {
+    int w = 0;
+    for (int x = 1; x < 7; ++x)
+      if (load[x] < load[w])
+        w = x;
+    return w;
+  };
+  auto put = [&](int w, int I, int J) {
+    t.I[w][t.cnt[w]] = I;
+    t.J[w][t.cnt[w]] = J;
+    ++t.cnt[w];
+    ++load[w];
+  };
+  for (int J = 0; J + 1 < R; ++J) {
+    const int w = pick();
+    put(w, J + 1, J);
+    put(w, J + 1, J + 1);
+  }
+  put(pick(), 0, 0);
+  for (int J = 0; J < R; ++J)
+    for (int I = J + 2; I < R; ++I)
+      put(pick(), I, J);
+  for (int w = 0; w < 7; ++w)  // column-major order within a wave
+    for (int a = 1; a < t.cnt[w]; ++a)
+      for (int b = a; b > 0; --b) {
+        const bool less = t.J[w][b] < t.J[w][b - 1] ||
+                          (t.J[w][b] == t.J[w][b - 1] &&
+                           t.I[w][b] < t.I[w][b - 1]);
+        if (!less)
+          break;
+        const int ti = t.I[w][b], tj = t.J[w][b];
+        t.I[w][b] = t.I[w][b - 1];
+        t.J[w][b] = t.J[w][b - 1];
+        t.I[w][b - 1] = ti;
+        t.J[w][b - 1] = tj;
+      }
+  return t;
+}
+
+constexpr int max_tiles_per_wave(const TileTable &t) {
+  int m = 1;
+  for (int w = 0; w < 7; ++w)
+    m = t.cnt[w] > m ? t.cnt[w] : m;
+  return m;
+}
+
 template <int R> struct TileShape {
   static constexpr int NMAX = 16 * R;
   static constexpr int NW = 8;              // waves per workgroup
   static constexpr int NT = 64 * NW;
   static constexpr int NWT = NW - 1;        // tile waves
   static constexpr int NTILE = R * (R + 1) / 2;
-  static constexpr int TPW = (NTILE + NWT - 1) / NWT;
+  static constexpr TileTable TAB = make_tile_table(R);
+  static constexpr int TPW = max_tiles_per_wave(TAB);
   static constexpr int LDW = NMAX + 4;      // panel column stride (doubles)
   static constexpr int PANEL = 2 * 16 * LDW;
   static constexpr int EVAL = NW * kFMax * 64;
@@ -350,7 +410,6 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     for (int p = 0; p < P; ++p) {
       __syncthreads();  // A
       __syncthreads();  // B
-      __syncthreads();  // C
       if (p + 1 < P)
         factor(min(16, n - 16 * (p + 1)), (p + 1) & 1, p + 1);
     }
@@ -407,15 +466,16 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     int tI[TPW], tJ[TPW];
     static_for<0, TPW>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
-      const int t = s * NWT + tw;
-      int J = 0, off = 0;
-      while (J < R && t >= off + (R - J)) {
-        off += R - J;
-        ++J;
-      }
-      const bool ok = t < NTILE;
-      tI[s] = ok ? J + (t - off) : R;
-      tJ[s] = ok ? J : R;
+      tI[s] = R;
+      tJ[s] = R;
+      static_for<0, 7>([&](auto wc) {
+        constexpr int w = decltype(wc)::value;
+        constexpr TileTable tab = TileShape<R>::TAB;
+        if (tw == w && s < tab.cnt[w]) {
+          tI[s] = tab.I[w][s < 16 ? s : 0];
+          tJ[s] = tab.J[w][s < 16 ? s : 0];
+        }
+      });
     });
 
     dbl4 acc[TPW];
@@ -455,6 +515,20 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 #pragma unroll
           for (int v = 0; v < 4; ++v)
             wp[buf][lc][16 * I + lr + 4 * v] = w4[v];
+          if (I == p + 1 && I < P) {
+            // this wave owns the next diagonal tile: update it with its own
+            // W_{p+1} and hand it to the control wave
+            __builtin_amdgcn_wave_barrier();
+            static_for<0, TPW>([&](auto dc) {
+              constexpr int d = decltype(dc)::value;
+              if (tI[d] == p + 1 && tJ[d] == p + 1) {
+                acc[d] = schur(acc[d], p + 1, p + 1, buf);
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  dtile[lr + 4 * v][lc] = acc[d][v];
+              }
+            });
+          }
         } else if (J == p && I == p && kk < 16 && 16 * p + kk < n + F) {
           // bordered rows inside the last diagonal block
 #pragma unroll
@@ -476,19 +550,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
               acc[s][v] = t4[v] * sc2;
         }
       });
-      __syncthreads();  // B
-      // the next diagonal tile first: update it and hand it over
-      static_for<0, TPW>([&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        const int I = tI[s], J = tJ[s];
-        if (I == p + 1 && J == p + 1 && J < P) {
-          acc[s] = schur(acc[s], I, J, buf);
-#pragma unroll
-          for (int v = 0; v < 4; ++v)
-            dtile[lr + 4 * v][lc] = acc[s][v];
-        }
-      });
-      __syncthreads();  // C
+      __syncthreads();  // B: panel published, next diagonal tile handed over
       // trailing update; publish the next panel's tiles
       static_for<0, TPW>([&](auto sc) {
         constexpr int s = decltype(sc)::value;

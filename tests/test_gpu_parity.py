@@ -76,6 +76,20 @@ def test_random_cloud_parity(rbf, oracle_mod):
     run_case(3, rbf, S, T, v, 40, -1.0, oracle_mod, TOL_PATCH_RADIUS_SUPPORT)
 
 
+def test_dense_targets_overflow_membership_rows(oracle_mod):
+    """Targets 60x denser than the sources: every patch holds far more
+    targets than the one-pass membership rows (256), so the lists come from
+    the geometry again; they must still equal the oracle's."""
+    rng = np.random.default_rng(12)
+    S = rng.uniform(0, 1, (1500, 3))
+    T = rng.uniform(0, 1, (90000, 3))
+    v = W.franke_3d(*S.T)
+    plan, _, _, _ = run_case(3, 0x11, S, T, v, 30, -1.0, oracle_mod,
+                             TOL_PATCH_RADIUS_SUPPORT)
+    off, _ = plan.csr(1)
+    assert np.diff(off).max() > 256
+
+
 def test_config1_parity(oracle_mod):
     S, T, npc, support = W.config("c1")
     v = W.franke_3d(*S.T)
