@@ -359,44 +359,54 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     // identity turns into Linv. Writes Linv, d and 1/d to LDS buffer `buf`
     // and the packed Linv of block p.
     auto factor = [&](int kk, int buf, int p) __attribute__((always_inline)) {
-      const int i = lane & 15;
-      double row[16], yr[16];
+      // Lane (q, i) = 16 q + i holds 8 values of row i: q = 0, 1 the
+      // columns 8q..8q+7 of the block, q = 2, 3 the columns 8(q-2).. of the
+      // identity that turns into Linv. The pivot row's values come from lane
+      // (q, k) of the same 16-lane row (DPP row_newbcast); the multiplier of
+      // row i is formed in the row holding column k and sent to the other
+      // three 16-lane rows with ds_bpermute. Lower-triangle entries of the
+      // block and zero columns of the identity are updated too (they are
+      // never read, or stay exactly zero), so every lane runs the same
+      // eight FMAs per pivot.
+      const int i = lane & 15, q = lane >> 4;
+      double v[8];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        row[c] = dtile[i][c];
-        yr[c] = (c == i) ? 1.0 : 0.0;
-      }
-      double myd = 1.0, myrd = 1.0;
+      for (int c = 0; c < 8; ++c)
+        v[c] = q < 2 ? dtile[i][8 * q + c]
+                     : ((8 * (q - 2) + c == i) ? 1.0 : 0.0);
       static_for<0, 16>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
+        constexpr int qk = k / 8, ck = k % 8;
         if (k < kk) {
-          const double d = row_bcast<k>(row[k]);
-          const double rd = fast_rcp(d);
-          if (i == k) {
-            myd = d;
-            myrd = rd;
-          }
-          const double l = (i > k && i < kk) ? row[k] * rd : 0.0;
-          static_for<k + 1, 16>([&](auto cc) __attribute__((always_inline)) {
-            constexpr int c = decltype(cc)::value;
-            row[c] = fma(-l, row_bcast<k>(row[c]), row[c]);
-          });
-          static_for<0, k + 1>([&](auto cc) __attribute__((always_inline)) {
-            constexpr int c = decltype(cc)::value;
-            yr[c] = fma(-l, row_bcast<k>(yr[c]), yr[c]);
-          });
+          // valid in the 16-lane row qk
+          const double d = row_bcast<k>(v[ck]);
+          const double lq = (i > k && i < kk) ? v[ck] * fast_rcp(d) : 0.0;
+          const int src = (16 * qk + i) * 4;
+          const double l = __hiloint2double(
+              __builtin_amdgcn_ds_bpermute(src, __double2hiint(lq)),
+              __builtin_amdgcn_ds_bpermute(src, __double2loint(lq)));
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            v[c] = fma(-l, row_bcast<k>(v[c]), v[c]);
         }
       });
-      if (lane < 16) {
-        dpan[buf][i] = myd;
-        dinvp[buf][i] = myrd;
-        if (i < kk && myd == 0.0)
-          spivot = 1;
+      if (q < 2 && (i >> 3) == q) {  // the diagonal entry of row i
+        double dd = 0.0;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-          linv[buf][i][c] = yr[c];
-          if (c < i)
-            linvAll[p][i * (i - 1) / 2 + c] = yr[c];
+        for (int c = 0; c < 8; ++c)
+          dd = (c == (i & 7)) ? v[c] : dd;
+        dpan[buf][i] = dd;
+        dinvp[buf][i] = i < kk ? fast_rcp(dd) : 1.0;
+        if (i < kk && dd == 0.0)
+          spivot = 1;
+      }
+      if (q >= 2) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int col = 8 * (q - 2) + c;
+          linv[buf][i][col] = v[c];
+          if (col < i)
+            linvAll[p][i * (i - 1) / 2 + col] = v[c];
         }
       }
     };
