@@ -154,7 +154,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   __shared__ double linvAll[R][120];         // strictly-lower Linv, packed
   __shared__ double dtile[16][17];           // diagonal tile hand-over
   __shared__ double dpan[2][16], dinvp[2][16];
-  __shared__ double xs[16];
+  __shared__ double xs[4][16];
   __shared__ double scoef[kFMax][4];
   __shared__ int spivot;
   double(*wp)[16][LDW] = reinterpret_cast<double(*)[16][LDW]>(sbuf);
@@ -436,28 +436,32 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
         sbuf + (R - 1) * 16 * 17);
     for (int B = P - 1; B >= 0; --B) {
       const bool valid = 16 * B + lc < n;
-      for (int f = 0; f < F; ++f) {
+      // Each sum over the 16 rows of a block is four partial sums over the
+      // rows 4m + j (in row order) added as (p0 + p1) + (p2 + p3), so a
+      // field gets the same bits whatever the field count.
+      if (F == 1) {
+        // one field: 16-lane row j takes the partial sum j
         double y = 0.0;
         if (B + 1 < P) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int r2 = lr + 4 * v;
             if (16 * (B + 1) + r2 < n)
-              y = fma(sub[B][r2][lc], rhs[f][16 * (B + 1) + r2], y);
+              y = fma(sub[B][r2][lc], rhs[0][16 * (B + 1) + r2], y);
           }
           y += __shfl_xor(y, 16, kWave);
           y += __shfl_xor(y, 32, kWave);
         }
-        const double x = valid ? rhs[f][16 * B + lc] - sacc[B][f][lc] - y : 0.0;
+        const double x = valid ? rhs[0][16 * B + lc] - sacc[B][0][lc] - y : 0.0;
         if (lr == 0)
-          xs[lc] = x;
+          xs[0][lc] = x;
         __builtin_amdgcn_wave_barrier();
         double part = 0.0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = lr + 4 * v;
           if (r >= lc) {
-            const double xr = xs[r];
+            const double xr = xs[0][r];
             part = (r == lc) ? part + xr
                              : fma(linvAll[B][r * (r - 1) / 2 + lc], xr, part);
           }
@@ -465,7 +469,39 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
         part += __shfl_xor(part, 16, kWave);
         part += __shfl_xor(part, 32, kWave);
         if (lr == 0 && valid)
-          rhs[f][16 * B + lc] = part;
+          rhs[0][16 * B + lc] = part;
+        __builtin_amdgcn_wave_barrier();
+      }
+      // several fields: lane (lr, lc) takes field f0 + lr, row lc of the
+      // block, four fields at a time, with all four partial sums
+      for (int f0 = 0; F > 1 && f0 < F; f0 += 4) {
+        const int f = f0 + lr;
+        const bool fv = f < F;
+        double y[4] = {0.0, 0.0, 0.0, 0.0};
+        if (B + 1 < P && fv) {
+#pragma unroll
+          for (int r2 = 0; r2 < 16; ++r2)
+            if (16 * (B + 1) + r2 < n)
+              y[r2 & 3] =
+                  fma(sub[B][r2][lc], rhs[f][16 * (B + 1) + r2], y[r2 & 3]);
+        }
+        const double ys = (y[0] + y[1]) + (y[2] + y[3]);
+        const double x =
+            (valid && fv) ? rhs[f][16 * B + lc] - sacc[B][f][lc] - ys : 0.0;
+        xs[lr][lc] = x;
+        __builtin_amdgcn_wave_barrier();
+        double pt[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          if (r >= lc) {
+            const double xr = xs[lr][r];
+            pt[r & 3] = (r == lc) ? pt[r & 3] + xr
+                                  : fma(linvAll[B][r * (r - 1) / 2 + lc], xr,
+                                        pt[r & 3]);
+          }
+        }
+        if (valid && fv)
+          rhs[f][16 * B + lc] = (pt[0] + pt[1]) + (pt[2] + pt[3]);
         __builtin_amdgcn_wave_barrier();
       }
       __syncthreads();  // alpha_B ready
