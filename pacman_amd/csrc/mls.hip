@@ -7,7 +7,8 @@
 // phi_j = (1 - |s_j| / rho)^2 (Wendland<0>) and the value
 // c_0 = e_0^T (P^T Phi P)+ P^T Phi f, P the Vandermonde matrix at s_j.
 //
-// Setup (one thread per target, targets processed in cell order):
+// Setup (one thread per target, targets processed in cell order; the search
+// and the least-squares weights are two kernels):
 //   * k nearest sources by a shell search over the cell grid of the sources,
 //     a sorted register list of (squared distance, index); ties go to the
 //     lowest source index, and the search stops once the k-th distance is
@@ -311,9 +312,12 @@ __device__ __forceinline__ bool lex_less(double d1, int o1, double d2,
   return d1 < d2 || (d1 == d2 && o1 < o2);
 }
 
+// k nearest sources of every target (targets in cell order): the original
+// source index to idxOut and the cell-order position to posOut, both
+// [t * K + j] in ascending (distance, index) order.
 template <int D>
-__global__ __launch_bounds__(kMlsBlock) void mls_coef_kernel(
-    GridView g, GridView tg, int *idxOut, double *coefOut) {
+__global__ __launch_bounds__(kMlsBlock) void mls_knn_kernel(
+    GridView g, GridView tg, int *idxOut, int *posOut) {
   constexpr int K = MlsBasis<D>::K;
   const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p0 >= tg.n)
@@ -356,27 +360,29 @@ __global__ __launch_bounds__(kMlsBlock) void mls_coef_kernel(
             zb = z1;
           }
           const int cell0 = (ix * g.dims[1] + iy) * g.dims[2];
+#pragma unroll 1
           for (int p = g.cell_start[cell0 + za]; p < g.cell_start[cell0 + zb + 1];
                ++p) {
             const double d2 = sqdiff3(q[0], q[1], q[2], g.x[p], g.y[p], g.z[p]);
             const int o = g.perm[p];
             if (lex_less(d2, o, bd[K - 1], bo[K - 1])) {
-              bd[K - 1] = d2;
-              bo[K - 1] = o;
-              bp[K - 1] = p;
+              // insert at pos = number of entries before it; shift the rest
+              int pos = 0;
 #pragma unroll
-              for (int j = K - 1; j > 0; --j)
-                if (lex_less(bd[j], bo[j], bd[j - 1], bo[j - 1])) {
-                  const double td = bd[j];
-                  bd[j] = bd[j - 1];
-                  bd[j - 1] = td;
-                  const int to = bo[j];
-                  bo[j] = bo[j - 1];
-                  bo[j - 1] = to;
-                  const int tp = bp[j];
-                  bp[j] = bp[j - 1];
-                  bp[j - 1] = tp;
-                }
+              for (int j = 0; j < K - 1; ++j)
+                pos += lex_less(bd[j], bo[j], d2, o) ? 1 : 0;
+#pragma unroll
+              for (int j = K - 1; j > 0; --j) {
+                const bool sh = j > pos, at = j == pos;
+                bd[j] = sh ? bd[j - 1] : (at ? d2 : bd[j]);
+                bo[j] = sh ? bo[j - 1] : (at ? o : bo[j]);
+                bp[j] = sh ? bp[j - 1] : (at ? p : bp[j]);
+              }
+              if (pos == 0) {
+                bd[0] = d2;
+                bo[0] = o;
+                bp[0] = p;
+              }
             }
           }
         }
@@ -394,20 +400,37 @@ __global__ __launch_bounds__(kMlsBlock) void mls_coef_kernel(
       break;
   }
 
-  // ---- weights --------------------------------------------------------------
-  double sx[K], sy[K], sz[K], w[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    sx[j] = g.x[bp[j]] - q[0];
-    sy[j] = g.y[bp[j]] - q[1];
-    sz[j] = g.z[bp[j]] - q[2];
-  }
-  mls_weights<D>(sx, sy, sz, w);
 #pragma unroll
   for (int j = 0; j < K; ++j) {
     idxOut[t * K + j] = bo[j];
-    coefOut[t * K + j] = w[j];
+    posOut[t * K + j] = bp[j];
   }
+}
+
+// Weights of every target from its neighbours (a separate kernel: the QR
+// needs about 200 registers, the search few, so each runs at its own
+// occupancy).
+template <int D>
+__global__ __launch_bounds__(kMlsBlock) void mls_weights_kernel(
+    GridView g, GridView tg, const int *pos, double *coefOut) {
+  constexpr int K = MlsBasis<D>::K;
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p0 >= tg.n)
+    return;
+  const int64_t t = tg.perm[p0];
+  const double q[3] = {tg.x[p0], tg.y[p0], tg.z[p0]};
+  double sx[K], sy[K], sz[K], w[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int bp = pos[t * K + j];
+    sx[j] = g.x[bp] - q[0];
+    sy[j] = g.y[bp] - q[1];
+    sz[j] = g.z[bp] - q[2];
+  }
+  mls_weights<D>(sx, sy, sz, w);
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+    coefOut[t * K + j] = w[j];
 }
 
 template <int K>
@@ -459,19 +482,26 @@ MlsPlan::MlsPlan(int dim, const double *dSrc, int64_t N, const double *dTgt,
   build_grid(dTgt, M, dim, tlo, thi, 4.0, tg, s);
   idx_.alloc((size_t)M * k_);
   coef_.alloc((size_t)M * k_);
+  DevBuf<int> pos((size_t)M * k_);
   const unsigned blocks = (unsigned)((M + kMlsBlock - 1) / kMlsBlock);
   switch (dim) {
   case 1:
-    mls_coef_kernel<1><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
-                                                    idx_.get(), coef_.get());
+    mls_knn_kernel<1><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
+                                                   idx_.get(), pos.get());
+    mls_weights_kernel<1><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
+                                                       pos.get(), coef_.get());
     break;
   case 2:
-    mls_coef_kernel<2><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
-                                                    idx_.get(), coef_.get());
+    mls_knn_kernel<2><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
+                                                   idx_.get(), pos.get());
+    mls_weights_kernel<2><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
+                                                       pos.get(), coef_.get());
     break;
   default:
-    mls_coef_kernel<3><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
-                                                    idx_.get(), coef_.get());
+    mls_knn_kernel<3><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
+                                                   idx_.get(), pos.get());
+    mls_weights_kernel<3><<<blocks, kMlsBlock, 0, s>>>(g.view, tg.view,
+                                                       pos.get(), coef_.get());
   }
   PACMAN_HIP_CHECK(hipGetLastError());
   // the grids are released at the end of this scope: wait for the kernel
