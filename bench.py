@@ -332,7 +332,9 @@ def bench_mls(args, S, T) -> int:
     """Moving-least-squares transfer, inputs in HBM.
 
     A step = plan setup (grids, k = 10 nearest sources and the QR weights of
-    every target) + one apply. Compulsory HBM bytes: sources 24 B read + the
+    every target) + one apply. CPU baseline: the oracle on 1000 sampled
+    targets (its neighbour search is brute force, so it is a port of the
+    method, not of a tuned search). Compulsory HBM bytes: sources 24 B read + the
     sorted copy (24 B + 4 B) written, value 8 B read; targets 24 B read, the
     k indices and weights (12 B each) written and read back, 8 B out.
     """
@@ -368,6 +370,22 @@ def bench_mls(args, S, T) -> int:
     apply_ms = (time.perf_counter() - a0) * 1e3 / 10
     ref = W.franke_3d(*T.T)
     u = out.cpu().numpy()
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+        import oracle
+        cores = min(16, os.cpu_count() or 1)
+        oracle.set_threads(cores)
+        sample = T[np.random.default_rng(12).choice(M, 1000, replace=False)]
+        fv = W.franke_3d(*S.T)
+        c0 = time.perf_counter()
+        oracle.mls_interpolate(3, S, fv, sample)
+        c1 = time.perf_counter()
+        cpu = {"value": len(sample) / (c1 - c0) / 1e6, "unit": "Mpts/s",
+               "cores": cores, "kind": "port",
+               "sample": f"1000 of the {M} targets against all {N} sources, "
+                         f"FP64 oracle (brute-force neighbour search), "
+                         f"{c1 - c0:.2f} s"}
     byts = N * (24 + 28 + 8) + M * (24 + 2 * 10 * 12 + 8)
     gbs = byts / (ms / 1e3) / 1e9
     res = {
@@ -384,9 +402,10 @@ def bench_mls(args, S, T) -> int:
                        ((u - ref) ** 2).sum() ** 0.5 / (ref ** 2).sum() ** 0.5)},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-                     "traffic": None, "kernel": "grid build + mls_coef_kernel "
-                     "+ mls_apply_kernel, compulsory bytes"},
-        "cpu_baseline": None,
+                     "traffic": None, "kernel": "grid build + mls_knn_kernel "
+                     "+ mls_weights_kernel + mls_apply_kernel, compulsory "
+                     "bytes"},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
     return 0
