@@ -55,6 +55,10 @@ def main() -> int:
                          "per step; mls: the moving-least-squares transfer "
                          "(setup + apply) on the same clouds; fe-nearest: the "
                          "nearest-source transfer on the same clouds")
+    ap.add_argument("--operator", action="store_true",
+                    help="apply mode: store the per-patch operator after the "
+                         "setup and apply with it (the build time is "
+                         "reported apart)")
     ap.add_argument("--graph", action="store_true",
                     help="apply mode: replay the apply as a HIP graph")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -254,6 +258,13 @@ def bench_apply(args, S, T, npc, support) -> int:
     vals = torch.from_numpy(W.trig_fields(S, F, seed=11)).to(dev)
     out = torch.empty((F, len(T)), dtype=torch.float64, device=dev)
     plan = pacman.rbf.Plan(3, args.rbf, dS, dT, npc, 0.15, support)
+    build_ms = None
+    if args.operator:
+        torch.cuda.synchronize()
+        b0 = time.perf_counter()
+        plan.build_operator()
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - b0) * 1e3
     for _ in range(max(args.warmup, 1)):
         plan.apply(vals, out=out, graph=args.graph)
     torch.cuda.synchronize()
@@ -272,8 +283,11 @@ def bench_apply(args, S, T, npc, support) -> int:
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: {len(S)} -> {M}, one setup, "
                                f"{F} trig fields per apply, graph="
-                               f"{bool(args.graph)}",
-                   "fields": F, "graph": bool(args.graph)},
+                               f"{bool(args.graph)}, stored operator="
+                               f"{bool(args.operator)}",
+                   "fields": F, "graph": bool(args.graph),
+                   "operator_build_ms": build_ms,
+                   "operator_gb": plan.operator_bytes / 1e9},
         "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)

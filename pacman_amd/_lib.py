@@ -73,6 +73,9 @@ SIGNATURES = {
     "pacman_mls_plan_k": (ctypes.c_int, [_vp]),
     "pacman_mls_plan_neighbours": (ctypes.c_int, [_vp, _vp, _vp]),
     "pacman_mls_plan_destroy": (ctypes.c_int, [_vp]),
+    "pacman_rbf_plan_build_operator": (ctypes.c_int, [_vp, _vp]),
+    "pacman_rbf_plan_drop_operator": (ctypes.c_int, [_vp]),
+    "pacman_rbf_plan_operator_bytes": (ctypes.c_longlong, [_vp]),
     "pacman_release_device_cache": (ctypes.c_int, []),
     "pacman_device_cache_bytes": (ctypes.c_longlong, []),
     "pacman_rbf_last_error": (ctypes.c_char_p, []),

@@ -749,6 +749,34 @@ int pacman_mls_plan_destroy(pacman_mls_plan plan) {
   return 0;
 }
 
+int pacman_rbf_plan_build_operator(pacman_rbf_plan plan, void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    plan->plan->buildOperator((hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_build_operator", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_build_operator", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_rbf_plan_drop_operator(pacman_rbf_plan plan) {
+  if (!plan || !plan->plan) {
+    set_error("pacman_rbf_plan_drop_operator", "plan is null");
+    return -1;
+  }
+  plan->plan->dropOperator();
+  return 0;
+}
+
+long long pacman_rbf_plan_operator_bytes(pacman_rbf_plan plan) {
+  return plan && plan->plan ? plan->plan->operatorBytes() : -1;
+}
+
 int pacman_release_device_cache(void) {
   pacman_amd::dev_release_cache();
   return 0;

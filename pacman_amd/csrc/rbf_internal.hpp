@@ -161,6 +161,10 @@ struct SolveArgs {
   const double *gSorted;    // F x M upstream values, target cell order
   int64_t M;
   double *gammaOut;         // (sum n) x F: A^-1 E^T (w g) per patch source
+  // Stored-operator build (transpose chain with unit upstream values): when
+  // >= 0, field f's upstream value is the Shepard weight at pair opRow0 + f
+  // of each patch and 0 at its other pairs (gSorted is not read).
+  int opRow0;
 };
 // Fields handled by one poly/solve launch.
 int solve_fields_per_launch();
@@ -186,6 +190,15 @@ void launch_solve_tile_adjoint(int kind, const SolveArgs &a, hipStream_t s);
 // gamma + q(x)^T G^-1 (V^T w g - Q^T gamma) at every patch source entry e.
 void launch_adjoint_poly(const SolveArgs &a, int64_t K, const double *gamma,
                          double *contribS, hipStream_t s);
+// Stored operator: op[opOff[k] + q * n_k + e] = the weight-scaled local
+// operator of patch k, row q (its pair q), column e (its source e). From the
+// transpose chain's per-source output of launches with opRow0 = q - f.
+void launch_op_store(const SolveArgs &a, int64_t K, const double *contribS,
+                     const int64_t *opOff, double *op, hipStream_t s);
+// contrib[slot(pair q of patch k) * F + f] = op row q . values of patch k
+// (a.vals: F x N, cell order), member pairs only.
+void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
+                     const double *op, hipStream_t s);
 // Source-major transpose of the patch source lists: srcTOff[N + 1] offsets,
 // srcTEnt entry indices (into srcIdx) per source in ascending patch order.
 void build_source_transpose(const int *srcIdx, int64_t total, int64_t N,
@@ -210,6 +223,9 @@ struct BlendArgs {
   double *out;              // F x M, original order
 };
 void launch_blend(const BlendArgs &a, hipStream_t s);
+// out[f * M + tperm[t]] = sum of contrib over the member slots of t (the
+// weights are inside the stored operator).
+void launch_blend_sum(const BlendArgs &a, hipStream_t s);
 // Shepard weight of every member slot (0 for the others).
 void launch_slot_weights(const BlendArgs &a, double *wslot, hipStream_t s);
 

@@ -361,6 +361,7 @@ SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
   a.pvOut = pv_.get();
   a.status = status_.get();
   a.M = M_;
+  a.opRow0 = -1;
   return a;
 }
 
@@ -477,8 +478,72 @@ void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
                            s);
 }
 
+// The local operator of every patch, scaled by the Shepard weights, kept in
+// HBM: row q of patch k maps the patch's source values to w * t at its pair
+// q. Built with the transpose chain (the same factorisation) on unit
+// upstream values, eight rows per launch.
+void RbfPumPlan::buildOperator(hipStream_t s) {
+  releaseGraph();
+  op_.release();
+  opBuilt_ = false;
+  if (M_ == 0)
+    return;
+  std::vector<int64_t> poff(K_ + 1), ooff(K_ + 1, 0);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(poff.data(), pairs_.off.get(),
+                                  8 * (K_ + 1), hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  int64_t maxRows = 0;
+  for (int64_t k = 0; k < K_; ++k) {
+    const int64_t m = poff[k + 1] - poff[k];
+    maxRows = std::max(maxRows, m);
+    ooff[k + 1] = ooff[k] + m * patchN_[k];
+  }
+  opOff_.alloc(K_ + 1);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(opOff_.get(), ooff.data(), 8 * (K_ + 1),
+                                  hipMemcpyHostToDevice, s));
+  op_.alloc(std::max<int64_t>(ooff[K_], 1));
+  const int Fb = solve_fields_per_launch();
+  const int64_t total = std::max<int64_t>(totalSrc_, 1);
+  if (wslot_.n < (size_t)std::max<int64_t>(pairs_.P, 1)) {
+    wslot_.alloc(std::max<int64_t>(pairs_.P, 1));
+    launch_slot_weights(blendArgs(1), wslot_.get(), s);
+  }
+  if (gamma_.n < (size_t)(total * Fb))
+    gamma_.alloc((size_t)total * Fb);
+  if (contribS_.n < (size_t)(total * Fb))
+    contribS_.alloc((size_t)total * Fb);
+  const Bins &B = bins_for(Fb, s);
+  SolveArgs a = solveArgs(Fb, nullptr);
+  a.wslot = wslot_.get();
+  a.gammaOut = gamma_.get();
+  for (int64_t r0 = 0; r0 < maxRows; r0 += Fb) {
+    a.opRow0 = (int)r0;
+    for (size_t b = 0; b < B.R.size(); ++b) {
+      a.patches = B.dev[b].get();
+      a.count = B.count[b];
+      launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
+    }
+    launch_adjoint_poly(a, K_, gamma_.get(), contribS_.get(), s);
+    launch_op_store(a, K_, contribS_.get(), opOff_.get(), op_.get(), s);
+  }
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  opBuilt_ = true;
+}
+
+void RbfPumPlan::dropOperator() {
+  releaseGraph();
+  op_.release();
+  opOff_.release();
+  opBuilt_ = false;
+}
+
+int64_t RbfPumPlan::operatorBytes() const {
+  return opBuilt_ ? (int64_t)(op_.n * sizeof(double)) : 0;
+}
+
 void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
                          hipStream_t s, bool events, double *dState) {
+  const bool useOp = opBuilt_ && dState == nullptr;
   const int64_t total = totalSrc_;
   const int perLaunch = solve_fields_per_launch();
   if (events)
@@ -490,6 +555,18 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
   for (int f0 = 0; f0 < F; f0 += perLaunch) {
     const int nf = std::min(perLaunch, F - f0);
     SolveArgs a = solveArgs(nf, valsSorted_.get() + (int64_t)f0 * N_);
+    if (useOp) {
+      if (f0 == 0 && events)
+        PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
+      launch_op_apply(a, K_, opOff_.get(), op_.get(), s);
+      BlendArgs bl = blendArgs(nf);
+      bl.out = dOut + (int64_t)f0 * M_;
+      if (F <= perLaunch)
+        blends.push_back(bl);
+      else
+        launch_blend_sum(bl, s);
+      continue;
+    }
     launch_poly(a, K_, coef_.get(), s);
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
@@ -516,8 +593,12 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
   }
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[2], s));
-  for (const BlendArgs &bl : blends)
-    launch_blend(bl, s);
+  for (const BlendArgs &bl : blends) {
+    if (useOp)
+      launch_blend_sum(bl, s);
+    else
+      launch_blend(bl, s);
+  }
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[3], s));
 }

@@ -46,6 +46,13 @@ public:
   void targetCoordGrad(const double *dState, int F, const double *dG,
                        double *dCoordGrad, hipStream_t s);
 
+  // Stores the weight-scaled local operator of every patch (sum over patches
+  // of pairs x sources doubles of HBM); later applies use it instead of
+  // solving: one read of the operator per apply. dropOperator frees it.
+  void buildOperator(hipStream_t s);
+  void dropOperator();
+  int64_t operatorBytes() const;
+
   const PlanParams &params() const { return prm_; }
   int64_t numSources() const { return N_; }
   int64_t numTargets() const { return M_; }
@@ -119,6 +126,9 @@ private:
   int maxN_ = 0;
   int64_t totalSrc_ = 0;  // sum over patches of the source counts
   DevBuf<int> pv_, status_;
+  bool opBuilt_ = false;
+  DevBuf<int64_t> opOff_;
+  DevBuf<double> op_;
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};
   hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

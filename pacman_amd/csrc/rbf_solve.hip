@@ -457,7 +457,9 @@ __global__ __launch_bounds__(256) void adjoint_poly_kernel(
       if (((unsigned)pe >> 31) == 0)
         continue;
       const int tp = pe & 0x7fffffff;
-      const double h = a.wslot[a.pairSlot[p0 + e]] * a.gSorted[f * a.M + tp];
+      const double ws = a.wslot[a.pairSlot[p0 + e]];
+      const double h = a.opRow0 >= 0 ? (e == a.opRow0 + f ? ws : 0.0)
+                                     : ws * a.gSorted[f * a.M + tp];
       r4[0] += h;
       r4[1] += (a.tx[tp] - c[0]) * sc * h;
       r4[2] += (a.ty[tp] - c[1]) * sc * h;
@@ -678,6 +680,140 @@ void launch_adjoint_poly(const SolveArgs &a, int64_t K, const double *gamma,
     return;
   adjoint_poly_kernel<<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, gamma,
                                                                contribS);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void op_store_kernel(
+    SolveArgs a, int K, const double *contribS, const int64_t *opOff,
+    double *op) {
+  const int patch = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (patch >= K)
+    return;
+  const int lane = threadIdx.x & 63;
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int mall = (int)(a.pairOff[patch + 1] - a.pairOff[patch]);
+  for (int f = 0; f < a.F; ++f) {
+    const int q = a.opRow0 + f;
+    if (q >= mall)
+      break;
+    double *row = op + opOff[patch] + (int64_t)q * n;
+    for (int e = lane; e < n; e += 64)
+      row[e] = contribS[(s0 + e) * a.F + f];
+  }
+}
+
+// One workgroup per patch: the patch's source values staged in LDS; 16
+// lanes per operator row (each row read as contiguous 128-byte pieces),
+// partial sums over the lanes e = l, l + 16, ... added by a fixed butterfly.
+constexpr int kOpBlock = 256;
+__global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
+    SolveArgs a, int K, const int64_t *opOff, const double *op) {
+  __shared__ double fv[kFMax][256];
+  const int patch = xcd_remap(blockIdx.x, K);
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int64_t p0 = a.pairOff[patch];
+  const int mall = (int)(a.pairOff[patch + 1] - p0);
+  const int F = a.F;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n; e += kOpBlock) {
+    const int p = a.srcIdx[s0 + e];
+    for (int f = 0; f < F; ++f)
+      fv[f][e] = a.vals[(int64_t)f * a.N + p];
+  }
+  __syncthreads();
+  const int lane = tid & 63, g = lane >> 4, l16 = lane & 15;
+  constexpr int kRowsPerPass = kOpBlock / 16;
+  for (int q0 = (tid >> 6) * 4; q0 < mall; q0 += kRowsPerPass) {
+    const int q = q0 + g;
+    const bool rv = q < mall && ((unsigned)a.pairTgt[p0 + q] >> 31) != 0;
+    double acc[kFMax];
+#pragma unroll
+    for (int f = 0; f < kFMax; ++f)
+      acc[f] = 0.0;
+    if (rv) {
+      const double *row = op + opOff[patch] + (int64_t)q * n;
+      int e = l16;
+      // four independent row loads in flight per lane
+      for (; e + 48 < n; e += 64) {
+        const double g0 = row[e], g1 = row[e + 16], g2 = row[e + 32],
+                     g3 = row[e + 48];
+#pragma unroll
+        for (int f = 0; f < kFMax; ++f)
+          if (f < F) {
+            acc[f] = fma(g0, fv[f][e], acc[f]);
+            acc[f] = fma(g1, fv[f][e + 16], acc[f]);
+            acc[f] = fma(g2, fv[f][e + 32], acc[f]);
+            acc[f] = fma(g3, fv[f][e + 48], acc[f]);
+          }
+      }
+      for (; e < n; e += 16) {
+        const double gv = row[e];
+#pragma unroll
+        for (int f = 0; f < kFMax; ++f)
+          if (f < F)
+            acc[f] = fma(gv, fv[f][e], acc[f]);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < kFMax; ++f)
+      if (f < F) {
+        double v = acc[f];
+        v += __shfl_xor(v, 8, 16);
+        v += __shfl_xor(v, 4, 16);
+        v += __shfl_xor(v, 2, 16);
+        v += __shfl_xor(v, 1, 16);
+        acc[f] = v;
+      }
+    if (rv && l16 == 0) {
+      const int slot = a.pairSlot[p0 + q];
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f)
+        if (f < F)
+          a.contrib[(int64_t)slot * F + f] = acc[f];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void blend_sum_kernel(BlendArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.M)
+    return;
+  const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
+  const int64_t o = a.tperm[t];
+  for (int f = 0; f < a.F; ++f) {
+    double acc = 0.0;
+    for (int64_t s = s0; s < s1; ++s)
+      if (a.slotMember[s])
+        acc += a.contrib[s * a.F + f];
+    a.out[(int64_t)f * a.M + o] = acc;
+  }
+}
+
+void launch_op_store(const SolveArgs &a, int64_t K, const double *contribS,
+                     const int64_t *opOff, double *op, hipStream_t s) {
+  if (K <= 0)
+    return;
+  op_store_kernel<<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, contribS,
+                                                          opOff, op);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
+                     const double *op, hipStream_t s) {
+  if (K <= 0)
+    return;
+  if (a.F < 1 || a.F > kFMax)
+    throw std::runtime_error("operator apply: 1 <= fields <= 8 per launch");
+  op_apply_kernel<<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff, op);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void launch_blend_sum(const BlendArgs &a, hipStream_t s) {
+  if (a.M <= 0)
+    return;
+  blend_sum_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 

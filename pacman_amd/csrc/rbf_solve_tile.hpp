@@ -309,13 +309,16 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
           y2 = a.tz[tp];
           w = a.wslot[a.pairSlot[p0 + e]];  // 0 unless a member slot
         }
+        const int orow = a.opRow0 >= 0 ? e - a.opRow0 : -1;
         sty[tid] = y0;
         sty[64 + tid] = y1;
         sty[128 + tid] = y2;
 #pragma unroll
         for (int f = 0; f < kFMax; ++f)
-          sth[f * 64 + tid] = (f < F && w != 0.0) ? w * a.gSorted[f * a.M + tp]
-                                                  : 0.0;
+          sth[f * 64 + tid] =
+              a.opRow0 >= 0 ? ((f < F && orow == f) ? w : 0.0)
+              : (f < F && w != 0.0) ? w * a.gSorted[f * a.M + tp]
+                                    : 0.0;
       }
       __syncthreads();
       if (j < n) {

@@ -258,6 +258,22 @@ class Plan:
             raise RuntimeError(_lib.last_error())
         return out
 
+    def build_operator(self, stream=None):
+        """Keep the weight-scaled local operator of every patch in device
+        memory; later applies read it instead of solving (see
+        operator_bytes for the memory it takes)."""
+        rc = self._lib.pacman_rbf_plan_build_operator(
+            self._h, ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def drop_operator(self):
+        self._lib.pacman_rbf_plan_drop_operator(self._h)
+
+    @property
+    def operator_bytes(self) -> int:
+        return int(self._lib.pacman_rbf_plan_operator_bytes(self._h))
+
     # -- introspection -----------------------------------------------------
     @property
     def radius(self) -> float:
