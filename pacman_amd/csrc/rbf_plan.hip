@@ -264,14 +264,16 @@ void RbfPumPlan::prepare(int F, hipStream_t s) {
   const int64_t P = std::max<int64_t>(pairs_.P, 1);
   if (valsSorted_.n < (size_t)(F * N_))
     valsSorted_.alloc((size_t)F * N_);
-  const int perLaunch = solve_fields_per_launch();
-  const int Fc = std::min(F, perLaunch);
+  const int perSolve = solve_fields_per_launch();
+  // fields per pass: up to kOpFMax over a stored operator, else per solve
+  const int Fc = std::min(F, opBuilt_ ? kOpFMax : perSolve);
   if (contrib_.n < (size_t)(P * Fc))
     contrib_.alloc((size_t)P * Fc);
-  if (coef_.n < (size_t)(std::max<int64_t>(K_, 1) * Fc * 4))
-    coef_.alloc((size_t)std::max<int64_t>(K_, 1) * Fc * 4);
-  for (int f0 = 0; f0 < F; f0 += perLaunch)
-    bins_for(std::min(perLaunch, F - f0), s);
+  const int Fs = std::min(F, perSolve);
+  if (coef_.n < (size_t)(std::max<int64_t>(K_, 1) * Fs * 4))
+    coef_.alloc((size_t)std::max<int64_t>(K_, 1) * Fs * 4);
+  for (int f0 = 0; f0 < F; f0 += perSolve)
+    bins_for(std::min(perSolve, F - f0), s);
 }
 
 void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
@@ -544,8 +546,9 @@ int64_t RbfPumPlan::operatorBytes() const {
 void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
                          hipStream_t s, bool events, double *dState) {
   const bool useOp = opBuilt_ && dState == nullptr;
+  // the stored operator takes up to kOpFMax fields per pass over it
+  const int perLaunch = useOp ? kOpFMax : solve_fields_per_launch();
   const int64_t total = totalSrc_;
-  const int perLaunch = solve_fields_per_launch();
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[0], s));
   launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);

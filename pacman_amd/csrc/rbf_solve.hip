@@ -707,9 +707,10 @@ __global__ __launch_bounds__(256) void op_store_kernel(
 // lanes per operator row (each row read as contiguous 128-byte pieces),
 // partial sums over the lanes e = l, l + 16, ... added by a fixed butterfly.
 constexpr int kOpBlock = 256;
+template <int FM>
 __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
     SolveArgs a, int K, const int64_t *opOff, const double *op) {
-  __shared__ double fv[kFMax][256];
+  __shared__ double fv[FM][256];
   const int patch = xcd_remap(blockIdx.x, K);
   const int64_t s0 = a.srcOff[patch];
   const int n = (int)(a.srcOff[patch + 1] - s0);
@@ -728,9 +729,9 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
   for (int q0 = (tid >> 6) * 4; q0 < mall; q0 += kRowsPerPass) {
     const int q = q0 + g;
     const bool rv = q < mall && ((unsigned)a.pairTgt[p0 + q] >> 31) != 0;
-    double acc[kFMax];
+    double acc[FM];
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
+    for (int f = 0; f < FM; ++f)
       acc[f] = 0.0;
     if (rv) {
       const double *row = op + opOff[patch] + (int64_t)q * n;
@@ -740,7 +741,7 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
         const double g0 = row[e], g1 = row[e + 16], g2 = row[e + 32],
                      g3 = row[e + 48];
 #pragma unroll
-        for (int f = 0; f < kFMax; ++f)
+        for (int f = 0; f < FM; ++f)
           if (f < F) {
             acc[f] = fma(g0, fv[f][e], acc[f]);
             acc[f] = fma(g1, fv[f][e + 16], acc[f]);
@@ -751,13 +752,13 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
       for (; e < n; e += 16) {
         const double gv = row[e];
 #pragma unroll
-        for (int f = 0; f < kFMax; ++f)
+        for (int f = 0; f < FM; ++f)
           if (f < F)
             acc[f] = fma(gv, fv[f][e], acc[f]);
       }
     }
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
+    for (int f = 0; f < FM; ++f)
       if (f < F) {
         double v = acc[f];
         v += __shfl_xor(v, 8, 16);
@@ -769,7 +770,7 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
     if (rv && l16 == 0) {
       const int slot = a.pairSlot[p0 + q];
 #pragma unroll
-      for (int f = 0; f < kFMax; ++f)
+      for (int f = 0; f < FM; ++f)
         if (f < F)
           a.contrib[(int64_t)slot * F + f] = acc[f];
     }
@@ -804,9 +805,10 @@ void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
                      const double *op, hipStream_t s) {
   if (K <= 0)
     return;
-  if (a.F < 1 || a.F > kFMax)
+  if (a.F < 1 || a.F > kOpFMax)
     throw std::runtime_error("operator apply: 1 <= fields <= 8 per launch");
-  op_apply_kernel<<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff, op);
+  op_apply_kernel<kOpFMax><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
+                                                            op);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
