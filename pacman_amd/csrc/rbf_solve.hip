@@ -783,12 +783,24 @@ __global__ __launch_bounds__(256) void blend_sum_kernel(BlendArgs a) {
     return;
   const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
   const int64_t o = a.tperm[t];
-  for (int f = 0; f < a.F; ++f) {
-    double acc = 0.0;
-    for (int64_t s = s0; s < s1; ++s)
-      if (a.slotMember[s])
-        acc += a.contrib[s * a.F + f];
-    a.out[(int64_t)f * a.M + o] = acc;
+  constexpr int kB = 8;
+  for (int f0 = 0; f0 < a.F; f0 += kB) {
+    double acc[kB];
+#pragma unroll
+    for (int f = 0; f < kB; ++f)
+      acc[f] = 0.0;
+    for (int64_t s = s0; s < s1; ++s) {
+      if (!a.slotMember[s])
+        continue;
+#pragma unroll
+      for (int f = 0; f < kB; ++f)
+        if (f0 + f < a.F)
+          acc[f] += a.contrib[s * a.F + f0 + f];
+    }
+#pragma unroll
+    for (int f = 0; f < kB; ++f)
+      if (f0 + f < a.F)
+        a.out[(int64_t)(f0 + f) * a.M + o] = acc[f];
   }
 }
 
