@@ -197,9 +197,8 @@ void launch_op_store(const SolveArgs &a, int64_t K, const double *contribS,
                      const int64_t *opOff, double *op, hipStream_t s);
 // contrib[slot(pair q of patch k) * F + f] = op row q . values of patch k
 // (a.vals: F x N, cell order), member pairs only; up to kOpFMax fields per
-// pass (more fields per pass make the kernel bound by its LDS reads of the
-// staged values: 32 fields in one pass ran slower than four passes of 8).
-constexpr int kOpFMax = 8;
+// pass over the operator (FP64 MFMA, 16 fields per output tile).
+constexpr int kOpFMax = 32;
 void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
                      const double *op, hipStream_t s);
 // Source-major transpose of the patch source lists: srcTOff[N + 1] offsets,

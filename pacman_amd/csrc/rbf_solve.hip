@@ -777,6 +777,77 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
   }
 }
 
+// MFMA form of the stored-operator apply: per patch, Out (rows x fields) =
+// G (rows x n) . V (n x fields), in 16 x 16 output tiles with FP64
+// 16x16x4 MFMAs. A wave takes a 16-row tile; lane (i + 16 k) loads the
+// 32-byte run G[i][c + 4k .. c + 4k + 3] of each 16-column chunk c, and step
+// s of the chunk multiplies column c + 4k + s of row i by the staged values
+// of the same column (the K dimension visits the chunk's columns in the
+// order k-major within each step; every column is used once). FT tiles of
+// 16 fields per pass (up to 32 fields).
+typedef double opd4 __attribute__((ext_vector_type(4)));
+template <int FT>
+__global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
+    SolveArgs a, int K, const int64_t *opOff, const double *op) {
+  constexpr int FW = 16 * FT;
+  __shared__ double fv[kMaxPatch][FW];
+  const int patch = xcd_remap(blockIdx.x, K);
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int64_t p0 = a.pairOff[patch];
+  const int mall = (int)(a.pairOff[patch + 1] - p0);
+  const int F = a.F;
+  const int tid = threadIdx.x;
+  for (int x = tid; x < n * FW; x += kOpBlock) {
+    const int e = x / FW, f = x % FW;
+    fv[e][f] = f < F ? a.vals[(int64_t)f * a.N + a.srcIdx[s0 + e]] : 0.0;
+  }
+  __syncthreads();
+  const int lane = tid & 63, i = lane & 15, k = lane >> 4;
+  const double *base = op + opOff[patch];
+  for (int rt = (tid >> 6); rt * 16 < mall; rt += kOpBlock / 64) {
+    const int r = rt * 16 + i;
+    const bool rok = r < mall;
+    const double *row = base + (int64_t)(rok ? r : 0) * n;
+    opd4 acc[FT];
+#pragma unroll
+    for (int t = 0; t < FT; ++t)
+      acc[t] = opd4{0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < n; c += 16) {
+      double g[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int col = c + 4 * k + s;
+        g[s] = (rok && col < n) ? row[col] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int col = c + 4 * k + s;
+#pragma unroll
+        for (int t = 0; t < FT; ++t) {
+          const double b = col < n ? fv[col][16 * t + i] : 0.0;
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[s], b, acc[t], 0, 0,
+                                                        0);
+        }
+      }
+    }
+    // lane l, register v: row l / 16 + 4 v of the tile, field l % 16
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = rt * 16 + k + 4 * v;
+      if (rr < mall && ((unsigned)a.pairTgt[p0 + rr] >> 31) != 0) {
+        const int slot = a.pairSlot[p0 + rr];
+#pragma unroll
+        for (int t = 0; t < FT; ++t) {
+          const int f = 16 * t + i;
+          if (f < F)
+            a.contrib[(int64_t)slot * F + f] = acc[t][v];
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void blend_sum_kernel(BlendArgs a) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.M)
@@ -818,9 +889,15 @@ void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
   if (K <= 0)
     return;
   if (a.F < 1 || a.F > kOpFMax)
-    throw std::runtime_error("operator apply: 1 <= fields <= 8 per launch");
-  op_apply_kernel<kOpFMax><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
-                                                            op);
+    throw std::runtime_error("operator apply: 1 <= fields <= 32 per launch");
+  if (a.F <= 8)  // a plain dot per row is as fast for few fields
+    op_apply_kernel<8><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff, op);
+  else if (a.F <= 16)
+    op_apply_mfma_kernel<1><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
+                                                             op);
+  else
+    op_apply_mfma_kernel<2><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
+                                                             op);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 

@@ -3,7 +3,8 @@
 The stored operator is built with the transpose chain on unit upstream
 values; the transfer it gives must equal the plan's ordinary apply (the
 forward chain) to FP64 round-off, for every basis, 2-D and 3-D, several
-fields (up to 8 per pass over the operator; 40 fields take five passes),
+fields (up to 32 per pass over the operator, an FP64 MFMA contraction above
+8 fields; 40 fields take two passes),
 targets much denser than the sources
 (many operator rows per patch), graph replays and after drop_operator.
 """
@@ -23,7 +24,8 @@ def _rel_max(a, b):
 
 @pytest.mark.parametrize("rbf,dim,F", [(0x11, 3, 1), (0x10, 3, 3),
                                        (0x14, 3, 1), (0x13, 2, 2),
-                                       (0x12, 3, 10), (0x11, 3, 40)])
+                                       (0x12, 3, 10), (0x11, 3, 20),
+                                       (0x11, 3, 40)])
 def test_operator_apply_equals_solve(rbf, dim, F):
     rng = np.random.default_rng(200 + rbf + dim + F)
     S = rng.uniform(0, 1, (6000, dim))
