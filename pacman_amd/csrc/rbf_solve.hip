@@ -809,28 +809,38 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
     const int r = rt * 16 + i;
     const bool rok = r < mall;
     const double *row = base + (int64_t)(rok ? r : 0) * n;
-    opd4 acc[FT];
+    // two accumulator sets over alternate 16-column chunks: independent
+    // MFMA chains, added once at the end
+    opd4 acc[FT], acc2[FT];
 #pragma unroll
-    for (int t = 0; t < FT; ++t)
+    for (int t = 0; t < FT; ++t) {
       acc[t] = opd4{0.0, 0.0, 0.0, 0.0};
-    for (int c = 0; c < n; c += 16) {
-      double g[4];
+      acc2[t] = opd4{0.0, 0.0, 0.0, 0.0};
+    }
+    for (int c = 0; c < n; c += 32) {
+      double g[8];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int col = c + 4 * k + s;
+      for (int s = 0; s < 8; ++s) {
+        const int col = c + 16 * (s >> 2) + 4 * k + (s & 3);
         g[s] = (rok && col < n) ? row[col] : 0.0;
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const int col = c + 4 * k + s;
+        const int col0 = c + 4 * k + s, col1 = col0 + 16;
 #pragma unroll
         for (int t = 0; t < FT; ++t) {
-          const double b = col < n ? fv[col][16 * t + i] : 0.0;
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[s], b, acc[t], 0, 0,
+          const double b0 = col0 < n ? fv[col0][16 * t + i] : 0.0;
+          const double b1 = col1 < n ? fv[col1][16 * t + i] : 0.0;
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[s], b0, acc[t], 0, 0,
                                                         0);
+          acc2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[4 + s], b1,
+                                                         acc2[t], 0, 0, 0);
         }
       }
     }
+#pragma unroll
+    for (int t = 0; t < FT; ++t)
+      acc[t] += acc2[t];
     // lane l, register v: row l / 16 + 4 v of the tile, field l % 16
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
