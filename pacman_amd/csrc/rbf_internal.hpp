@@ -218,6 +218,7 @@ struct BlendArgs {
   const double *contrib;    // P x F
   const double *tx, *ty, *tz;
   const int *tperm;         // cell order -> original target index
+  const int *tinv;          // original target index -> cell order
   const double *cx, *cy, *cz;
   int64_t M;
   int F;
@@ -228,6 +229,8 @@ void launch_blend(const BlendArgs &a, hipStream_t s);
 // out[f * M + tperm[t]] = sum of contrib over the member slots of t (the
 // weights are inside the stored operator).
 void launch_blend_sum(const BlendArgs &a, hipStream_t s);
+// inv[perm[p]] = p for p < n.
+void launch_invert_perm(const int *perm, int64_t n, int *inv, hipStream_t s);
 // Shepard weight of every member slot (0 for the others).
 void launch_slot_weights(const BlendArgs &a, double *wslot, hipStream_t s);
 

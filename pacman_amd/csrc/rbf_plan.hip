@@ -377,6 +377,7 @@ BlendArgs RbfPumPlan::blendArgs(int F) const {
   bl.ty = gt_.view.y;
   bl.tz = gt_.view.z;
   bl.tperm = gt_.view.perm;
+  bl.tinv = tinv_.get();
   bl.cx = cx_.get();
   bl.cy = cy_.get();
   bl.cz = cz_.get();
@@ -504,6 +505,8 @@ void RbfPumPlan::buildOperator(hipStream_t s) {
   PACMAN_HIP_CHECK(hipMemcpyAsync(opOff_.get(), ooff.data(), 8 * (K_ + 1),
                                   hipMemcpyHostToDevice, s));
   op_.alloc(std::max<int64_t>(ooff[K_], 1));
+  tinv_.alloc(std::max<int64_t>(M_, 1));
+  launch_invert_perm(gt_.view.perm, M_, tinv_.get(), s);
   const int Fb = solve_fields_per_launch();
   const int64_t total = std::max<int64_t>(totalSrc_, 1);
   if (wslot_.n < (size_t)std::max<int64_t>(pairs_.P, 1)) {

@@ -129,6 +129,7 @@ private:
   bool opBuilt_ = false;
   DevBuf<int64_t> opOff_;
   DevBuf<double> op_;
+  DevBuf<int> tinv_;  // original target index -> cell order (operator apply)
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};
   hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

@@ -911,10 +911,57 @@ void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
+// The same sum with one thread per target in original order: the outputs
+// are written contiguously and each slot's fields are one contiguous read;
+// for many fields this beats scattering F separate outputs per target.
+__global__ __launch_bounds__(256) void blend_sum_orig_kernel(BlendArgs a) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= a.M)
+    return;
+  const int64_t t = a.tinv[o];
+  const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
+  constexpr int kB = 8;
+  for (int f0 = 0; f0 < a.F; f0 += kB) {
+    double acc[kB];
+#pragma unroll
+    for (int f = 0; f < kB; ++f)
+      acc[f] = 0.0;
+    for (int64_t s = s0; s < s1; ++s) {
+      if (!a.slotMember[s])
+        continue;
+#pragma unroll
+      for (int f = 0; f < kB; ++f)
+        if (f0 + f < a.F)
+          acc[f] += a.contrib[s * a.F + f0 + f];
+    }
+#pragma unroll
+    for (int f = 0; f < kB; ++f)
+      if (f0 + f < a.F)
+        a.out[(int64_t)(f0 + f) * a.M + o] = acc[f];
+  }
+}
+
+__global__ void invert_perm_kernel(const int *perm, int64_t n, int *inv) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n)
+    inv[perm[p]] = (int)p;
+}
+
 void launch_blend_sum(const BlendArgs &a, hipStream_t s) {
   if (a.M <= 0)
     return;
-  blend_sum_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
+  if (a.F > 8 && a.tinv)
+    blend_sum_orig_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
+  else
+    blend_sum_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void launch_invert_perm(const int *perm, int64_t n, int *inv,
+                        hipStream_t s) {
+  if (n <= 0)
+    return;
+  invert_perm_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(perm, n, inv);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
