@@ -81,9 +81,9 @@ int pacman_rbf_plan_target_coord_grad(pacman_rbf_plan plan,
 /* Stored operator for repeated transfers (no reference counterpart): keeps
  * the weight-scaled local operator of every patch in device memory (sum over
  * patches of targets x sources doubles; pacman_rbf_plan_operator_bytes), so
- * that later applies, graph replays included, read it instead of solving
- * every patch again. Synchronous. drop_operator frees it; the gradient entry
- * points keep using the solve. */
+ * that later applies, graph replays included, and transpose applies read it
+ * instead of solving every patch again. Synchronous. drop_operator frees it;
+ * the target-coordinate gradient (apply_state) keeps using the solve. */
 int pacman_rbf_plan_build_operator(pacman_rbf_plan plan, void *stream);
 int pacman_rbf_plan_drop_operator(pacman_rbf_plan plan);
 long long pacman_rbf_plan_operator_bytes(pacman_rbf_plan plan);

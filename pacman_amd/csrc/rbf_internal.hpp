@@ -229,6 +229,11 @@ void launch_blend(const BlendArgs &a, hipStream_t s);
 // out[f * M + tperm[t]] = sum of contrib over the member slots of t (the
 // weights are inside the stored operator).
 void launch_blend_sum(const BlendArgs &a, hipStream_t s);
+// contribS[(srcOff[k] + e) * F + f] = sum over the member pairs q of patch k
+// of op row q, column e, times a.gSorted[f][target of q] (transpose apply
+// through the stored operator; fields <= 8 per launch).
+void launch_op_transpose(const SolveArgs &a, int64_t K, const int64_t *opOff,
+                         const double *op, double *contribS, hipStream_t s);
 // inv[perm[p]] = p for p < n.
 void launch_invert_perm(const int *perm, int64_t n, int *inv, hipStream_t s);
 // Shepard weight of every member slot (0 for the others).

@@ -424,12 +424,17 @@ void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
     a.wslot = wslot_.get();
     a.gSorted = gSorted_.get();
     a.gammaOut = gamma_.get();
-    for (size_t b = 0; b < B.R.size(); ++b) {
-      a.patches = B.dev[b].get();
-      a.count = B.count[b];
-      launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
+    if (opBuilt_) {
+      // the stored rows already hold the weights: a column sum per source
+      launch_op_transpose(a, K_, opOff_.get(), op_.get(), contribS_.get(), s);
+    } else {
+      for (size_t b = 0; b < B.R.size(); ++b) {
+        a.patches = B.dev[b].get();
+        a.count = B.count[b];
+        launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
+      }
+      launch_adjoint_poly(a, K_, gamma_.get(), contribS_.get(), s);
     }
-    launch_adjoint_poly(a, K_, gamma_.get(), contribS_.get(), s);
     launch_source_reduce(srcTOff_.get(), srcTEnt_.get(), N_, nf,
                          contribS_.get(), gs_.view.perm,
                          dGrad + (int64_t)f0 * N_, s);

@@ -941,6 +941,63 @@ __global__ __launch_bounds__(256) void blend_sum_orig_kernel(BlendArgs a) {
   }
 }
 
+// Transpose through the stored operator: contribS[(s0 + e) * F + f] =
+// sum over the member rows q of patch k of op[q][e] g_f(y_q) (rows in
+// ascending order, staged 256 at a time); one thread per source column
+// (consecutive threads read consecutive columns of a row).
+__global__ __launch_bounds__(256) void op_transpose_kernel(
+    SolveArgs a, int K, const int64_t *opOff, const double *op,
+    double *contribS) {
+  __shared__ double gv[kFMax][256];
+  const int patch = xcd_remap(blockIdx.x, K);
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int64_t p0 = a.pairOff[patch];
+  const int mall = (int)(a.pairOff[patch + 1] - p0);
+  const int F = a.F;
+  const int tid = threadIdx.x;
+  const double *base = op + opOff[patch];
+  double acc[kFMax];
+#pragma unroll
+  for (int f = 0; f < kFMax; ++f)
+    acc[f] = 0.0;
+  for (int q0 = 0; q0 < mall; q0 += 256) {
+    const int q = q0 + tid;
+    if (q < mall) {
+      const int pe = a.pairTgt[p0 + q];
+      const bool mem = ((unsigned)pe >> 31) != 0;
+      const int tp = pe & 0x7fffffff;
+      for (int f = 0; f < F; ++f)
+        gv[f][tid] = mem ? a.gSorted[(int64_t)f * a.M + tp] : 0.0;
+    }
+    __syncthreads();
+    const int qn = min(256, mall - q0);
+    if (tid < n)
+      for (int j = 0; j < qn; ++j) {
+        const double g = base[(int64_t)(q0 + j) * n + tid];
+#pragma unroll
+        for (int f = 0; f < kFMax; ++f)
+          if (f < F)
+            acc[f] = fma(g, gv[f][j], acc[f]);
+      }
+    __syncthreads();
+  }
+  if (tid < n)
+    for (int f = 0; f < F; ++f)
+      contribS[(s0 + tid) * F + f] = acc[f];
+}
+
+void launch_op_transpose(const SolveArgs &a, int64_t K, const int64_t *opOff,
+                         const double *op, double *contribS, hipStream_t s) {
+  if (K <= 0)
+    return;
+  if (a.F < 1 || a.F > kFMax)
+    throw std::runtime_error("operator transpose: 1 <= fields <= 8 per launch");
+  op_transpose_kernel<<<(unsigned)K, 256, 0, s>>>(a, (int)K, opOff, op,
+                                                  contribS);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
 __global__ void invert_perm_kernel(const int *perm, int64_t n, int *inv) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n)

@@ -76,3 +76,23 @@ def test_operator_at_1m_points():
     plan.build_operator()
     got = plan.apply(v).cpu().numpy()
     assert _rel_max(got, ref) < 1e-10
+
+
+@pytest.mark.parametrize("F", [1, 3, 11])
+def test_operator_transpose_equals_solve(F):
+    """The transpose apply (source-value gradient) through the stored
+    operator equals the transpose chain's, and the pair keeps the
+    dot-product identity."""
+    rng = np.random.default_rng(300 + F)
+    S = rng.uniform(0, 1, (5000, 3))
+    T = rng.uniform(0, 1, (4000, 3))
+    g = rng.normal(size=(F, len(T)))
+    f = rng.normal(size=(F, len(S)))
+    plan = pacman.rbf.Plan(3, 0x11, S, T, 60, 0.15, -1.0)
+    ref = plan.apply_transpose(g)
+    plan.build_operator()
+    got = plan.apply_transpose(g)
+    assert _rel_max(got, ref) < 1e-10
+    lhs = np.sum(g * plan.apply(f))
+    rhs = np.sum(got * f)
+    assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + abs(rhs))
