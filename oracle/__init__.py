@@ -1,6 +1,7 @@
 """CPU FP64 oracle for the RBF-PUM transfer path (test infrastructure only).
 
-This package is the checker. Only ``tests/``, ``__graft_entry__.smoke()`` and
+This package checks results; it is never the thing measured or shipped.
+Only ``tests/``, ``__graft_entry__.smoke()`` and
 the ``cpu_baseline`` leg of ``bench.py`` may import it. The product library
 (``pacman_amd``) never loads it.
 

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
 // calls this file; only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg use it, and only as the checker.
+// cpu_baseline leg use it, and only to check or time against.
 //
 // The reference (src/mls/interpolator.hpp:47-75, src/mls_interface.cpp:
 // 29-107) hands the points to ArborX::Interpolation::MovingLeastSquares with

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
 // calls this file; only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg use it, and only as the checker.
+// cpu_baseline leg use it, and only to check or time against.
 //
 // It restates, step by step, the reference algorithm:
 //   FindRadius        src/rbf_pum/clustering/clusters_radius.hpp:22-125
