@@ -59,6 +59,10 @@ def main() -> int:
                     help="apply mode: store the per-patch operator after the "
                          "setup and apply with it (the build time is "
                          "reported apart)")
+    ap.add_argument("--operator-fp32", action="store_true",
+                    help="apply mode: as --operator, with the operator stored "
+                         "in FP32 (FP64 sums); the max relative difference "
+                         "from the FP64 solve is reported")
     ap.add_argument("--graph", action="store_true",
                     help="apply mode: replay the apply as a HIP graph")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -258,14 +262,22 @@ def bench_apply(args, S, T, npc, support) -> int:
     vals = torch.from_numpy(W.trig_fields(S, F, seed=11)).to(dev)
     out = torch.empty((F, len(T)), dtype=torch.float64, device=dev)
     plan = pacman.rbf.Plan(3, args.rbf, dS, dT, npc, 0.15, support)
-    build_ms = None
-    if args.operator:
+    build_ms = fp32_err = None
+    use_op = args.operator or args.operator_fp32
+    prec = "fp32" if args.operator_fp32 else "fp64"
+    if use_op:
+        ref = plan.apply(vals) if args.operator_fp32 else None
         torch.cuda.synchronize()
         b0 = time.perf_counter()
-        plan.build_operator()
+        plan.build_operator(precision=prec)
         torch.cuda.synchronize()
         build_ms = (time.perf_counter() - b0) * 1e3
-    for _ in range(max(args.warmup, 1)):
+        if ref is not None:
+            got = plan.apply(vals)
+            fp32_err = float((got - ref).abs().max() / ref.abs().max())
+            del got, ref
+    for _ in range(max(args.warmup, 1)):   # the timed step, untimed
+        vals.mul_(-1.0)
         plan.apply(vals, out=out, graph=args.graph)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -284,9 +296,11 @@ def bench_apply(args, S, T, npc, support) -> int:
         "config": {"workload": f"{args.config}: {len(S)} -> {M}, one setup, "
                                f"{F} trig fields per apply, graph="
                                f"{bool(args.graph)}, stored operator="
-                               f"{bool(args.operator)}",
+                               f"{prec if use_op else False}",
                    "fields": F, "graph": bool(args.graph),
                    "operator_build_ms": build_ms,
+                   "operator_precision": prec if use_op else None,
+                   "operator_fp32_max_rel_vs_fp64": fp32_err,
                    "operator_gb": plan.operator_bytes / 1e9},
         "cpu_baseline": None,
     }

@@ -85,6 +85,10 @@ int pacman_rbf_plan_target_coord_grad(pacman_rbf_plan plan,
  * instead of solving every patch again. Synchronous. drop_operator frees it;
  * the target-coordinate gradient (apply_state) keeps using the solve. */
 int pacman_rbf_plan_build_operator(pacman_rbf_plan plan, void *stream);
+/* The same with the operator rounded to FP32 (half the memory and the bytes
+ * each apply reads; products and sums stay FP64). Results differ from the
+ * FP64 operator's by the rounding of its entries, about 1e-7 relative. */
+int pacman_rbf_plan_build_operator_fp32(pacman_rbf_plan plan, void *stream);
 int pacman_rbf_plan_drop_operator(pacman_rbf_plan plan);
 long long pacman_rbf_plan_operator_bytes(pacman_rbf_plan plan);
 

@@ -74,6 +74,7 @@ SIGNATURES = {
     "pacman_mls_plan_neighbours": (ctypes.c_int, [_vp, _vp, _vp]),
     "pacman_mls_plan_destroy": (ctypes.c_int, [_vp]),
     "pacman_rbf_plan_build_operator": (ctypes.c_int, [_vp, _vp]),
+    "pacman_rbf_plan_build_operator_fp32": (ctypes.c_int, [_vp, _vp]),
     "pacman_rbf_plan_drop_operator": (ctypes.c_int, [_vp]),
     "pacman_rbf_plan_operator_bytes": (ctypes.c_longlong, [_vp]),
     "pacman_release_device_cache": (ctypes.c_int, []),

@@ -764,6 +764,21 @@ int pacman_rbf_plan_build_operator(pacman_rbf_plan plan, void *stream) {
   }
 }
 
+int pacman_rbf_plan_build_operator_fp32(pacman_rbf_plan plan, void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    plan->plan->buildOperator((hipStream_t)stream, true);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_build_operator_fp32", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_build_operator_fp32", "unknown exception");
+    return -2;
+  }
+}
+
 int pacman_rbf_plan_drop_operator(pacman_rbf_plan plan) {
   if (!plan || !plan->plan) {
     set_error("pacman_rbf_plan_drop_operator", "plan is null");

@@ -190,17 +190,23 @@ void launch_solve_tile_adjoint(int kind, const SolveArgs &a, hipStream_t s);
 // gamma + q(x)^T G^-1 (V^T w g - Q^T gamma) at every patch source entry e.
 void launch_adjoint_poly(const SolveArgs &a, int64_t K, const double *gamma,
                          double *contribS, hipStream_t s);
+// Storage of the stored operator: FP64, or FP32 applied with FP64
+// accumulation. Exactly one of the pointers is set.
+struct OpStore {
+  double *f64 = nullptr;
+  float *f32 = nullptr;
+};
 // Stored operator: op[opOff[k] + q * n_k + e] = the weight-scaled local
 // operator of patch k, row q (its pair q), column e (its source e). From the
 // transpose chain's per-source output of launches with opRow0 = q - f.
 void launch_op_store(const SolveArgs &a, int64_t K, const double *contribS,
-                     const int64_t *opOff, double *op, hipStream_t s);
+                     const int64_t *opOff, const OpStore &op, hipStream_t s);
 // contrib[slot(pair q of patch k) * F + f] = op row q . values of patch k
 // (a.vals: F x N, cell order), member pairs only; up to kOpFMax fields per
 // pass over the operator (FP64 MFMA, 16 fields per output tile).
 constexpr int kOpFMax = 32;
 void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
-                     const double *op, hipStream_t s);
+                     const OpStore &op, hipStream_t s);
 // Source-major transpose of the patch source lists: srcTOff[N + 1] offsets,
 // srcTEnt entry indices (into srcIdx) per source in ascending patch order.
 void build_source_transpose(const int *srcIdx, int64_t total, int64_t N,
@@ -233,7 +239,7 @@ void launch_blend_sum(const BlendArgs &a, hipStream_t s);
 // of op row q, column e, times a.gSorted[f][target of q] (transpose apply
 // through the stored operator; fields <= 8 per launch).
 void launch_op_transpose(const SolveArgs &a, int64_t K, const int64_t *opOff,
-                         const double *op, double *contribS, hipStream_t s);
+                         const OpStore &op, double *contribS, hipStream_t s);
 // inv[perm[p]] = p for p < n.
 void launch_invert_perm(const int *perm, int64_t n, int *inv, hipStream_t s);
 // Shepard weight of every member slot (0 for the others).

@@ -426,7 +426,7 @@ void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
     a.gammaOut = gamma_.get();
     if (opBuilt_) {
       // the stored rows already hold the weights: a column sum per source
-      launch_op_transpose(a, K_, opOff_.get(), op_.get(), contribS_.get(), s);
+      launch_op_transpose(a, K_, opOff_.get(), opStore(), contribS_.get(), s);
     } else {
       for (size_t b = 0; b < B.R.size(); ++b) {
         a.patches = B.dev[b].get();
@@ -489,10 +489,13 @@ void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
 // The local operator of every patch, scaled by the Shepard weights, kept in
 // HBM: row q of patch k maps the patch's source values to w * t at its pair
 // q. Built with the transpose chain (the same factorisation) on unit
-// upstream values, eight rows per launch.
-void RbfPumPlan::buildOperator(hipStream_t s) {
+// upstream values, eight rows per launch; stored in FP64, or rounded to FP32
+// (halves its memory and the bytes each apply reads; applied with FP64
+// accumulation).
+void RbfPumPlan::buildOperator(hipStream_t s, bool fp32) {
   releaseGraph();
   op_.release();
+  op32_.release();
   opBuilt_ = false;
   if (M_ == 0)
     return;
@@ -509,7 +512,10 @@ void RbfPumPlan::buildOperator(hipStream_t s) {
   opOff_.alloc(K_ + 1);
   PACMAN_HIP_CHECK(hipMemcpyAsync(opOff_.get(), ooff.data(), 8 * (K_ + 1),
                                   hipMemcpyHostToDevice, s));
-  op_.alloc(std::max<int64_t>(ooff[K_], 1));
+  if (fp32)
+    op32_.alloc(std::max<int64_t>(ooff[K_], 1));
+  else
+    op_.alloc(std::max<int64_t>(ooff[K_], 1));
   tinv_.alloc(std::max<int64_t>(M_, 1));
   launch_invert_perm(gt_.view.perm, M_, tinv_.get(), s);
   const int Fb = solve_fields_per_launch();
@@ -534,7 +540,7 @@ void RbfPumPlan::buildOperator(hipStream_t s) {
       launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
     }
     launch_adjoint_poly(a, K_, gamma_.get(), contribS_.get(), s);
-    launch_op_store(a, K_, contribS_.get(), opOff_.get(), op_.get(), s);
+    launch_op_store(a, K_, contribS_.get(), opOff_.get(), opStore(), s);
   }
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   opBuilt_ = true;
@@ -543,12 +549,23 @@ void RbfPumPlan::buildOperator(hipStream_t s) {
 void RbfPumPlan::dropOperator() {
   releaseGraph();
   op_.release();
+  op32_.release();
   opOff_.release();
   opBuilt_ = false;
 }
 
 int64_t RbfPumPlan::operatorBytes() const {
-  return opBuilt_ ? (int64_t)(op_.n * sizeof(double)) : 0;
+  return opBuilt_ ? (int64_t)(op_.n * sizeof(double) + op32_.n * sizeof(float))
+                  : 0;
+}
+
+OpStore RbfPumPlan::opStore() const {
+  OpStore o;
+  if (op32_.n)
+    o.f32 = op32_.get();
+  else
+    o.f64 = op_.get();
+  return o;
 }
 
 void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
@@ -569,7 +586,7 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     if (useOp) {
       if (f0 == 0 && events)
         PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
-      launch_op_apply(a, K_, opOff_.get(), op_.get(), s);
+      launch_op_apply(a, K_, opOff_.get(), opStore(), s);
       BlendArgs bl = blendArgs(nf);
       bl.out = dOut + (int64_t)f0 * M_;
       if (F <= perLaunch)

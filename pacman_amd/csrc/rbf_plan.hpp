@@ -47,9 +47,10 @@ public:
                        double *dCoordGrad, hipStream_t s);
 
   // Stores the weight-scaled local operator of every patch (sum over patches
-  // of pairs x sources doubles of HBM); later applies use it instead of
-  // solving: one read of the operator per apply. dropOperator frees it.
-  void buildOperator(hipStream_t s);
+  // of pairs x sources entries of HBM, FP64 or with fp32 FP32); later applies
+  // use it instead of solving: one read of the operator per apply.
+  // dropOperator frees it.
+  void buildOperator(hipStream_t s, bool fp32 = false);
   void dropOperator();
   int64_t operatorBytes() const;
 
@@ -129,6 +130,8 @@ private:
   bool opBuilt_ = false;
   DevBuf<int64_t> opOff_;
   DevBuf<double> op_;
+  DevBuf<float> op32_;  // set instead of op_ for an FP32 operator
+  OpStore opStore() const;
   DevBuf<int> tinv_;  // original target index -> cell order (operator apply)
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};

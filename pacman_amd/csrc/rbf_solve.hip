@@ -683,9 +683,9 @@ void launch_adjoint_poly(const SolveArgs &a, int64_t K, const double *gamma,
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void op_store_kernel(
-    SolveArgs a, int K, const double *contribS, const int64_t *opOff,
-    double *op) {
+    SolveArgs a, int K, const double *contribS, const int64_t *opOff, T *op) {
   const int patch = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (patch >= K)
     return;
@@ -697,20 +697,24 @@ __global__ __launch_bounds__(256) void op_store_kernel(
     const int q = a.opRow0 + f;
     if (q >= mall)
       break;
-    double *row = op + opOff[patch] + (int64_t)q * n;
+    T *row = op + opOff[patch] + (int64_t)q * n;
     for (int e = lane; e < n; e += 64)
-      row[e] = contribS[(s0 + e) * a.F + f];
+      row[e] = (T)contribS[(s0 + e) * a.F + f];
   }
 }
 
-// One workgroup per patch: the patch's source values staged in LDS; 16
-// lanes per operator row (each row read as contiguous 128-byte pieces),
-// partial sums over the lanes e = l, l + 16, ... added by a fixed butterfly.
+// One workgroup per patch: the patch's source values staged in LDS (zero
+// past the patch's n sources); 16 lanes per operator row, lane l reading
+// columns l, l + 16, ... (each row read as contiguous 128-byte pieces), all
+// of a row's loads and its pair's target and slot issued before the first
+// use; partial sums over the lanes added by a fixed butterfly.
 constexpr int kOpBlock = 256;
-template <int FM>
-__global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
-    SolveArgs a, int K, const int64_t *opOff, const double *op) {
-  __shared__ double fv[FM][256];
+template <int FM, class T>
+__global__ __launch_bounds__(kOpBlock) __attribute__((amdgpu_waves_per_eu(
+    FM <= 2 ? 4 : FM <= 4 ? 3 : 2))) void op_apply_kernel(
+    SolveArgs a, int K, const int64_t *opOff, const T *op) {
+  constexpr int kCols = kMaxPatch / 16;  // loads per lane and row
+  __shared__ double fv[FM][kMaxPatch];
   const int patch = xcd_remap(blockIdx.x, K);
   const int64_t s0 = a.srcOff[patch];
   const int n = (int)(a.srcOff[patch + 1] - s0);
@@ -718,43 +722,58 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
   const int mall = (int)(a.pairOff[patch + 1] - p0);
   const int F = a.F;
   const int tid = threadIdx.x;
-  for (int e = tid; e < n; e += kOpBlock) {
-    const int p = a.srcIdx[s0 + e];
-    for (int f = 0; f < F; ++f)
-      fv[f][e] = a.vals[(int64_t)f * a.N + p];
+  // fields F .. FM - 1 and columns past n are staged as zeros
+  for (int e = tid; e < kMaxPatch; e += kOpBlock) {
+    const int p = e < n ? a.srcIdx[s0 + e] : 0;
+#pragma unroll
+    for (int f = 0; f < FM; ++f)
+      fv[f][e] = (e < n && f < F) ? a.vals[(int64_t)f * a.N + p] : 0.0;
   }
   __syncthreads();
   const int lane = tid & 63, g = lane >> 4, l16 = lane & 15;
   constexpr int kRowsPerPass = kOpBlock / 16;
+  const T *base = op + opOff[patch];
   for (int q0 = (tid >> 6) * 4; q0 < mall; q0 += kRowsPerPass) {
     const int q = q0 + g;
-    const bool rv = q < mall && ((unsigned)a.pairTgt[p0 + q] >> 31) != 0;
+    const int qc = min(q, mall - 1);
+    const int pe = a.pairTgt[p0 + qc];
+    const int slot = a.pairSlot[p0 + qc];
+    // columns past n read the row's last entry; their staged value is 0
+    const T *row = base + (int64_t)qc * n;
     double acc[FM];
 #pragma unroll
     for (int f = 0; f < FM; ++f)
       acc[f] = 0.0;
-    if (rv) {
-      const double *row = op + opOff[patch] + (int64_t)q * n;
-      int e = l16;
-      // four independent row loads in flight per lane
-      for (; e + 48 < n; e += 64) {
-        const double g0 = row[e], g1 = row[e + 16], g2 = row[e + 32],
-                     g3 = row[e + 48];
+    if constexpr (FM <= 4) {
+      // every load of the row in flight at once
+      T raw[kCols];
 #pragma unroll
-        for (int f = 0; f < FM; ++f)
-          if (f < F) {
-            acc[f] = fma(g0, fv[f][e], acc[f]);
-            acc[f] = fma(g1, fv[f][e + 16], acc[f]);
-            acc[f] = fma(g2, fv[f][e + 32], acc[f]);
-            acc[f] = fma(g3, fv[f][e + 48], acc[f]);
-          }
+      for (int j = 0; j < kCols; ++j) {
+        raw[j] = (T)0;
+        if (16 * j < n)  // uniform over the block
+          raw[j] = row[min(l16 + 16 * j, n - 1)];
       }
-      for (; e < n; e += 16) {
-        const double gv = row[e];
+#pragma unroll
+      for (int j = 0; j < kCols; ++j) {
+        const double gv = (double)raw[j];
 #pragma unroll
         for (int f = 0; f < FM; ++f)
-          if (f < F)
-            acc[f] = fma(gv, fv[f][e], acc[f]);
+          acc[f] = fma(gv, fv[f][l16 + 16 * j], acc[f]);
+      }
+    } else {
+      // four loads in flight per step (fewer registers for many fields)
+      for (int j0 = 0; 16 * j0 < n; j0 += 4) {
+        T raw[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          raw[j] = row[min(l16 + 16 * (j0 + j), n - 1)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double gv = (double)raw[j];
+#pragma unroll
+          for (int f = 0; f < FM; ++f)
+            acc[f] = fma(gv, fv[f][l16 + 16 * (j0 + j)], acc[f]);
+        }
       }
     }
 #pragma unroll
@@ -767,8 +786,7 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
         v += __shfl_xor(v, 1, 16);
         acc[f] = v;
       }
-    if (rv && l16 == 0) {
-      const int slot = a.pairSlot[p0 + q];
+    if (q < mall && ((unsigned)pe >> 31) != 0 && l16 == 0) {
 #pragma unroll
       for (int f = 0; f < FM; ++f)
         if (f < F)
@@ -786,9 +804,9 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_kernel(
 // order k-major within each step; every column is used once). FT tiles of
 // 16 fields per pass (up to 32 fields).
 typedef double opd4 __attribute__((ext_vector_type(4)));
-template <int FT>
+template <int FT, class T>
 __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
-    SolveArgs a, int K, const int64_t *opOff, const double *op) {
+    SolveArgs a, int K, const int64_t *opOff, const T *op) {
   constexpr int FW = 16 * FT;
   __shared__ double fv[kMaxPatch][FW];
   const int patch = xcd_remap(blockIdx.x, K);
@@ -798,17 +816,22 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
   const int mall = (int)(a.pairOff[patch + 1] - p0);
   const int F = a.F;
   const int tid = threadIdx.x;
-  for (int x = tid; x < n * FW; x += kOpBlock) {
+  // rows n .. (n rounded up to 32) - 1 are zero: the B operand needs no
+  // column mask
+  const int nr = (n + 31) & ~31;
+  for (int x = tid; x < nr * FW; x += kOpBlock) {
     const int e = x / FW, f = x % FW;
-    fv[e][f] = f < F ? a.vals[(int64_t)f * a.N + a.srcIdx[s0 + e]] : 0.0;
+    fv[e][f] = (f < F && e < n)
+                   ? a.vals[(int64_t)f * a.N + a.srcIdx[s0 + e]]
+                   : 0.0;
   }
   __syncthreads();
   const int lane = tid & 63, i = lane & 15, k = lane >> 4;
-  const double *base = op + opOff[patch];
+  const T *base = op + opOff[patch];
   for (int rt = (tid >> 6); rt * 16 < mall; rt += kOpBlock / 64) {
     const int r = rt * 16 + i;
     const bool rok = r < mall;
-    const double *row = base + (int64_t)(rok ? r : 0) * n;
+    const T *row = base + (int64_t)(rok ? r : 0) * n;
     // two accumulator sets over alternate 16-column chunks: independent
     // MFMA chains, added once at the end
     opd4 acc[FT], acc2[FT];
@@ -817,20 +840,43 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
       acc[t] = opd4{0.0, 0.0, 0.0, 0.0};
       acc2[t] = opd4{0.0, 0.0, 0.0, 0.0};
     }
+    // Eight operator loads per 32-column chunk, issued unconditionally: the
+    // column is clamped into the row (its B entry is zero, so the finite
+    // value read there adds nothing) and rows past the patch's read row 0
+    // (their outputs are not stored). The next chunk's loads are issued
+    // before the current chunk's MFMAs.
+    T raw[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      raw[s] = row[min(16 * (s >> 2) + 4 * k + (s & 3), n - 1)];
+    // the output rows of this lane (l / 16 + 4 v): target flag and slot,
+    // loaded before the MFMAs; slot -1 for rows not stored
+    int oslot[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = rt * 16 + k + 4 * v;
+      const int rc = min(rr, mall - 1);
+      const int pe = a.pairTgt[p0 + rc];
+      const int sl = a.pairSlot[p0 + rc];
+      oslot[v] = (rr < mall && ((unsigned)pe >> 31) != 0) ? sl : -1;
+    }
     for (int c = 0; c < n; c += 32) {
       double g[8];
 #pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const int col = c + 16 * (s >> 2) + 4 * k + (s & 3);
-        g[s] = (rok && col < n) ? row[col] : 0.0;
+      for (int s = 0; s < 8; ++s)
+        g[s] = (double)raw[s];
+      if (c + 32 < n) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          raw[s] = row[min(c + 32 + 16 * (s >> 2) + 4 * k + (s & 3), n - 1)];
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int col0 = c + 4 * k + s, col1 = col0 + 16;
 #pragma unroll
         for (int t = 0; t < FT; ++t) {
-          const double b0 = col0 < n ? fv[col0][16 * t + i] : 0.0;
-          const double b1 = col1 < n ? fv[col1][16 * t + i] : 0.0;
+          const double b0 = fv[col0][16 * t + i];
+          const double b1 = fv[col1][16 * t + i];
           acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[s], b0, acc[t], 0, 0,
                                                         0);
           acc2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[4 + s], b1,
@@ -844,9 +890,8 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
     // lane l, register v: row l / 16 + 4 v of the tile, field l % 16
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int rr = rt * 16 + k + 4 * v;
-      if (rr < mall && ((unsigned)a.pairTgt[p0 + rr] >> 31) != 0) {
-        const int slot = a.pairSlot[p0 + rr];
+      if (oslot[v] >= 0) {
+        const int slot = oslot[v];
 #pragma unroll
         for (int t = 0; t < FT; ++t) {
           const int f = 16 * t + i;
@@ -886,28 +931,51 @@ __global__ __launch_bounds__(256) void blend_sum_kernel(BlendArgs a) {
 }
 
 void launch_op_store(const SolveArgs &a, int64_t K, const double *contribS,
-                     const int64_t *opOff, double *op, hipStream_t s) {
+                     const int64_t *opOff, const OpStore &op, hipStream_t s) {
   if (K <= 0)
     return;
-  op_store_kernel<<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, contribS,
-                                                          opOff, op);
+  const unsigned blocks = (unsigned)((K + 3) / 4);
+  if (op.f32)
+    op_store_kernel<<<blocks, 256, 0, s>>>(a, (int)K, contribS, opOff, op.f32);
+  else
+    op_store_kernel<<<blocks, 256, 0, s>>>(a, (int)K, contribS, opOff, op.f64);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
+template <class T>
+static void op_apply_launch(const SolveArgs &a, int64_t K,
+                            const int64_t *opOff, const T *op, hipStream_t s) {
+  // a plain dot per row is as fast for few fields
+  if (a.F == 1)
+    op_apply_kernel<1, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
+                                                           op);
+  else if (a.F == 2)
+    op_apply_kernel<2, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
+                                                           op);
+  else if (a.F <= 4)
+    op_apply_kernel<4, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
+                                                           op);
+  else if (a.F <= 8)
+    op_apply_kernel<8, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
+                                                           op);
+  else if (a.F <= 16)
+    op_apply_mfma_kernel<1, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K,
+                                                                opOff, op);
+  else
+    op_apply_mfma_kernel<2, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K,
+                                                                opOff, op);
+}
+
 void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
-                     const double *op, hipStream_t s) {
+                     const OpStore &op, hipStream_t s) {
   if (K <= 0)
     return;
   if (a.F < 1 || a.F > kOpFMax)
     throw std::runtime_error("operator apply: 1 <= fields <= 32 per launch");
-  if (a.F <= 8)  // a plain dot per row is as fast for few fields
-    op_apply_kernel<8><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff, op);
-  else if (a.F <= 16)
-    op_apply_mfma_kernel<1><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
-                                                             op);
+  if (op.f32)
+    op_apply_launch(a, K, opOff, (const float *)op.f32, s);
   else
-    op_apply_mfma_kernel<2><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
-                                                             op);
+    op_apply_launch(a, K, opOff, (const double *)op.f64, s);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
@@ -945,9 +1013,9 @@ __global__ __launch_bounds__(256) void blend_sum_orig_kernel(BlendArgs a) {
 // sum over the member rows q of patch k of op[q][e] g_f(y_q) (rows in
 // ascending order, staged 256 at a time); one thread per source column
 // (consecutive threads read consecutive columns of a row).
+template <class T>
 __global__ __launch_bounds__(256) void op_transpose_kernel(
-    SolveArgs a, int K, const int64_t *opOff, const double *op,
-    double *contribS) {
+    SolveArgs a, int K, const int64_t *opOff, const T *op, double *contribS) {
   __shared__ double gv[kFMax][256];
   const int patch = xcd_remap(blockIdx.x, K);
   const int64_t s0 = a.srcOff[patch];
@@ -956,7 +1024,7 @@ __global__ __launch_bounds__(256) void op_transpose_kernel(
   const int mall = (int)(a.pairOff[patch + 1] - p0);
   const int F = a.F;
   const int tid = threadIdx.x;
-  const double *base = op + opOff[patch];
+  const T *base = op + opOff[patch];
   double acc[kFMax];
 #pragma unroll
   for (int f = 0; f < kFMax; ++f)
@@ -974,7 +1042,7 @@ __global__ __launch_bounds__(256) void op_transpose_kernel(
     const int qn = min(256, mall - q0);
     if (tid < n)
       for (int j = 0; j < qn; ++j) {
-        const double g = base[(int64_t)(q0 + j) * n + tid];
+        const double g = (double)base[(int64_t)(q0 + j) * n + tid];
 #pragma unroll
         for (int f = 0; f < kFMax; ++f)
           if (f < F)
@@ -988,13 +1056,17 @@ __global__ __launch_bounds__(256) void op_transpose_kernel(
 }
 
 void launch_op_transpose(const SolveArgs &a, int64_t K, const int64_t *opOff,
-                         const double *op, double *contribS, hipStream_t s) {
+                         const OpStore &op, double *contribS, hipStream_t s) {
   if (K <= 0)
     return;
   if (a.F < 1 || a.F > kFMax)
     throw std::runtime_error("operator transpose: 1 <= fields <= 8 per launch");
-  op_transpose_kernel<<<(unsigned)K, 256, 0, s>>>(a, (int)K, opOff, op,
-                                                  contribS);
+  if (op.f32)
+    op_transpose_kernel<<<(unsigned)K, 256, 0, s>>>(
+        a, (int)K, opOff, (const float *)op.f32, contribS);
+  else
+    op_transpose_kernel<<<(unsigned)K, 256, 0, s>>>(
+        a, (int)K, opOff, (const double *)op.f64, contribS);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 

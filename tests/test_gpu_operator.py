@@ -7,6 +7,11 @@ fields (up to 32 per pass over the operator, an FP64 MFMA contraction above
 8 fields; 40 fields take two passes),
 targets much denser than the sources
 (many operator rows per patch), graph replays and after drop_operator.
+The FP32-stored operator (FP64 sums) differs from the solve only by the
+rounding of its entries: each output moves by at most 2^-24 sum_e |op_qe|
+|v_e|, which is checked here as a relative max difference below 1e-5 (the
+operator rows of these clouds have absolute sums of order 10; the measured
+difference is printed).
 """
 
 import numpy as np
@@ -96,3 +101,31 @@ def test_operator_transpose_equals_solve(F):
     lhs = np.sum(g * plan.apply(f))
     rhs = np.sum(got * f)
     assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + abs(rhs))
+
+
+@pytest.mark.parametrize("rbf,F", [(0x11, 1), (0x12, 5), (0x11, 20)])
+def test_operator_fp32_close_to_solve(rbf, F):
+    rng = np.random.default_rng(500 + rbf + F)
+    S = rng.uniform(0, 1, (6000, 3))
+    T = rng.uniform(0, 1, (5000, 3))
+    V = np.stack([np.sin((1 + i) * S[:, 0] + 0.2 * i) + S[:, -1] ** 2
+                  for i in range(F)])
+    plan = pacman.rbf.Plan(3, rbf, S, T, 60, 0.15, -1.0)
+    ref = plan.apply(V)
+    plan.build_operator()
+    b64 = plan.operator_bytes
+    plan.build_operator(precision="fp32")
+    assert plan.operator_bytes * 2 == b64
+    got = plan.apply(V)
+    err = _rel_max(got, ref)
+    print(f"fp32 operator rbf={hex(rbf)} F={F} max rel {err:.2e}")
+    assert 0 < err < 1e-5, err
+    # transpose through the FP32 operator
+    g = rng.normal(size=(min(F, 8), len(T)))
+    plan.drop_operator()
+    tref = plan.apply_transpose(g)
+    plan.build_operator(precision="fp32")
+    terr = _rel_max(plan.apply_transpose(g), tref)
+    assert terr < 1e-5, terr
+    with pytest.raises(ValueError):
+        plan.build_operator(precision="fp16")
