@@ -711,7 +711,7 @@ __global__ __launch_bounds__(256) void op_store_kernel(
 constexpr int kOpBlock = 256;
 template <int FM, class T>
 __global__ __launch_bounds__(kOpBlock) __attribute__((amdgpu_waves_per_eu(
-    FM <= 2 ? 4 : FM <= 4 ? 3 : 2))) void op_apply_kernel(
+    FM <= 2 ? 4 : 3))) void op_apply_kernel(
     SolveArgs a, int K, const int64_t *opOff, const T *op) {
   constexpr int kCols = kMaxPatch / 16;  // loads per lane and row
   __shared__ double fv[FM][kMaxPatch];
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(kOpBlock) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
     for (int f = 0; f < FM; ++f)
       acc[f] = 0.0;
-    if constexpr (FM <= 4) {
+    {
       // every load of the row in flight at once
       T raw[kCols];
 #pragma unroll
@@ -759,21 +759,6 @@ __global__ __launch_bounds__(kOpBlock) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int f = 0; f < FM; ++f)
           acc[f] = fma(gv, fv[f][l16 + 16 * j], acc[f]);
-      }
-    } else {
-      // four loads in flight per step (fewer registers for many fields)
-      for (int j0 = 0; 16 * j0 < n; j0 += 4) {
-        T raw[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          raw[j] = row[min(l16 + 16 * (j0 + j), n - 1)];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double gv = (double)raw[j];
-#pragma unroll
-          for (int f = 0; f < FM; ++f)
-            acc[f] = fma(gv, fv[f][l16 + 16 * (j0 + j)], acc[f]);
-        }
       }
     }
 #pragma unroll
@@ -845,31 +830,17 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
     // value read there adds nothing) and rows past the patch's read row 0
     // (their outputs are not stored). The next chunk's loads are issued
     // before the current chunk's MFMAs.
-    T raw[8];
+    // Loads run two chunks ahead (ra: even chunks, rb: odd chunks).
+    auto load_chunk = [&](int c, T *r8) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s)
-      raw[s] = row[min(16 * (s >> 2) + 4 * k + (s & 3), n - 1)];
-    // the output rows of this lane (l / 16 + 4 v): target flag and slot,
-    // loaded before the MFMAs; slot -1 for rows not stored
-    int oslot[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int rr = rt * 16 + k + 4 * v;
-      const int rc = min(rr, mall - 1);
-      const int pe = a.pairTgt[p0 + rc];
-      const int sl = a.pairSlot[p0 + rc];
-      oslot[v] = (rr < mall && ((unsigned)pe >> 31) != 0) ? sl : -1;
-    }
-    for (int c = 0; c < n; c += 32) {
+      for (int s = 0; s < 8; ++s)
+        r8[s] = row[min(c + 16 * (s >> 2) + 4 * k + (s & 3), n - 1)];
+    };
+    auto mfma_chunk = [&](int c, const T *r8) {
       double g[8];
 #pragma unroll
       for (int s = 0; s < 8; ++s)
-        g[s] = (double)raw[s];
-      if (c + 32 < n) {
-#pragma unroll
-        for (int s = 0; s < 8; ++s)
-          raw[s] = row[min(c + 32 + 16 * (s >> 2) + 4 * k + (s & 3), n - 1)];
-      }
+        g[s] = (double)r8[s];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int col0 = c + 4 * k + s, col1 = col0 + 16;
@@ -882,6 +853,39 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
           acc2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(g[4 + s], b1,
                                                          acc2[t], 0, 0, 0);
         }
+      }
+    };
+    T ra[8], rb[8];
+    load_chunk(0, ra);
+    load_chunk(32, rb);
+    // the output rows of this lane (l / 16 + 4 v): target flag and slot,
+    // loaded before the MFMAs; slot -1 for rows not stored
+    int oslot[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = rt * 16 + k + 4 * v;
+      const int rc = min(rr, mall - 1);
+      const int pe = a.pairTgt[p0 + rc];
+      const int sl = a.pairSlot[p0 + rc];
+      oslot[v] = (rr < mall && ((unsigned)pe >> 31) != 0) ? sl : -1;
+    }
+    for (int c = 0; c < n; c += 64) {
+      // chunk c (c < n), then chunk c + 32 if it starts inside the patch
+      // (staged rows up to n rounded up to 32 are zero)
+      T cur[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+        cur[s] = ra[s];
+      if (c + 64 < n)
+        load_chunk(c + 64, ra);
+      mfma_chunk(c, cur);
+      if (c + 32 < n) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          cur[s] = rb[s];
+        if (c + 96 < n)
+          load_chunk(c + 96, rb);
+        mfma_chunk(c + 32, cur);
       }
     }
 #pragma unroll
@@ -955,9 +959,6 @@ static void op_apply_launch(const SolveArgs &a, int64_t K,
   else if (a.F <= 4)
     op_apply_kernel<4, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
                                                            op);
-  else if (a.F <= 8)
-    op_apply_kernel<8, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K, opOff,
-                                                           op);
   else if (a.F <= 16)
     op_apply_mfma_kernel<1, T><<<(unsigned)K, kOpBlock, 0, s>>>(a, (int)K,
                                                                 opOff, op);
@@ -979,33 +980,37 @@ void launch_op_apply(const SolveArgs &a, int64_t K, const int64_t *opOff,
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
-// The same sum with one thread per target in original order: the outputs
-// are written contiguously and each slot's fields are one contiguous read;
-// for many fields this beats scattering F separate outputs per target.
-__global__ __launch_bounds__(256) void blend_sum_orig_kernel(BlendArgs a) {
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= a.M)
-    return;
-  const int64_t t = a.tinv[o];
-  const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
-  constexpr int kB = 8;
-  for (int f0 = 0; f0 < a.F; f0 += kB) {
-    double acc[kB];
-#pragma unroll
-    for (int f = 0; f < kB; ++f)
-      acc[f] = 0.0;
-    for (int64_t s = s0; s < s1; ++s) {
-      if (!a.slotMember[s])
-        continue;
-#pragma unroll
-      for (int f = 0; f < kB; ++f)
-        if (f0 + f < a.F)
-          acc[f] += a.contrib[s * a.F + f0 + f];
+// The same sum for 2 .. 32 fields, 64 consecutive original targets per
+// workgroup: a group of L lanes (L >= F) takes one target at a time and reads
+// each of its slots' F contiguous values in one coalesced piece (lane f, field
+// f; slots in ascending order, as above), the sums are staged in LDS and
+// written out field by field as 64 consecutive outputs.
+template <int L>
+__global__ __launch_bounds__(256) void blend_sum_tile_kernel(BlendArgs a) {
+  constexpr int kT = 64;  // targets per workgroup
+  __shared__ double tile[kT][L + 1];
+  const int64_t o0 = (int64_t)blockIdx.x * kT;
+  const int tid = threadIdx.x;
+  const int f = tid % L, grp = tid / L;
+  constexpr int kGroups = 256 / L;
+  const int F = a.F;
+  for (int j = grp; j < kT; j += kGroups) {
+    const int64_t o = o0 + j;
+    double acc = 0.0;
+    if (o < a.M) {
+      const int64_t t = a.tinv[o];
+      const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
+      for (int64_t sl = s0; sl < s1; ++sl)
+        if (a.slotMember[sl] && f < F)
+          acc += a.contrib[sl * F + f];
     }
-#pragma unroll
-    for (int f = 0; f < kB; ++f)
-      if (f0 + f < a.F)
-        a.out[(int64_t)(f0 + f) * a.M + o] = acc[f];
+    tile[j][f] = acc;
+  }
+  __syncthreads();
+  for (int x = tid; x < kT * F; x += 256) {
+    const int ff = x / kT, j = x % kT;
+    if (o0 + j < a.M)
+      a.out[(int64_t)ff * a.M + o0 + j] = tile[j][ff];
   }
 }
 
@@ -1079,10 +1084,17 @@ __global__ void invert_perm_kernel(const int *perm, int64_t n, int *inv) {
 void launch_blend_sum(const BlendArgs &a, hipStream_t s) {
   if (a.M <= 0)
     return;
-  if (a.F > 8 && a.tinv)
-    blend_sum_orig_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
-  else
+  const unsigned tiles = (unsigned)((a.M + 63) / 64);
+  if (a.F >= 2 && a.F <= 32 && a.tinv) {
+    if (a.F <= 8)
+      blend_sum_tile_kernel<8><<<tiles, 256, 0, s>>>(a);
+    else if (a.F <= 16)
+      blend_sum_tile_kernel<16><<<tiles, 256, 0, s>>>(a);
+    else
+      blend_sum_tile_kernel<32><<<tiles, 256, 0, s>>>(a);
+  } else {
     blend_sum_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
+  }
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
