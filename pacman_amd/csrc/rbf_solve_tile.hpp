@@ -57,6 +57,28 @@ __device__ __forceinline__ double fast_rcp(double d) {
   return fma(r, e, r);
 }
 
+// x += (x of lane K of this 16-lane row) * nl, as one FP64 FMA with a DPP
+// row_newbcast source (the compiler forms a separate DPP move and FMA through
+// one temporary, which serialises a pivot step). The x read through DPP was
+// last written by the previous pivot step, several instructions earlier.
+template <int K>
+__device__ __forceinline__ void fmac_row_bcast(double &x, double nl) {
+  asm volatile("v_fmac_f64_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf "
+               "bank_mask:0xf"
+               : "+v"(x)
+               : "v"(nl), "i"(K));
+}
+// x of lane K of this 16-lane row; the two wait states cover a VALU write
+// of x just before.
+template <int K> __device__ __forceinline__ double row_bcast_asm(double x) {
+  double r;
+  asm volatile("s_nop 1\n\tv_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf "
+               "bank_mask:0xf"
+               : "=v"(r)
+               : "v"(x), "i"(K));
+  return r;
+}
+
 // Owners of the lower-triangle 16x16 tiles among the 7 tile waves. Tile
 // (J+1, J) and the diagonal tile (J+1, J+1) go to the same wave: that wave
 // computes W_{J+1} during panel J and applies it to the diagonal tile at
@@ -357,60 +379,56 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   if (wave == 0) {
     // ---------------- control wave: eliminations -------------------------
     // Elimination of the diagonal block in dtile (rows < kk pivot): lane i
-    // (mod 16) holds row i of the block and row i of an identity; the pivot
-    // row is broadcast along each 16-lane row (DPP row_newbcast). The
-    // identity turns into Linv. Writes Linv, d and 1/d to LDS buffer `buf`
-    // and the packed Linv of block p.
+    // (mod 16) holds row i of the block and a quarter of row i of an
+    // identity; the pivot row is broadcast along each 16-lane row (DPP
+    // row_newbcast). The identity turns into Linv. Writes Linv, d and 1/d to
+    // LDS buffer `buf` and the packed Linv of block p.
     auto factor = [&](int kk, int buf, int p) __attribute__((always_inline)) {
-      // Lane (q, i) = 16 q + i holds 8 values of row i: q = 0, 1 the
-      // columns 8q..8q+7 of the block, q = 2, 3 the columns 8(q-2).. of the
-      // identity that turns into Linv. The pivot row's values come from lane
-      // (q, k) of the same 16-lane row (DPP row_newbcast); the multiplier of
-      // row i is formed in the row holding column k and sent to the other
-      // three 16-lane rows with ds_bpermute. Lower-triangle entries of the
-      // block and zero columns of the identity are updated too (they are
-      // never read, or stay exactly zero), so every lane runs the same
-      // eight FMAs per pivot.
+      // Lane (q, i) = 16 q + i holds all 16 columns of row i of the block
+      // (the four 16-lane rows hold the same copy and update it the same
+      // way) and columns 4q .. 4q + 3 of row i of the identity. The pivot
+      // and the multiplier of row i are then formed in every 16-lane row
+      // from its own copy, with no exchange between rows: a pivot step is a
+      // chain of one broadcast, the reciprocal and the update of column
+      // k + 1. Columns at or left of the pivot are not updated (never read
+      // again).
       const int i = lane & 15, q = lane >> 4;
-      double v[8];
+      double v[16], w[4];
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
-        v[c] = q < 2 ? dtile[i][8 * q + c]
-                     : ((8 * (q - 2) + c == i) ? 1.0 : 0.0);
+      for (int c = 0; c < 16; ++c)
+        v[c] = dtile[i][c];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        w[c] = (4 * q + c == i) ? 1.0 : 0.0;
       static_for<0, 16>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
-        constexpr int qk = k / 8, ck = k % 8;
         if (k < kk) {
-          // valid in the 16-lane row qk
-          const double d = row_bcast<k>(v[ck]);
-          const double lq = (i > k && i < kk) ? v[ck] * fast_rcp(d) : 0.0;
-          const int src = (16 * qk + i) * 4;
-          const double l = __hiloint2double(
-              __builtin_amdgcn_ds_bpermute(src, __double2hiint(lq)),
-              __builtin_amdgcn_ds_bpermute(src, __double2loint(lq)));
+          const double d = row_bcast_asm<k>(v[k]);
+          const double nl = (i > k && i < kk) ? -(v[k] * fast_rcp(d)) : 0.0;
 #pragma unroll
-          for (int c = 0; c < 8; ++c)
-            v[c] = fma(-l, row_bcast<k>(v[c]), v[c]);
+          for (int c = k + 1; c < 16; ++c)
+            fmac_row_bcast<k>(v[c], nl);
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            fmac_row_bcast<k>(w[c], nl);
         }
       });
-      if (q < 2 && (i >> 3) == q) {  // the diagonal entry of row i
+      if (q == 0) {  // the diagonal entry of row i
         double dd = 0.0;
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-          dd = (c == (i & 7)) ? v[c] : dd;
+        for (int c = 0; c < 16; ++c)
+          dd = (c == i) ? v[c] : dd;
         dpan[buf][i] = dd;
         dinvp[buf][i] = i < kk ? fast_rcp(dd) : 1.0;
         if (i < kk && dd == 0.0)
           spivot = 1;
       }
-      if (q >= 2) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int col = 8 * (q - 2) + c;
-          linv[buf][i][col] = v[c];
-          if (col < i)
-            linvAll[p][i * (i - 1) / 2 + col] = v[c];
-        }
+      for (int c = 0; c < 4; ++c) {
+        const int col = 4 * q + c;
+        linv[buf][i][col] = w[c];
+        if (col < i)
+          linvAll[p][i * (i - 1) / 2 + col] = w[c];
       }
     };
 
