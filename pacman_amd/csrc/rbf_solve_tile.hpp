@@ -79,6 +79,23 @@ template <int K> __device__ __forceinline__ double row_bcast_asm(double x) {
   return r;
 }
 
+// acc += (src of lane K of this 16-lane row) * m as one FP64 FMA with a DPP
+// source; src must not have been written by the two previous instructions
+// (dpp_src_ready after writing it).
+template <int K>
+__device__ __forceinline__ void fmac_bcast_of(double &acc, double src,
+                                              double m) {
+  asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
+               "bank_mask:0xf"
+               : "+v"(acc)
+               : "v"(src), "v"(m), "i"(K));
+}
+// Two wait states after src was written (ordered after its producer by the
+// operand), before DPP reads of src.
+__device__ __forceinline__ void dpp_src_ready(double src) {
+  asm volatile("s_nop 1" : : "v"(src));
+}
+
 // Owners of the lower-triangle 16x16 tiles among the 7 tile waves. Tile
 // (J+1, J) and the diagonal tile (J+1, J+1) go to the same wave: that wave
 // computes W_{J+1} during panel J and applies it to the diagonal tile at
@@ -455,42 +472,47 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     double(*sub)[16][17] = reinterpret_cast<double(*)[16][17]>(sbuf);
     double(*sacc)[kFMax][16] = reinterpret_cast<double(*)[kFMax][16]>(
         sbuf + (R - 1) * 16 * 17);
+    double alpha1 = 0.0;  // one field: alpha_{B+1} in every lane (row lc)
     for (int B = P - 1; B >= 0; --B) {
       const bool valid = 16 * B + lc < n;
       // Each sum over the 16 rows of a block is four partial sums over the
       // rows 4m + j (in row order) added as (p0 + p1) + (p2 + p3), so a
       // field gets the same bits whatever the field count.
       if (F == 1) {
-        // one field: 16-lane row j takes the partial sum j
-        double y = 0.0;
+        // one field: every lane forms all four partial sums itself, reading
+        // alpha_{B+1} and x_B of the other rows through DPP row_newbcast
+        // (the operands are loaded from LDS first, off the chain); rows
+        // past n and entries above the diagonal carry weight 0, the
+        // diagonal of Linv weight 1
+        double sb[16], li[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          sb[r] = (B + 1 < P && 16 * (B + 1) + r < n) ? sub[B][r][lc] : 0.0;
+          li[r] = r > lc ? linvAll[B][r * (r - 1) / 2 + lc]
+                         : (r == lc ? 1.0 : 0.0);
+        }
+        const double rb =
+            valid ? rhs[0][16 * B + lc] - sacc[B][0][lc] : 0.0;
+        double yp[4] = {0.0, 0.0, 0.0, 0.0};
         if (B + 1 < P) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int r2 = lr + 4 * v;
-            if (16 * (B + 1) + r2 < n)
-              y = fma(sub[B][r2][lc], rhs[0][16 * (B + 1) + r2], y);
-          }
-          y += __shfl_xor(y, 16, kWave);
-          y += __shfl_xor(y, 32, kWave);
+          dpp_src_ready(alpha1);
+          static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            fmac_bcast_of<r>(yp[r & 3], alpha1, sb[r]);
+          });
         }
-        const double x = valid ? rhs[0][16 * B + lc] - sacc[B][0][lc] - y : 0.0;
-        if (lr == 0)
-          xs[0][lc] = x;
-        __builtin_amdgcn_wave_barrier();
-        double part = 0.0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = lr + 4 * v;
-          if (r >= lc) {
-            const double xr = xs[0][r];
-            part = (r == lc) ? part + xr
-                             : fma(linvAll[B][r * (r - 1) / 2 + lc], xr, part);
-          }
-        }
-        part += __shfl_xor(part, 16, kWave);
-        part += __shfl_xor(part, 32, kWave);
+        const double y = (yp[0] + yp[1]) + (yp[2] + yp[3]);
+        const double x = valid ? rb - y : 0.0;
+        double pp[4] = {0.0, 0.0, 0.0, 0.0};
+        dpp_src_ready(x);
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+          constexpr int r = decltype(rc)::value;
+          fmac_bcast_of<r>(pp[r & 3], x, li[r]);
+        });
+        const double part = (pp[0] + pp[1]) + (pp[2] + pp[3]);
         if (lr == 0 && valid)
           rhs[0][16 * B + lc] = part;
+        alpha1 = part;
         __builtin_amdgcn_wave_barrier();
       }
       // several fields: lane (lr, lc) takes field f0 + lr, row lc of the
