@@ -287,6 +287,17 @@ def bench_apply(args, S, T, npc, support) -> int:
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     M = len(T)
+    op_rate = None
+    if use_op:
+        # whole-step rates of the stored-operator contraction: every entry is
+        # read once and used in 2 F flops (FP64 MFMA above 4 fields)
+        ob = plan.operator_bytes
+        entries = ob / (4 if prec == "fp32" else 8)
+        op_rate = {"operator_read_tb_s": ob / (ms / 1e3) / 1e12,
+                   "operator_read_frac_of_8tb_s": ob / (ms / 1e3) / 8e12,
+                   "contraction_tflop_s": 2 * entries * F / (ms / 1e3) / 1e12,
+                   "contraction_frac_of_fp64_peak":
+                       2 * entries * F / (ms / 1e3) / 78.6e12}
     res = {
         "metric": "RBF-PUM repeated apply Mpts*fields/s (FP64)",
         "value": M * F / (ms / 1e3) / 1e6, "unit": "Mpts*fields/s",
@@ -301,7 +312,8 @@ def bench_apply(args, S, T, npc, support) -> int:
                    "operator_build_ms": build_ms,
                    "operator_precision": prec if use_op else None,
                    "operator_fp32_max_rel_vs_fp64": fp32_err,
-                   "operator_gb": plan.operator_bytes / 1e9},
+                   "operator_gb": plan.operator_bytes / 1e9,
+                   "operator_rates_per_step": op_rate},
         "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)

@@ -193,7 +193,6 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   __shared__ double linvAll[R][120];         // strictly-lower Linv, packed
   __shared__ double dtile[16][17];           // diagonal tile hand-over
   __shared__ double dpan[2][16], dinvp[2][16];
-  __shared__ double xs[4][16];
   __shared__ double scoef[kFMax][4];
   __shared__ int spivot;
   double(*wp)[16][LDW] = reinterpret_cast<double(*)[16][LDW]>(sbuf);
@@ -472,81 +471,56 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
     double(*sub)[16][17] = reinterpret_cast<double(*)[16][17]>(sbuf);
     double(*sacc)[kFMax][16] = reinterpret_cast<double(*)[kFMax][16]>(
         sbuf + (R - 1) * 16 * 17);
-    double alpha1 = 0.0;  // one field: alpha_{B+1} in every lane (row lc)
+    // alpha_{B+1} of field 4 g + lr at row lc, per group g of four fields
+    double alpha1[kFMax / 4] = {};
     for (int B = P - 1; B >= 0; --B) {
       const bool valid = 16 * B + lc < n;
-      // Each sum over the 16 rows of a block is four partial sums over the
-      // rows 4m + j (in row order) added as (p0 + p1) + (p2 + p3), so a
-      // field gets the same bits whatever the field count.
-      if (F == 1) {
-        // one field: every lane forms all four partial sums itself, reading
-        // alpha_{B+1} and x_B of the other rows through DPP row_newbcast
-        // (the operands are loaded from LDS first, off the chain); rows
-        // past n and entries above the diagonal carry weight 0, the
-        // diagonal of Linv weight 1
-        double sb[16], li[16];
+      // Lane (lr, lc) takes field 4 g + lr, row lc of the block, for each
+      // group g of four fields. Every lane forms all four partial sums of a
+      // block sum itself (rows 4m + j, in row order, added as
+      // (p0 + p1) + (p2 + p3), so a field gets the same bits whatever the
+      // field count), reading alpha_{B+1} and x_B of the other rows of its
+      // field through DPP row_newbcast. The operands are loaded from LDS
+      // first, off the chain; rows past n and entries above the diagonal
+      // carry weight 0, the diagonal of Linv weight 1.
+      double sb[16], li[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          sb[r] = (B + 1 < P && 16 * (B + 1) + r < n) ? sub[B][r][lc] : 0.0;
-          li[r] = r > lc ? linvAll[B][r * (r - 1) / 2 + lc]
-                         : (r == lc ? 1.0 : 0.0);
-        }
-        const double rb =
-            valid ? rhs[0][16 * B + lc] - sacc[B][0][lc] : 0.0;
-        double yp[4] = {0.0, 0.0, 0.0, 0.0};
-        if (B + 1 < P) {
-          dpp_src_ready(alpha1);
+      for (int r = 0; r < 16; ++r) {
+        sb[r] = (B + 1 < P && 16 * (B + 1) + r < n) ? sub[B][r][lc] : 0.0;
+        li[r] = r > lc ? linvAll[B][r * (r - 1) / 2 + lc]
+                       : (r == lc ? 1.0 : 0.0);
+      }
+      static_for<0, kFMax / 4>([&](auto gc) __attribute__((always_inline)) {
+        constexpr int g = decltype(gc)::value;
+        if (4 * g < F) {
+          const int f = 4 * g + lr;
+          const bool fv = f < F;
+          const double rb = (valid && fv)
+                                ? rhs[f][16 * B + lc] - sacc[B][f][lc]
+                                : 0.0;
+          double yp[4] = {0.0, 0.0, 0.0, 0.0};
+          if (B + 1 < P) {
+            dpp_src_ready(alpha1[g]);
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+              constexpr int r = decltype(rc)::value;
+              fmac_bcast_of<r>(yp[r & 3], alpha1[g], sb[r]);
+            });
+          }
+          const double y = (yp[0] + yp[1]) + (yp[2] + yp[3]);
+          const double x = (valid && fv) ? rb - y : 0.0;
+          double pp[4] = {0.0, 0.0, 0.0, 0.0};
+          dpp_src_ready(x);
           static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
-            fmac_bcast_of<r>(yp[r & 3], alpha1, sb[r]);
+            fmac_bcast_of<r>(pp[r & 3], x, li[r]);
           });
+          const double part = (pp[0] + pp[1]) + (pp[2] + pp[3]);
+          if (valid && fv)
+            rhs[f][16 * B + lc] = part;
+          alpha1[g] = part;
         }
-        const double y = (yp[0] + yp[1]) + (yp[2] + yp[3]);
-        const double x = valid ? rb - y : 0.0;
-        double pp[4] = {0.0, 0.0, 0.0, 0.0};
-        dpp_src_ready(x);
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-          constexpr int r = decltype(rc)::value;
-          fmac_bcast_of<r>(pp[r & 3], x, li[r]);
-        });
-        const double part = (pp[0] + pp[1]) + (pp[2] + pp[3]);
-        if (lr == 0 && valid)
-          rhs[0][16 * B + lc] = part;
-        alpha1 = part;
-        __builtin_amdgcn_wave_barrier();
-      }
-      // several fields: lane (lr, lc) takes field f0 + lr, row lc of the
-      // block, four fields at a time, with all four partial sums
-      for (int f0 = 0; F > 1 && f0 < F; f0 += 4) {
-        const int f = f0 + lr;
-        const bool fv = f < F;
-        double y[4] = {0.0, 0.0, 0.0, 0.0};
-        if (B + 1 < P && fv) {
-#pragma unroll
-          for (int r2 = 0; r2 < 16; ++r2)
-            if (16 * (B + 1) + r2 < n)
-              y[r2 & 3] =
-                  fma(sub[B][r2][lc], rhs[f][16 * (B + 1) + r2], y[r2 & 3]);
-        }
-        const double ys = (y[0] + y[1]) + (y[2] + y[3]);
-        const double x =
-            (valid && fv) ? rhs[f][16 * B + lc] - sacc[B][f][lc] - ys : 0.0;
-        xs[lr][lc] = x;
-        __builtin_amdgcn_wave_barrier();
-        double pt[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          if (r >= lc) {
-            const double xr = xs[lr][r];
-            pt[r & 3] = (r == lc) ? pt[r & 3] + xr
-                                  : fma(linvAll[B][r * (r - 1) / 2 + lc], xr,
-                                        pt[r & 3]);
-          }
-        }
-        if (valid && fv)
-          rhs[f][16 * B + lc] = (pt[0] + pt[1]) + (pt[2] + pt[3]);
-        __builtin_amdgcn_wave_barrier();
-      }
+      });
+      __builtin_amdgcn_wave_barrier();
       __syncthreads();  // alpha_B ready
     }
   } else {
@@ -687,23 +661,29 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
       (&sacc[0][0][0])[t] = 0.0;
     __syncthreads();  // F
     for (int B = P - 1; B >= 0; --B) {
-      // alpha_{B+1} is known: add L_{B+1,J}^T alpha_{B+1} to S_J, J < B
+      // alpha_{B+1} is known: add L_{B+1,J}^T alpha_{B+1} to S_J, J < B, as
+      // one 16 x 16 x 16 FP64 MFMA product for all fields at once: the
+      // tile's registers are already the A operand of L^T (lane (lc, lr),
+      // step v: L[4 v + lr][lc]); B = alpha (fields past F zero); D row
+      // lr + 4 v, column lc = field.
       static_for<0, TPW>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         const int I = tI[s], J = tJ[s];
         if (I == B + 1 && J < B && I < P) {
-          for (int f = 0; f < F; ++f) {
-            double part = 0.0;
+          dbl4 d4 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int r = 16 * I + lr + 4 * v;
-              if (r < n)
-                part = fma(acc[s][v], rhs[f][r], part);
-            }
-            part += __shfl_xor(part, 16, kWave);
-            part += __shfl_xor(part, 32, kWave);
-            if (lr == 0)
-              sacc[J][f][lc] += part;
+          for (int v = 0; v < 4; ++v) {
+            const int r = 16 * I + 4 * v + lr;
+            const double bv = (lc < F && r < n) ? rhs[lc < F ? lc : 0][r]
+                                                : 0.0;
+            // rows past n (bordered right-hand sides) masked on both sides
+            const double av = r < n ? acc[s][v] : 0.0;
+            d4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d4, 0, 0, 0);
+          }
+          if (lc < F) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              sacc[J][lc][lr + 4 * v] += d4[v];
           }
         }
       });
