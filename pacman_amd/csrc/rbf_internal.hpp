@@ -225,6 +225,7 @@ struct BlendArgs {
   const double *tx, *ty, *tz;
   const int *tperm;         // cell order -> original target index
   const int *tinv;          // original target index -> cell order
+  const double *wslot;      // Shepard weight per slot (0: not a member)
   const double *cx, *cy, *cz;
   int64_t M;
   int F;

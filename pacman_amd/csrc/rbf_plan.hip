@@ -274,6 +274,23 @@ void RbfPumPlan::prepare(int F, hipStream_t s) {
     coef_.alloc((size_t)std::max<int64_t>(K_, 1) * Fs * 4);
   for (int f0 = 0; f0 < F; f0 += perSolve)
     bins_for(std::min(perSolve, F - f0), s);
+  ensure_tinv(F, s);
+}
+
+// Original target index -> cell order and the slot weights, for the blends
+// that write several fields in original order (built on first use by such an
+// apply).
+void RbfPumPlan::ensure_tinv(int F, hipStream_t s) {
+  if (F < 2)
+    return;
+  if (tinv_.n < (size_t)std::max<int64_t>(M_, 1)) {
+    tinv_.alloc(std::max<int64_t>(M_, 1));
+    launch_invert_perm(gt_.view.perm, M_, tinv_.get(), s);
+  }
+  if (wslot_.n < (size_t)std::max<int64_t>(pairs_.P, 1)) {
+    wslot_.alloc(std::max<int64_t>(pairs_.P, 1));
+    launch_slot_weights(blendArgs(1), wslot_.get(), s);
+  }
 }
 
 void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
@@ -378,6 +395,7 @@ BlendArgs RbfPumPlan::blendArgs(int F) const {
   bl.tz = gt_.view.z;
   bl.tperm = gt_.view.perm;
   bl.tinv = tinv_.get();
+  bl.wslot = wslot_.n ? wslot_.get() : nullptr;
   bl.cx = cx_.get();
   bl.cy = cy_.get();
   bl.cz = cz_.get();
@@ -516,8 +534,7 @@ void RbfPumPlan::buildOperator(hipStream_t s, bool fp32) {
     op32_.alloc(std::max<int64_t>(ooff[K_], 1));
   else
     op_.alloc(std::max<int64_t>(ooff[K_], 1));
-  tinv_.alloc(std::max<int64_t>(M_, 1));
-  launch_invert_perm(gt_.view.perm, M_, tinv_.get(), s);
+  ensure_tinv(2, s);
   const int Fb = solve_fields_per_launch();
   const int64_t total = std::max<int64_t>(totalSrc_, 1);
   if (wslot_.n < (size_t)std::max<int64_t>(pairs_.P, 1)) {

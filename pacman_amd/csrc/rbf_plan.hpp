@@ -132,6 +132,7 @@ private:
   DevBuf<double> op_;
   DevBuf<float> op32_;  // set instead of op_ for an FP32 operator
   OpStore opStore() const;
+  void ensure_tinv(int F, hipStream_t s);
   DevBuf<int> tinv_;  // original target index -> cell order (operator apply)
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};

@@ -410,12 +410,47 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
 #pragma unroll
       for (int f = 0; f < kB; ++f)
         if (f0 + f < a.F)
-          acc[f] += w * a.contrib[s * a.F + f0 + f];
+          acc[f] = fma(w, a.contrib[s * a.F + f0 + f], acc[f]);
     }
 #pragma unroll
     for (int f = 0; f < kB; ++f)
       if (f0 + f < a.F)
         a.out[(int64_t)(f0 + f) * a.M + o] = acc[f];
+  }
+}
+
+// The same blend for 2 .. 8 fields with the plan's stored slot weights
+// (slot_weights_kernel: the same Shepard sums and quotients as blend_kernel,
+// 0 for non-member slots), 64 consecutive original targets per workgroup: a
+// group of 8 lanes takes one target at a time (lane f, field f), reads each
+// member slot's fields as one contiguous piece with no dependent loads, and
+// the results are staged in LDS and written field by field as 64 consecutive
+// outputs instead of F scattered writes per target.
+__global__ __launch_bounds__(256) void blend_tile_kernel(BlendArgs a) {
+  constexpr int kT = 64, kL = 8, kGroups = 256 / kL;
+  __shared__ double tile[kT][kL + 1];
+  const int64_t o0 = (int64_t)blockIdx.x * kT;
+  const int tid = threadIdx.x, f = tid % kL, grp = tid / kL;
+  const int F = a.F;
+  for (int j = grp; j < kT; j += kGroups) {
+    const int64_t o = o0 + j;
+    double acc = 0.0;
+    if (o < a.M) {
+      const int64_t t = a.tinv[o];
+      const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
+      for (int64_t s = s0; s < s1; ++s) {
+        const double w = a.wslot[s];
+        if (w != 0.0 && f < F)  // member slots only (their weight is > 0)
+          acc = fma(w, a.contrib[s * F + f], acc);
+      }
+    }
+    tile[j][f] = acc;
+  }
+  __syncthreads();
+  for (int x = tid; x < kT * F; x += 256) {
+    const int ff = x / kT, j = x % kT;
+    if (o0 + j < a.M)
+      a.out[(int64_t)ff * a.M + o0 + j] = tile[j][ff];
   }
 }
 
@@ -1151,7 +1186,10 @@ void launch_source_reduce(const int64_t *srcTOff, const int *srcTEnt,
 void launch_blend(const BlendArgs &a, hipStream_t s) {
   if (a.M <= 0)
     return;
-  blend_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
+  if (a.F >= 2 && a.F <= 8 && a.tinv && a.wslot)
+    blend_tile_kernel<<<(unsigned)((a.M + 63) / 64), 256, 0, s>>>(a);
+  else
+    blend_kernel<<<(unsigned)((a.M + 255) / 256), 256, 0, s>>>(a);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
