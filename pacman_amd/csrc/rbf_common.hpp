@@ -30,11 +30,15 @@ __host__ __device__ inline double sqdiff3(double lx, double ly, double lz,
 // Wendland kernels of src/rbf_pum/solver/rbf_functions.hpp:22-150.
 // KIND 0..4 = C0, C2, C4, C6, C8; p = r * radiusInv, zero outside [0, 1).
 // NONNEG: the caller guarantees r >= 0 and rinv > 0 (a distance), so the
-// mask reduces to p < 1 with the same result.
+// mask reduces to p < 1 with the same result; it is applied as q = max(1 - p,
+// 0), which leaves q unchanged inside the support and makes every kernel
+// (a power of q times a polynomial positive on [0, 1]) +0 outside.
 template <int KIND, typename T, bool NONNEG = false>
 __host__ __device__ inline T wendland(T r, T rinv) {
   const T p = r * rinv;
-  const T q = T(1) - p;
+  T q = T(1) - p;
+  if constexpr (NONNEG)
+    q = fmax(q, T(0));
   const T q2 = q * q;
   T v;
   if constexpr (KIND == 0) {
@@ -55,7 +59,7 @@ __host__ __device__ inline T wendland(T r, T rinv) {
                (T(150) * p + T(15)));
   }
   if constexpr (NONNEG)
-    return p < T(1) ? v : T(0);
+    return v;
   else
     return (r >= T(0) && p < T(1) && p >= T(0)) ? v : T(0);
 }

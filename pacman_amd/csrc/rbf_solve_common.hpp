@@ -25,17 +25,18 @@ __device__ __forceinline__ void static_for(Fn &&f) {
 
 // sqrt for the dense arithmetic: hardware reciprocal square root refined by
 // two Newton-Raphson (Goldschmidt) steps; within an ulp for the distances
-// used here (0 <= x < 1e300). Membership decisions never use it.
+// used here (0 <= x < 1e300). Membership decisions never use it. x = 0 takes
+// the estimate of 1e-300 (a finite reciprocal root), so the refinement
+// returns exactly 0 there with no select.
 __device__ inline double fast_sqrt(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
+  const double y = __builtin_amdgcn_rsq(fmax(x, 1e-300));
   double s = x * y;
   double h = 0.5 * y;
   const double r = fma(-s, h, 0.5);
   s = fma(s, r, s);
   h = fma(h, r, h);
   const double d = fma(-s, s, x);
-  s = fma(d, h, s);
-  return x > 0.0 ? s : 0.0;
+  return fma(d, h, s);
 }
 
 // Value of v in lane `lane` of the wave (all lanes active; lane a
