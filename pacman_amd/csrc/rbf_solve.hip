@@ -757,6 +757,8 @@ __global__ __launch_bounds__(kOpBlock) __attribute__((amdgpu_waves_per_eu(
   const int mall = (int)(a.pairOff[patch + 1] - p0);
   const int F = a.F;
   const int tid = threadIdx.x;
+  if (n == 0 || mall == 0)  // the whole block: nothing to read or write
+    return;
   // fields F .. FM - 1 and columns past n are staged as zeros
   for (int e = tid; e < kMaxPatch; e += kOpBlock) {
     const int p = e < n ? a.srcIdx[s0 + e] : 0;
@@ -836,6 +838,8 @@ __global__ __launch_bounds__(kOpBlock) void op_apply_mfma_kernel(
   const int mall = (int)(a.pairOff[patch + 1] - p0);
   const int F = a.F;
   const int tid = threadIdx.x;
+  if (n == 0 || mall == 0)  // the whole block: nothing to read or write
+    return;
   // rows n .. (n rounded up to 32) - 1 are zero: the B operand needs no
   // column mask
   const int nr = (n + 31) & ~31;
