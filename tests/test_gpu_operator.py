@@ -29,8 +29,9 @@ def _rel_max(a, b):
 
 @pytest.mark.parametrize("rbf,dim,F", [(0x11, 3, 1), (0x10, 3, 3),
                                        (0x14, 3, 1), (0x13, 2, 2),
-                                       (0x12, 3, 10), (0x11, 3, 20),
-                                       (0x11, 3, 40)])
+                                       (0x11, 3, 2), (0x11, 3, 5),
+                                       (0x11, 3, 8), (0x12, 3, 10),
+                                       (0x11, 3, 20), (0x11, 3, 40)])
 def test_operator_apply_equals_solve(rbf, dim, F):
     rng = np.random.default_rng(200 + rbf + dim + F)
     S = rng.uniform(0, 1, (6000, dim))
