@@ -438,10 +438,13 @@ __global__ __launch_bounds__(256) void blend_tile_kernel(BlendArgs a) {
     if (o < a.M) {
       const int64_t t = a.tinv[o];
       const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
+      // weight and value loaded unconditionally (no load waits on another);
+      // member slots only (their weight is > 0)
+#pragma unroll 4
       for (int64_t s = s0; s < s1; ++s) {
         const double w = a.wslot[s];
-        if (w != 0.0 && f < F)  // member slots only (their weight is > 0)
-          acc = fma(w, a.contrib[s * F + f], acc);
+        const double c = a.contrib[s * F + (f < F ? f : 0)];
+        acc = (w != 0.0 && f < F) ? fma(w, c, acc) : acc;
       }
     }
     tile[j][f] = acc;
@@ -1039,9 +1042,13 @@ __global__ __launch_bounds__(256) void blend_sum_tile_kernel(BlendArgs a) {
     if (o < a.M) {
       const int64_t t = a.tinv[o];
       const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
-      for (int64_t sl = s0; sl < s1; ++sl)
-        if (a.slotMember[sl] && f < F)
-          acc += a.contrib[sl * F + f];
+      // flag and value loaded unconditionally (no load waits on another)
+#pragma unroll 4
+      for (int64_t sl = s0; sl < s1; ++sl) {
+        const bool mem = a.slotMember[sl] != 0;
+        const double c = a.contrib[sl * F + (f < F ? f : 0)];
+        acc = (mem && f < F) ? acc + c : acc;
+      }
     }
     tile[j][f] = acc;
   }
