@@ -387,10 +387,24 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
     return;
   const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
   const double yx = a.tx[t], yy = a.ty[t], yz = a.tz[t];
+  // the first kSlots kernel values are kept for the weighted sum (static
+  // indices: registers); slots past them are evaluated again
+  constexpr int kSlots = 8;
+  double ph[kSlots];
   double sum = 0.0;
-  for (int64_t s = s0; s < s1; ++s) {
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    ph[j] = 0.0;
+    if (s0 + j < s1) {
+      const int k = a.slotPatch[s0 + j];
+      // DistanceNoCheck(target, center), solver.hpp:158-161
+      const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
+      ph[j] = wendland<1, double, true>(d, a.rinv);
+      sum += ph[j];
+    }
+  }
+  for (int64_t s = s0 + kSlots; s < s1; ++s) {
     const int k = a.slotPatch[s];
-    // DistanceNoCheck(target, center), solver.hpp:158-161
     const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
     sum += wendland<1, double, true>(d, a.rinv);
   }
@@ -401,7 +415,17 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
 #pragma unroll
     for (int f = 0; f < kB; ++f)
       acc[f] = 0.0;
-    for (int64_t s = s0; s < s1; ++s) {
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      if (s0 + j < s1 && a.slotMember[s0 + j]) {
+        const double w = ph[j] / sum;
+#pragma unroll
+        for (int f = 0; f < kB; ++f)
+          if (f0 + f < a.F)
+            acc[f] = fma(w, a.contrib[(s0 + j) * a.F + f0 + f], acc[f]);
+      }
+    }
+    for (int64_t s = s0 + kSlots; s < s1; ++s) {
       if (!a.slotMember[s])
         continue;
       const int k = a.slotPatch[s];
