@@ -598,13 +598,30 @@ __global__ __launch_bounds__(256) void slot_weights_kernel(BlendArgs a,
     return;
   const int64_t s0 = a.toff[t], s1 = a.toff[t + 1];
   const double yx = a.tx[t], yy = a.ty[t], yz = a.tz[t];
+  // as blend_kernel: the first kSlots kernel values stay in registers
+  constexpr int kSlots = 8;
+  double ph[kSlots];
   double sum = 0.0;
-  for (int64_t s = s0; s < s1; ++s) {
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    ph[j] = 0.0;
+    if (s0 + j < s1) {
+      const int k = a.slotPatch[s0 + j];
+      const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
+      ph[j] = wendland<1, double, true>(d, a.rinv);
+      sum += ph[j];
+    }
+  }
+  for (int64_t s = s0 + kSlots; s < s1; ++s) {
     const int k = a.slotPatch[s];
     const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
     sum += wendland<1, double, true>(d, a.rinv);
   }
-  for (int64_t s = s0; s < s1; ++s) {
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j)
+    if (s0 + j < s1)
+      wslot[s0 + j] = a.slotMember[s0 + j] ? ph[j] / sum : 0.0;
+  for (int64_t s = s0 + kSlots; s < s1; ++s) {
     const int k = a.slotPatch[s];
     const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
     wslot[s] = a.slotMember[s] ? wendland<1, double, true>(d, a.rinv) / sum
