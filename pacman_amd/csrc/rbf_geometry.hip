@@ -240,17 +240,40 @@ __global__ void grid_gather_kernel(const double *pts, int64_t n, int dim,
 template <class F>
 __device__ inline void wave_box_scan(const GridView &g, const BoxRange &b,
                                      F &&f) {
+  // Consecutive column runs share one pass of the wave while they fit in its
+  // 64 lanes (lanes in column order, points in order within a column), so a
+  // box of short runs takes a few passes instead of one per column; a run
+  // longer than the wave takes passes of its own, as before.
   const int lane = threadIdx.x % kWave;
-  for (int ix = b.x0; ix <= b.x1; ++ix)
-    for (int iy = b.y0; iy <= b.y1; ++iy) {
-      const int base = (ix * g.dims[1] + iy) * g.dims[2];
-      const int s = g.cell_start[base + b.z0];
-      const int e = g.cell_start[base + b.z1 + 1];
-      for (int p0 = s; p0 < e; p0 += kWave) {
-        const int p = p0 + lane;
-        f(p, p < e);
-      }
+  const int ny = b.y1 - b.y0 + 1;
+  const int ncol = (b.x1 - b.x0 + 1) * ny;
+  int col = 0, cs = 0, ce = 0;  // run of column `col` (uniform)
+  auto load_run = [&](int c) {
+    const int ix = b.x0 + c / ny, iy = b.y0 + c % ny;
+    const int base = (ix * g.dims[1] + iy) * g.dims[2];
+    cs = g.cell_start[base + b.z0];
+    ce = g.cell_start[base + b.z1 + 1];
+  };
+  if (ncol > 0)
+    load_run(0);
+  while (col < ncol) {
+    if (ce - cs > kWave) {  // a long run: whole-wave passes, then the rest
+      for (; ce - cs > kWave; cs += kWave)
+        f(cs + lane, true);
+      continue;
     }
+    // pack this run and the following ones while they fit
+    int used = 0, p = -1;
+    while (col < ncol && used + (ce - cs) <= kWave) {
+      const int len = ce - cs;
+      if (lane >= used && lane < used + len)
+        p = cs + (lane - used);
+      used += len;
+      if (++col < ncol)
+        load_run(col);
+    }
+    f(p < 0 ? 0 : p, p >= 0);
+  }
 }
 
 // Lexicographic (d2, original index) minimum across the wave.
