@@ -726,40 +726,47 @@ __global__ void pair_count_kernel(const int *pairTgt, int64_t P, int *tcnt) {
     atomicAdd(&tcnt[pairTgt[p] & 0x7fffffff], 1);
 }
 
-__global__ void pair_scatter_kernel(const int *pairTgt, int64_t P,
-                                    const int64_t *toff, int *tfill,
-                                    int *tlist) {
+// Lists every pair under its target: the pair id with the member flag in the
+// top bit, and its patch beside it (read here in pair order, coalesced).
+__global__ void pair_scatter_kernel(const int *pairTgt, const int *pairPatch,
+                                    int64_t P, const int64_t *toff, int *tfill,
+                                    int *tlist, int *tpatch) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P)
     return;
-  const int t = pairTgt[p] & 0x7fffffff;
-  tlist[toff[t] + atomicAdd(&tfill[t], 1)] = (int)p;
+  const int pe = pairTgt[p];
+  const int t = pe & 0x7fffffff;
+  const int64_t at = toff[t] + atomicAdd(&tfill[t], 1);
+  tlist[at] = (int)((unsigned)p | ((unsigned)pe & 0x80000000u));
+  tpatch[at] = pairPatch[p];
 }
 
-// Per target: order its pairs by patch (pair ids grow with the patch id) and
-// record each pair's slot.
+// Per target: order its pairs by pair id (pair ids grow with the patch id, so
+// this is patch order) and record each pair's slot, patch and member flag.
 __global__ void target_slots_kernel(const int64_t *toff, int64_t M, int *tlist,
-                                    const int *pairTgt, const int *pairPatch,
-                                    int *pairSlot, int *slotPatch,
+                                    int *tpatch, int *pairSlot, int *slotPatch,
                                     unsigned char *slotMember) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M)
     return;
   const int64_t s = toff[t], e = toff[t + 1];
   for (int64_t i = s + 1; i < e; ++i) {
-    const int v = tlist[i];
+    const int v = tlist[i], vp = tpatch[i];
+    const int key = v & 0x7fffffff;
     int64_t j = i - 1;
-    while (j >= s && tlist[j] > v) {
+    while (j >= s && (tlist[j] & 0x7fffffff) > key) {
       tlist[j + 1] = tlist[j];
+      tpatch[j + 1] = tpatch[j];
       --j;
     }
     tlist[j + 1] = v;
+    tpatch[j + 1] = vp;
   }
   for (int64_t i = s; i < e; ++i) {
-    const int p = tlist[i];
-    pairSlot[p] = (int)i;
-    slotPatch[i] = pairPatch[p];
-    slotMember[i] = (unsigned char)((unsigned)pairTgt[p] >> 31);
+    const int v = tlist[i];
+    pairSlot[v & 0x7fffffff] = (int)i;
+    slotPatch[i] = tpatch[i];
+    slotMember[i] = (unsigned char)((unsigned)v >> 31);
   }
 }
 
@@ -1013,7 +1020,7 @@ void patch_targets(const GridView &tgt, const double *cx, const double *cy,
   ps.slot.alloc(P1);
   ps.slotPatch.alloc(P1);
   ps.slotMember.alloc(P1);
-  DevBuf<int> pairPatch(P1), tlist(P1);
+  DevBuf<int> pairPatch(P1), tlist(P1), tpatch(P1);
   if (K > 0) {
     member_compact_kernel<<<wb, kBlock, 0, s>>>(
         tgt, cx, cy, cz, K, r, r - kEps, true, kMemberCap, scratch.get(),
@@ -1032,13 +1039,14 @@ void patch_targets(const GridView &tgt, const double *cx, const double *cy,
   exclusive_scan_i32_to_i64(tcnt.get(), ps.toff.get(), M, s);
   if (P > 0) {
     pair_scatter_kernel<<<blocks_for(P, kBlock), kBlock, 0, s>>>(
-        ps.tgt.get(), P, ps.toff.get(), tfill.get(), tlist.get());
+        ps.tgt.get(), pairPatch.get(), P, ps.toff.get(), tfill.get(),
+        tlist.get(), tpatch.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   if (M > 0) {
     target_slots_kernel<<<blocks_for(M, kBlock), kBlock, 0, s>>>(
-        ps.toff.get(), M, tlist.get(), ps.tgt.get(), pairPatch.get(),
-        ps.slot.get(), ps.slotPatch.get(), ps.slotMember.get());
+        ps.toff.get(), M, tlist.get(), tpatch.get(), ps.slot.get(),
+        ps.slotPatch.get(), ps.slotMember.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
