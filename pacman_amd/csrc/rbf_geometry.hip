@@ -158,16 +158,16 @@ void scan_device(const Tin *in, Tout *out, int64_t n, hipStream_t s,
     scan_add_kernel<Tout><<<(unsigned)nb, kBlock, 0, s>>>(out, n, offs.get());
     set_last_kernel<Tout><<<1, 1, 0, s>>>(out, n, offs.get() + nb);
     PACMAN_HIP_CHECK(hipGetLastError());
-    // offs must outlive the kernels that read it
-    PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   }
   PACMAN_HIP_CHECK(hipGetLastError());
+  // Temporaries (offs, sums) may be released while the kernels still run:
+  // the device cache reuses a released block only after a device
+  // synchronisation, and without the cache hipFree synchronises.
   if (fetchTotal) {
     PACMAN_HIP_CHECK(hipMemcpyAsync(totalHost, out + n, sizeof(Tout),
                                     hipMemcpyDeviceToHost, s));
+    PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   }
-  // sums must outlive the kernels that read it
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------
@@ -888,7 +888,8 @@ void build_grid(const double *pts, int64_t n, int dim, const double *lo,
         pts, n, dim, g.perm.get(), g.x.get(), g.y.get(), g.z.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  // key, cnt and fill are released here while the kernels may still run
+  // (see scan_device)
 }
 
 void probe_radius(const GridView &src, const double *probes, int nprobe,
