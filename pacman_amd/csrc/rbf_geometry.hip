@@ -963,7 +963,7 @@ int64_t make_centers(const Lattice &Lh, const GridView &src,
       nb, f3.get(), pos.get(), cx.get(), cy.get(), cz.get(), ocx.get(),
       ocy.get(), ocz.get(), ocand.get());
   PACMAN_HIP_CHECK(hipGetLastError());
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  // temporaries are released while the kernels may still run (scan_device)
   return K;
 }
 
@@ -994,7 +994,7 @@ void patch_sources(const GridView &src, const double *cx, const double *cy,
         off.get(), idx.get(), nullptr);
     PACMAN_HIP_CHECK(hipGetLastError());
   }
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  // temporaries are released while the kernels may still run (scan_device)
 }
 
 void patch_targets(const GridView &tgt, const double *cx, const double *cy,
@@ -1050,7 +1050,7 @@ void patch_targets(const GridView &tgt, const double *cx, const double *cy,
         ps.slotPatch.get(), ps.slotMember.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  // temporaries are released while the kernels may still run (scan_device)
 }
 
 } // namespace pacman_amd
