@@ -61,8 +61,22 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the HIP build needs ROCm")
 
 
-def _compile(src: str, extra: list[str]) -> str:
+def _stale(src: str, obj: str) -> bool:
+    """True when obj is older than src or any header of csrc/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [
+        os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps += [os.path.join(HERE, "..", "include", h)
+             for h in os.listdir(os.path.join(HERE, "..", "include"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src: str, extra: list[str], incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [hipcc(), *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -70,14 +84,16 @@ def _compile(src: str, extra: list[str]) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every unit (incremental=True: only units older than their
+    source or a header) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     workers = max(1, min(len(UNITS), os.cpu_count() or 1))
     # The solve units take longest: start them first.
     order = sorted(UNITS.items(), key=lambda kv: not kv[0].startswith(
         "rbf_solve_tile"))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), order))
+        objs = list(ex.map(lambda kv: _compile(*kv, incremental), order))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o",
            tmp]
@@ -91,5 +107,5 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

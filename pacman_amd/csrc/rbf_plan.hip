@@ -262,16 +262,13 @@ void RbfPumPlan::prepare(int F, hipStream_t s) {
   if (F < 1)
     throw std::invalid_argument("apply needs at least one field");
   const int64_t P = std::max<int64_t>(pairs_.P, 1);
-  if (valsSorted_.n < (size_t)(F * N_))
-    valsSorted_.alloc((size_t)F * N_);
+  grow(valsSorted_, (size_t)F * N_);
   const int perSolve = solve_fields_per_launch();
   // fields per pass: up to kOpFMax over a stored operator, else per solve
   const int Fc = std::min(F, opBuilt_ ? kOpFMax : perSolve);
-  if (contrib_.n < (size_t)(P * Fc))
-    contrib_.alloc((size_t)P * Fc);
+  grow(contrib_, (size_t)P * Fc);
   const int Fs = std::min(F, perSolve);
-  if (coef_.n < (size_t)(std::max<int64_t>(K_, 1) * Fs * 4))
-    coef_.alloc((size_t)std::max<int64_t>(K_, 1) * Fs * 4);
+  grow(coef_, (size_t)std::max<int64_t>(K_, 1) * Fs * 4);
   for (int f0 = 0; f0 < F; f0 += perSolve)
     bins_for(std::min(perSolve, F - f0), s);
   ensure_tinv(F, s);
@@ -283,14 +280,16 @@ void RbfPumPlan::prepare(int F, hipStream_t s) {
 void RbfPumPlan::ensure_tinv(int F, hipStream_t s) {
   if (F < 2)
     return;
-  if (tinv_.n < (size_t)std::max<int64_t>(M_, 1)) {
-    tinv_.alloc(std::max<int64_t>(M_, 1));
+  if (grow(tinv_, std::max<int64_t>(M_, 1)))
     launch_invert_perm(gt_.view.perm, M_, tinv_.get(), s);
-  }
-  if (wslot_.n < (size_t)std::max<int64_t>(pairs_.P, 1)) {
-    wslot_.alloc(std::max<int64_t>(pairs_.P, 1));
+  ensure_wslot(s);
+}
+
+// Shepard weight of every member slot (built once; read by the multi-field
+// blends, the transpose chain and the operator build).
+void RbfPumPlan::ensure_wslot(hipStream_t s) {
+  if (grow(wslot_, std::max<int64_t>(pairs_.P, 1)))
     launch_slot_weights(blendArgs(1), wslot_.get(), s);
-  }
 }
 
 void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
@@ -316,7 +315,8 @@ void RbfPumPlan::applyGraph(const double *dVals, int F, double *dOut,
     for (auto &e : gev_)
       PACMAN_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
-  if (!gexec_ || gVals_ != dVals || gOut_ != dOut || gF_ != F) {
+  if (!gexec_ || gVals_ != dVals || gOut_ != dOut || gF_ != F ||
+      gGen_ != bufGen_) {
     releaseGraph();
     PACMAN_HIP_CHECK(
         hipStreamBeginCapture(gstream_, hipStreamCaptureModeThreadLocal));
@@ -334,6 +334,7 @@ void RbfPumPlan::applyGraph(const double *dVals, int F, double *dOut,
     gVals_ = dVals;
     gOut_ = dOut;
     gF_ = F;
+    gGen_ = bufGen_;
   }
   PACMAN_HIP_CHECK(hipEventRecord(gev_[0], s));
   PACMAN_HIP_CHECK(hipStreamWaitEvent(gstream_, gev_[0], 0));
@@ -419,8 +420,7 @@ void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
   const int perLaunch = solve_fields_per_launch();
   const int Fc = std::min(F, perLaunch);
   if (!adjReady_) {
-    wslot_.alloc(std::max<int64_t>(pairs_.P, 1));
-    launch_slot_weights(blendArgs(1), wslot_.get(), s);
+    ensure_wslot(s);
     srcTOff_.alloc(N_ + 1);
     srcTEnt_.alloc(total);
     build_source_transpose(srcIdx_.get(), csrSize(0), N_, srcTOff_.get(),
@@ -537,10 +537,7 @@ void RbfPumPlan::buildOperator(hipStream_t s, bool fp32) {
   ensure_tinv(2, s);
   const int Fb = solve_fields_per_launch();
   const int64_t total = std::max<int64_t>(totalSrc_, 1);
-  if (wslot_.n < (size_t)std::max<int64_t>(pairs_.P, 1)) {
-    wslot_.alloc(std::max<int64_t>(pairs_.P, 1));
-    launch_slot_weights(blendArgs(1), wslot_.get(), s);
-  }
+  ensure_wslot(s);
   if (gamma_.n < (size_t)(total * Fb))
     gamma_.alloc((size_t)total * Fb);
   if (contribS_.n < (size_t)(total * Fb))

@@ -111,6 +111,19 @@ private:
   void enqueue(const double *dVals, int F, double *dOut, hipStream_t s,
                bool events, double *dState = nullptr);
   void releaseGraph();
+  // Grows a buffer that enqueue() reads or writes; a reallocation bumps the
+  // generation that a captured graph was recorded against, so the next graph
+  // apply captures again instead of replaying stale pointers. Returns true
+  // when it reallocated.
+  template <typename T> bool grow(DevBuf<T> &b, size_t n) {
+    if (b.n >= n)
+      return false;
+    b.alloc(n);
+    ++bufGen_;
+    return true;
+  }
+  void ensure_wslot(hipStream_t s);
+  uint64_t bufGen_ = 0, gGen_ = 0;
   SolveArgs solveArgs(int F, const double *vals) const;
   BlendArgs blendArgs(int F) const;
   bool adjReady_ = false;

@@ -99,6 +99,59 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data), 0
 
 
+def _check_points(p, dim, name):
+    if len(p.shape) != 2 or int(p.shape[1]) != dim:
+        raise ValueError(f"{name} must have shape (n, {dim}), got "
+                         f"{tuple(p.shape)}")
+
+
+def _fields_in(v, n, name, convert=True):
+    """(contiguous float64 fields, F, squeeze) of a (F, n) or (n,) array or
+    tensor; raises ValueError on a length or rank mismatch. A device tensor
+    of another dtype is converted unless convert is False (graph replay keeps
+    the caller's buffer)."""
+    if len(v.shape) not in (1, 2) or int(v.shape[-1]) != n:
+        raise ValueError(f"{name} must have shape (F, {n}) or ({n},), got "
+                         f"{tuple(v.shape)}")
+    squeeze = len(v.shape) == 1
+    F = 1 if squeeze else int(v.shape[0])
+    if F < 1:
+        raise ValueError(f"{name} holds no field")
+    if hasattr(v, "data_ptr"):
+        import torch
+        if v.dtype != torch.float64:
+            if not convert:
+                raise ValueError(f"{name} must be float64")
+            v = v.to(torch.float64)
+        v = v.contiguous()
+    else:
+        v = _as_f64(v)
+    return v, F, squeeze
+
+
+def _out_for(out, F, m, like, squeeze):
+    """A new (F, m) float64 output in the place of `like`, or the caller's
+    `out` after checking its shape, dtype, contiguity and place."""
+    if out is None:
+        if hasattr(like, "data_ptr"):
+            import torch
+            return torch.empty((F, m), dtype=torch.float64, device=like.device)
+        return np.zeros((F, m))
+    want = (m,) if squeeze else (F, m)
+    if tuple(out.shape) not in (want, (F, m)):
+        raise ValueError(f"out must have shape {want}, got {tuple(out.shape)}")
+    if hasattr(out, "data_ptr"):
+        import torch
+        if out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 tensor")
+    else:
+        if (not isinstance(out, np.ndarray) or out.dtype != np.float64
+                or not out.flags.c_contiguous or not out.flags.writeable):
+            raise ValueError("out must be a writeable C-contiguous float64 "
+                             "array")
+    return out
+
+
 class Plan:
     """Persistent RBF-PUM plan (setup once, apply to any number of fields).
 
@@ -117,12 +170,20 @@ class Plan:
         if int(rbf_function) not in _BASES:
             raise ValueError("unsupported rbf_function")
         src, tgt = source_points, target_points
+        if hasattr(src, "data_ptr") != hasattr(tgt, "data_ptr"):
+            raise ValueError("source and target points must both be numpy "
+                             "arrays or both torch tensors")
         if not hasattr(src, "data_ptr"):
             src, tgt = _as_f64(src), _as_f64(tgt)
         else:
             src, tgt = src.contiguous(), tgt.contiguous()
             if str(src.dtype) != "torch.float64" or str(tgt.dtype) != "torch.float64":
                 raise ValueError("points must be float64")
+        if self.dim not in (1, 2, 3):
+            raise RuntimeError(
+                "The dimension of the points can only be: 1, 2 or 3.\n")
+        _check_points(src, self.dim, "source_points")
+        _check_points(tgt, self.dim, "target_points")
         self._keep = (src, tgt)
         self.num_sources = int(src.shape[0])
         self.num_targets = int(tgt.shape[0])
@@ -154,19 +215,9 @@ class Plan:
         graph captured on the first call for these buffers: for repeated
         transfers that reuse the same value and output tensors.
         """
-        v = source_values
-        squeeze = len(v.shape) == 1
-        F = 1 if squeeze else int(v.shape[0])
-        if hasattr(v, "data_ptr"):
-            import torch
-            v = v.contiguous()
-            if out is None:
-                out = torch.empty((F, self.num_targets), dtype=torch.float64,
-                                  device=v.device)
-        else:
-            v = _as_f64(v)
-            if out is None:
-                out = np.zeros((F, self.num_targets))
+        v, F, squeeze = _fields_in(source_values, self.num_sources,
+                                   "source_values", convert=not graph)
+        out = _out_for(out, F, self.num_targets, v, squeeze)
         pv, dev = _ptr(v)
         po, devo = _ptr(out)
         if dev != devo:
@@ -189,19 +240,9 @@ class Plan:
         The gradient of a loss with respect to the source values, given its
         gradient with respect to the target values.
         """
-        g = target_grad
-        squeeze = len(g.shape) == 1
-        F = 1 if squeeze else int(g.shape[0])
-        if hasattr(g, "data_ptr"):
-            import torch
-            g = g.contiguous().to(torch.float64)
-            if out is None:
-                out = torch.empty((F, self.num_sources), dtype=torch.float64,
-                                  device=g.device)
-        else:
-            g = _as_f64(g)
-            if out is None:
-                out = np.zeros((F, self.num_sources))
+        g, F, squeeze = _fields_in(target_grad, self.num_targets,
+                                   "target_grad")
+        out = _out_for(out, F, self.num_sources, g, squeeze)
         pg, dev = _ptr(g)
         po, devo = _ptr(out)
         if dev != devo:
@@ -220,9 +261,7 @@ class Plan:
         v = source_values
         if not (hasattr(v, "data_ptr") and v.is_cuda):
             raise ValueError("apply_with_state needs device tensors")
-        squeeze = v.dim() == 1
-        F = 1 if squeeze else int(v.shape[0])
-        v = v.contiguous().to(torch.float64)
+        v, F, squeeze = _fields_in(v, self.num_sources, "source_values")
         out = torch.empty((F, self.num_targets), dtype=torch.float64,
                           device=v.device)
         n = int(self._lib.pacman_rbf_plan_state_size(self._h, F))
@@ -244,9 +283,12 @@ class Plan:
         g = target_grad
         if not (hasattr(g, "data_ptr") and g.is_cuda):
             raise ValueError("target_coord_grad needs device tensors")
-        F = 1 if g.dim() == 1 else int(g.shape[0])
-        g = g.contiguous().to(torch.float64)
+        g, F, _ = _fields_in(g, self.num_targets, "target_grad")
         n = int(self._lib.pacman_rbf_plan_state_size(self._h, F))
+        if (not hasattr(state, "data_ptr") or state.dtype != torch.float64
+                or not state.is_contiguous() or state.device != g.device):
+            raise ValueError("state must be a contiguous float64 tensor on "
+                             "the device of target_grad")
         if state.numel() < n:
             raise ValueError("state is smaller than this field count needs")
         out = torch.empty((self.num_targets, self.dim), dtype=torch.float64,

@@ -160,3 +160,59 @@ def test_fe_constants_and_validation():
     assert "sourcePoints is null" in _lib.last_error()
     assert lib.pacman_fe_interpolate_c(3, 4, 0xF0, p, 4, v, None, 0, None, 0,
                                        None, p, 0, o, s) == 0
+
+
+def test_pacman_alias_is_the_same_package():
+    """``import pacman`` (the reference's module name) gives this library."""
+    import pacman as ref_name
+    import pacman.rbf
+    assert ref_name.rbf is pacman.rbf is pacman_amd_rbf()
+    assert ref_name.execspaces.available() == ["HIP"]
+    assert ref_name.rbf.functions.WENDLANDC8 == 0x14
+    assert ref_name.MLS is ref_name.mls
+
+
+def pacman_amd_rbf():
+    import pacman_amd
+    return pacman_amd.rbf
+
+
+def test_plan_point_shapes_validated():
+    rng = np.random.default_rng(0)
+    S, T = rng.uniform(size=(50, 3)), rng.uniform(size=(40, 3))
+    with pytest.raises(ValueError):
+        pacman.rbf.Plan(2, 0x11, S, T)  # 3 columns, dimension 2
+    with pytest.raises(ValueError):
+        pacman.rbf.Plan(3, 0x11, S, T[:, :2])
+    with pytest.raises(ValueError):
+        pacman.rbf.Plan(3, 0x11, S.ravel(), T)
+    with pytest.raises(RuntimeError):
+        pacman.rbf.Plan(4, 0x11, rng.uniform(size=(50, 4)),
+                        rng.uniform(size=(40, 4)))
+
+
+def test_plan_field_and_output_checks():
+    """The helpers every Plan method runs before passing pointers to C."""
+    from pacman_amd.rbf import _fields_in, _out_for
+    v, F, sq = _fields_in(np.arange(10.0), 10, "v")
+    assert (F, sq) == (1, True) and v.dtype == np.float64
+    v, F, sq = _fields_in(np.zeros((3, 10), np.float32), 10, "v")
+    assert (F, sq) == (3, False) and v.dtype == np.float64
+    for bad in (np.zeros(9), np.zeros((2, 11)), np.zeros((2, 2, 10)),
+                np.zeros((0, 10))):
+        with pytest.raises(ValueError):
+            _fields_in(bad, 10, "v")
+    like = np.zeros((2, 10))
+    assert _out_for(None, 2, 7, like, False).shape == (2, 7)
+    _out_for(np.zeros((2, 7)), 2, 7, like, False)
+    _out_for(np.zeros(7), 1, 7, like, True)
+    for bad in (np.zeros((2, 6)), np.zeros((2, 7), np.float32),
+                np.zeros((7, 2)).T, np.zeros((3, 7))):
+        with pytest.raises(ValueError):
+            _out_for(bad, 2, 7, like, False)
+    torch = pytest.importorskip("torch")
+    with pytest.raises(ValueError):  # graph replay keeps the caller's buffer
+        _fields_in(torch.zeros(3, 10, dtype=torch.float32), 10, "v",
+                   convert=False)
+    with pytest.raises(ValueError):
+        _out_for(torch.zeros(2, 7, dtype=torch.float32), 2, 7, like, False)

@@ -65,3 +65,58 @@ def test_device_cache_reuse_gives_identical_results():
     assert lib.pacman_device_cache_bytes() == 0
     fresh = pacman.rbf.Plan(3, 0x11, S, T, 80, 0.15, -1.0).apply(f)
     assert np.array_equal(first, fresh)
+
+
+def test_graph_recaptured_after_buffers_grow():
+    """An eager apply with more fields reallocates the plan's scratch buffers;
+    the next graph apply with the old field count must capture again (a
+    replay of the first graph would write through freed pointers)."""
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(32)
+    S = rng.uniform(0, 1, (5000, 3))
+    T = rng.uniform(0, 1, (4000, 3))
+    V1 = W.trig_fields(S, 1, seed=2)
+    V8 = W.trig_fields(S, 8, seed=3)
+    dS, dT = torch.from_numpy(S).to(dev), torch.from_numpy(T).to(dev)
+    dV1, dV8 = torch.from_numpy(V1).to(dev), torch.from_numpy(V8).to(dev)
+    plan = pacman.rbf.Plan(3, 0x11, dS, dT, 60, 0.15, -1.0)
+    eager1 = plan.apply(dV1).clone()
+    out = torch.empty_like(eager1)
+    plan.apply(dV1, out=out, graph=True)          # captured at F = 1
+    torch.cuda.synchronize()
+    assert torch.equal(out, eager1)
+    eager8 = plan.apply(dV8).clone()              # grows the buffers
+    out.fill_(float("nan"))
+    plan.apply(dV1, out=out, graph=True)          # must capture again
+    torch.cuda.synchronize()
+    assert torch.equal(out, eager1)
+    assert torch.equal(plan.apply(dV8), eager8)
+    plan.apply_transpose(torch.ones_like(eager1))  # adjoint buffers
+    plan.apply(dV1, out=out, graph=True)
+    torch.cuda.synchronize()
+    assert torch.equal(out, eager1)
+
+
+@pytest.mark.parametrize("nfields", [1, 12])
+def test_graph_replay_matches_oracle(nfields, oracle_mod):
+    """Replays compared with the oracle itself (not only the eager apply)."""
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(33)
+    S = rng.uniform(0, 1, (5000, 3))
+    T = rng.uniform(0, 1, (4000, 3))
+    V = W.trig_fields(S, nfields, seed=4)
+    ref = oracle_mod.Setup(3, 0x11, S, T, 60, 0.15, -1.0).apply(V)
+    plan = pacman.rbf.Plan(3, 0x11, torch.from_numpy(S).to(dev),
+                           torch.from_numpy(T).to(dev), 60, 0.15, -1.0)
+    dV = torch.from_numpy(V).to(dev)
+    out = torch.empty((nfields, len(T)), dtype=torch.float64, device=dev)
+    for rep in range(3):
+        out.fill_(float("nan"))
+        plan.apply(dV, out=out, graph=True)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print(f"graph replay {rep} F={nfields}: rel-L2 vs oracle {err:.2e}")
+        assert err < 1e-10

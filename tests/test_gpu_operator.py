@@ -130,3 +130,23 @@ def test_operator_fp32_close_to_solve(rbf, F):
     assert terr < 1e-5, terr
     with pytest.raises(ValueError):
         plan.build_operator(precision="fp16")
+
+
+@pytest.mark.parametrize("F,fp32", [(1, False), (6, False), (24, False),
+                                    (24, True)])
+def test_operator_apply_matches_oracle(F, fp32, oracle_mod):
+    """The stored-operator apply (dot path up to 4 fields, FP64 MFMA
+    contraction from 5) compared with the oracle directly."""
+    rng = np.random.default_rng(300 + F)
+    S = rng.uniform(0, 1, (5000, 3))
+    T = rng.uniform(0, 1, (4500, 3))
+    V = W.trig_fields(S, F, seed=F)
+    ref = oracle_mod.Setup(3, 0x11, S, T, 60, 0.15, -1.0).apply(V)
+    plan = pacman.rbf.Plan(3, 0x11, S, T, 60, 0.15, -1.0)
+    plan.build_operator(precision="fp32" if fp32 else "fp64")
+    got = plan.apply(V)
+    err = _rel_max(got, ref)
+    tol = 1e-5 if fp32 else 1e-10
+    print(f"operator F={F} fp32={fp32}: max rel vs oracle {err:.2e} "
+          f"(tolerance {tol:g})")
+    assert err < tol

@@ -182,6 +182,179 @@ def test_membership_bruteforce(oracle_mod):
     assert np.allclose(sums, 1.0, rtol=0, atol=1e-14)
 
 
+def _psi_c2(p):
+    """Wendland C2, (1 - p)^4 (4 p + 1) on [0, 1), closed form."""
+    q = np.where(p < 1.0, 1.0 - p, 0.0)
+    return q ** 4 * (4.0 * p + 1.0)
+
+
+def _weights_numpy(T, C, off, idx, scale, kernel):
+    """Shepard weights of every (target, patch) entry of the per-target lists:
+    kernel(|y - c| * (1 / scale)) / sum over the target's list."""
+    out = np.zeros(len(idx))
+    for t in range(len(T)):
+        ks = idx[off[t]:off[t + 1]]
+        if len(ks) == 0:
+            continue
+        p = np.sqrt(_sq(pad3(T[t:t + 1])[0], pad3(C[ks]))) * (1.0 / scale)
+        v = kernel(p)
+        out[off[t]:off[t + 1]] = v / v.sum()
+    return out
+
+
+@pytest.mark.parametrize("dim,rbf", [(3, 0x12), (2, 0x13), (3, 0x10)])
+def test_shepard_weights_bruteforce(dim, rbf, oracle_mod):
+    """PU weights (solver.hpp:128-169): w_k(y) = psi(|y - c_k| / r) / sum,
+    psi = Wendland C2 scaled by the patch radius r (interpolator.hpp:86,91),
+    whatever the interpolation basis and its support. The basis here is not
+    C2 and the support (0.1) is not r, so using phi instead of psi, or the
+    support instead of r, fails the comparison (checked below)."""
+    rng = np.random.default_rng(40 + dim)
+    S = rng.uniform(0, 1, (1500, dim))
+    T = rng.uniform(0, 1, (900, dim))
+    o = oracle_mod.Setup(dim, rbf, S, T, 30, 0.15, 0.1)
+    C, _ = o.centers()
+    C = C[:, :dim]
+    r = o.radius
+    off, idx = o.csr(2)
+    w = o.weights()
+    assert len(w) == len(idx) and len(idx) > 2 * len(T)
+    ref = _weights_numpy(T, C, off, idx, r, _psi_c2)
+    np.testing.assert_allclose(w, ref, rtol=1e-14, atol=1e-16)
+    # planted mistakes: the basis phi instead of psi, the support instead of r
+    phi = np.vectorize(lambda p: oracle_mod.wendland(rbf, p, 1.0))
+    wrong_phi = _weights_numpy(T, C, off, idx, r, phi)
+    wrong_delta = _weights_numpy(T, C, off, idx, 0.1,
+                                 lambda p: _psi_c2(np.minimum(p, 1.0 - 1e-12)))
+    for bad in (wrong_phi, wrong_delta):
+        assert np.max(np.abs(bad - w)) > 1e-3
+
+
+def create_clusters_numpy(S, T, dim, r, overlap=0.15, variant=None):
+    """CreateClusters (clustering.hpp:137-270) restated by brute force:
+    lattice from the source box (last axis fastest), kill candidates whose
+    nearest source, then nearest target, is farther than r (both from the
+    lattice position), snap survivors to their nearest source (lowest index
+    among ties), the literal sequential prune of the host loop (:226-254),
+    re-tag against the targets from the snapped position (:256-262), stable
+    compaction. `variant` plants a step-order mistake (tests only)."""
+    S3, T3 = pad3(S), pad3(T)
+    lo, hi = S.min(0), S.max(0)
+    spacing = np.sqrt(4.0 / dim) * r * (1.0 - overlap)
+    shape, dist = [], []
+    for a in range(dim):
+        edge = abs(hi[a] - lo[a])
+        shp = int(np.ceil(max(1.0, edge / spacing)))
+        dist.append(edge / shp)
+        shape.append(shp + 1)
+    nb = int(np.prod(shape))
+    coords = np.zeros((nb, dim), np.int64)
+    rem = np.arange(nb)
+    for a in range(dim - 1, -1, -1):
+        coords[:, a] = rem % shape[a]
+        rem //= shape[a]
+    cand = np.zeros((nb, 3))
+    for a in range(dim):
+        cand[:, a] = lo[a] + dist[a] * coords[:, a].astype(np.float64)
+    r2 = r * r
+    alive = np.ones(nb, bool)
+
+    def nearest(c, P3):
+        d2 = _sq(c, P3)
+        i = int(np.argmin(d2))        # first index among ties
+        return i, d2[i]
+
+    for L in range(nb):
+        ls = cand[L].copy()
+        i, d2 = nearest(ls, S3)
+        if d2 > r2:
+            alive[L] = False
+            continue
+        if variant == "tag_targets_after_snap":
+            cand[L] = S3[i]
+        _, d2t = nearest(cand[L], T3)
+        if d2t > r2:
+            alive[L] = False
+            continue
+        cand[L] = S3[i]
+    thr = 0.4 * min(dist)
+    thr *= thr
+    offs = []
+    for t in range(3 ** dim):
+        o, tt = [0] * dim, t
+        for d in range(dim):
+            o[d] = tt % 3 - 1
+            tt //= 3
+        if any(o):
+            idx, stride = 0, 1
+            for d in range(dim - 1, -1, -1):
+                idx += o[d] * stride
+                stride *= shape[d]
+            offs.append(idx)
+    for i in range(nb):
+        if not alive[i]:
+            continue
+        for off in offs:
+            j = i + off
+            if 0 <= j < nb and alive[j] and _sq(cand[i], cand[j]) < thr:
+                alive[i] = False
+                break
+    if variant != "no_retag":
+        for L in range(nb):
+            if not alive[L]:
+                continue
+            pos = cand[L]
+            if variant == "retag_from_lattice":
+                pos = np.zeros(3)
+                for a in range(dim):
+                    pos[a] = lo[a] + dist[a] * float(coords[L, a])
+            if nearest(pos, T3)[1] > r2:
+                alive[L] = False
+    keep = np.nonzero(alive)[0]
+    return cand[keep], keep
+
+
+def _cluster_clouds():
+    rng = np.random.default_rng(77)
+    g = np.linspace(0.0, 1.0, 6)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    grid = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    clus = np.vstack([rng.normal(c, 0.04, (120, 2))
+                      for c in rng.uniform(0.2, 0.8, (3, 2))])
+    return [
+        ("1d", rng.uniform(0, 1, (70, 1)), rng.uniform(-0.1, 1.1, (50, 1)), 6),
+        ("2d", rng.uniform(0, 1, (180, 2)), rng.uniform(0, 1, (90, 2)), 10),
+        ("2d-clustered", clus, rng.uniform(0, 1, (150, 2)), 12),
+        ("3d", rng.uniform(0, 1, (400, 3)), rng.uniform(0, 1, (250, 3)), 15),
+        # a regular grid: nearest-source ties everywhere (lowest index wins)
+        ("3d-grid", grid, grid[::2] + 0.05, 8),
+        ("3d-sparse-targets", rng.uniform(0, 1, (500, 3)),
+         rng.uniform(0.3, 0.7, (40, 3)), 12),
+    ]
+
+
+def test_create_clusters_bruteforce(oracle_mod):
+    """Centres and candidate indices of the oracle equal the brute-force
+    restatement exactly, on 1-D/2-D/3-D clouds including a regular grid;
+    each planted step-order mistake changes the result on some cloud."""
+    misses = {"tag_targets_after_snap": 0, "retag_from_lattice": 0,
+              "no_retag": 0}
+    for name, S, T, npc in _cluster_clouds():
+        dim = S.shape[1]
+        r, _, _ = find_radius_numpy(S, dim, npc)
+        o = oracle_mod.Setup(dim, 0x11, S, T, npc, 0.15, -1.0)
+        assert o.radius == r, name
+        C, cand = o.centers()
+        refC, refCand = create_clusters_numpy(S, T, dim, r)
+        np.testing.assert_array_equal(cand, refCand, err_msg=name)
+        np.testing.assert_array_equal(C, refC, err_msg=name)
+        for v in misses:
+            bc, bi = create_clusters_numpy(S, T, dim, r, variant=v)
+            if len(bi) != len(cand) or np.any(bi != cand) or np.any(bc != C):
+                misses[v] += 1
+    assert all(m > 0 for m in misses.values()), misses
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_grid_queries_equal_bruteforce(seed, oracle_mod):
     rng = np.random.default_rng(seed)
