@@ -119,6 +119,33 @@ int pacman_rbf_plan_centers(pacman_rbf_plan plan, double *centers,
 long long pacman_rbf_plan_csr_size(pacman_rbf_plan plan, int which);
 int pacman_rbf_plan_csr(pacman_rbf_plan plan, int which, long long *offsets,
                         long long *indices);
+/* Patches holding more than 192 source points (no reference counterpart:
+ * the reference solves every patch the same way). They are solved by a
+ * persistent workspace kernel (blocked L D L^T in device memory, any size);
+ * the transpose apply, the stored operator and the target-coordinate
+ * gradient need every patch within 192 points and return an error
+ * otherwise. */
+long long pacman_rbf_plan_num_large_patches(pacman_rbf_plan plan);
+
+/* Arithmetic of the workspace solve (no reference counterpart; the
+ * reference is FP64 throughout, serial_solve.hpp:76-84). precision 0: FP64.
+ * precision 1 (mixed): FP32 assembly and factorisation, FP64 iterative
+ * refinement with a matrix-free FP64 residual (at most maxRefinements
+ * steps) and a per-patch FP64 fallback when it does not converge; results
+ * agree with FP64 to refinement accuracy (about 1e-13 relative). Patches of
+ * up to 192 points are solved in FP64 either way. Takes effect at the next
+ * apply. */
+int pacman_rbf_plan_set_precision(pacman_rbf_plan plan, int precision,
+                                  int maxRefinements);
+
+/* Counters of the last apply (synchronises `stream`): counters3[0] patches
+ * found numerically singular (pivot <= 1e-12 phi(0): duplicate or
+ * near-duplicate sources; their values are not meaningful),
+ * counters3[1] refinement steps summed over patches, counters3[2] patches
+ * that fell back to FP64. */
+int pacman_rbf_plan_apply_counters(pacman_rbf_plan plan, int *counters3,
+                                   void *stream);
+
 /* After an apply: active polynomial columns and pivot status per patch. */
 int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
                                int *status);

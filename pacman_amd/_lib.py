@@ -52,6 +52,10 @@ SIGNATURES = {
     "pacman_rbf_plan_num_patches": (ctypes.c_longlong, [_vp]),
     "pacman_rbf_plan_num_pairs": (ctypes.c_longlong, [_vp]),
     "pacman_rbf_plan_max_patch_size": (ctypes.c_int, [_vp]),
+    "pacman_rbf_plan_num_large_patches": (ctypes.c_longlong, [_vp]),
+    "pacman_rbf_plan_set_precision": (ctypes.c_int, [_vp, ctypes.c_int,
+                                                     ctypes.c_int]),
+    "pacman_rbf_plan_apply_counters": (ctypes.c_int, [_vp, _ip, _vp]),
     "pacman_rbf_plan_lattice": (ctypes.c_int, [_vp, _llp, _llp]),
     "pacman_rbf_plan_centers": (ctypes.c_int, [_vp, _dp, _llp]),
     "pacman_rbf_plan_csr_size": (ctypes.c_longlong, [_vp, ctypes.c_int]),

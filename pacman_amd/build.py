@@ -46,6 +46,7 @@ UNITS = {
     "dev_alloc.hip": [],
     "rbf_solve.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "rbf_grad.hip": [],
+    "rbf_solve_ws.hip": [],
     "rbf_solve_tile_a.hip": [],
     "rbf_solve_tile_b.hip": [],
     "rbf_solve_tile_c.hip": [],

@@ -101,6 +101,15 @@ void run_host_transfer(int dim, int kind, const double *src, int64_t ns,
     PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
                                     hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  // The reference's unpivoted LU returns inf / NaN silently on duplicate
+  // sources (serial_solve.hpp:76-84); here such a patch is an error.
+  int c[3];
+  plan.applyCounters(c, s);
+  if (c[0] > 0)
+    throw std::runtime_error(
+        std::to_string(c[0]) +
+        " patch(es) are numerically singular (a pivot of the local RBF "
+        "matrix vanished: duplicate or near-duplicate source points)");
 }
 
 // Nearest-source transfer on host arrays.
@@ -824,6 +833,36 @@ long long pacman_rbf_plan_num_pairs(pacman_rbf_plan plan) {
 
 int pacman_rbf_plan_max_patch_size(pacman_rbf_plan plan) {
   return plan && plan->plan ? plan->plan->maxPatchSize() : -1;
+}
+
+long long pacman_rbf_plan_num_large_patches(pacman_rbf_plan plan) {
+  return plan && plan->plan ? (long long)plan->plan->numLargePatches() : -1;
+}
+
+int pacman_rbf_plan_set_precision(pacman_rbf_plan plan, int precision,
+                                  int maxRefinements) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    plan->plan->setPrecision(precision, maxRefinements);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_set_precision", e.what());
+    return -1;
+  }
+}
+
+int pacman_rbf_plan_apply_counters(pacman_rbf_plan plan, int *counters3,
+                                   void *stream) {
+  try {
+    if (!plan || !plan->plan || !counters3)
+      throw std::invalid_argument("plan or output is null");
+    plan->plan->applyCounters(counters3, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_apply_counters", e.what());
+    return -1;
+  }
 }
 
 int pacman_rbf_plan_lattice(pacman_rbf_plan plan, long long *shape3,

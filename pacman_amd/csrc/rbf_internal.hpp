@@ -22,8 +22,11 @@
 
 namespace pacman_amd {
 
-// Largest patch (sources) the register-resident solve takes.
+// Largest patch (sources) the register-resident solve takes; larger ones go
+// through the workspace solve (rbf_solve_ws.hip).
 constexpr int kMaxPatch = 192;
+// phi(0) of Wendland C0 .. C8 (rbf_functions.hpp:22-150, not normalised).
+constexpr double kPhi0[5] = {1.0, 1.0, 3.0, 1.0, 15.0};
 
 // Device memory through the library's cache (dev_alloc.hip): blocks freed
 // by one plan are reused by the next allocation of the same rounded size.
@@ -165,7 +168,49 @@ struct SolveArgs {
   // >= 0, field f's upstream value is the Shepard weight at pair opRow0 + f
   // of each patch and 0 at its other pairs (gSorted is not read).
   int opRow0;
+  // Per-patch 1 / support radius (adaptive patches); null: dinv for all.
+  const double *patchDinv;
+  // Pivots d <= 1e-12 phi(0) (or not finite) mark a numerically singular
+  // patch; each one found adds 1 to *singular (when set).
+  double phi0;
+  int *singular;
 };
+
+// ---- workspace solve (rbf_solve_ws.hip), patches of any size -------------
+constexpr int kWsBlock = 32;   // block edge of the workspace factorisation
+// counters[]: work queue of the first pass, fallback list length, singular
+// patches, refinement iterations, work queue of the FP64 fallback pass,
+// fallbacks of the whole apply
+constexpr int kWsQueue = 0, kWsFallbacks = 1, kWsSingular = 2,
+              kWsIrIters = 3, kWsQueue2 = 4, kWsFallbackTotal = 5,
+              kWsCounters = 6;
+struct WsArgs {
+  SolveArgs a;              // patches = queue (largest first), count
+  const double *coef;       // K x F x 4 polynomial coefficients
+  int nbMax;                // block rows per slot (covers n + F)
+  void *work;               // per slot: lower block triangle + Linv blocks
+  int64_t slotBlocks;       // elements (of the storage type) per slot
+  int64_t linvOffset;       // first Linv block within a slot
+  double *vec;              // per slot: coordinates, pivots, 4 x F vectors
+  int64_t slotVec;          // doubles per slot
+  int *counters;            // kWsCounters ints
+  int *fallback;            // patches the FP32 refinement hands to FP64
+  const int *countPtr;      // non-null: queue length read on the device
+  int queueIdx;             // counter that serves as this launch's queue
+  int maxIR;                // refinement iteration limit (FP32)
+  double phi0;              // phi(0) of the basis (pivot threshold scale)
+};
+int64_t ws_slot_blocks(int nbMax);
+int64_t ws_linv_offset(int nbMax);
+int64_t ws_slot_vec(int nbMax);
+// all = true: every counter (start of an apply); false: the queues and the
+// fallback list only (next chunk of fields).
+void launch_ws_reset(int *counters, bool all, hipStream_t s);
+// One persistent launch of `grid` workgroups over the queue: FP64 storage
+// (fp32 = false; singular pivots counted) or FP32 storage with FP64
+// refinement (non-converging patches appended to the fallback list).
+void launch_ws_solve(bool fp32, int kind, const WsArgs &w, int grid,
+                     hipStream_t s);
 // Fields handled by one poly/solve launch.
 int solve_fields_per_launch();
 // Polynomial fit of every patch (one wave per patch): coef[K][F][4], pvOut.

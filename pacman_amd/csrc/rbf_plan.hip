@@ -11,6 +11,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 
 namespace pacman_amd {
 
@@ -149,15 +150,13 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   maxN_ = 0;
   for (int64_t kk = 0; kk < K_; ++kk)
     maxN_ = std::max(maxN_, (int)(off[kk + 1] - off[kk]));
-  if (maxN_ > kMaxPatch)
-    throw std::runtime_error(
-        "a patch holds " + std::to_string(maxN_) +
-        " source points; this build solves patches of up to " +
-        std::to_string(kMaxPatch) + " points (lower nodesPerCluster)");
   patchN_.resize(K_);
   for (int64_t kk = 0; kk < K_; ++kk)
     patchN_[kk] = (int)(off[kk + 1] - off[kk]);
   totalSrc_ = off[K_];
+  counters_.alloc(kWsCounters);
+  launch_ws_reset(counters_.get(), true, s);
+  plan_workspace(s);
   bins_.resize(solve_fields_per_launch());
   bins_for(1, s);
   pv_.alloc(std::max<int64_t>(K_, 1));
@@ -176,7 +175,8 @@ const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
     return B;
   std::vector<std::vector<int>> bins(solve_bin_rows(kMaxPatch, F) + 1);
   for (int64_t kk = 0; kk < K_; ++kk)
-    bins[solve_bin_rows(patchN_[kk], F)].push_back((int)kk);
+    if (patchN_[kk] <= kMaxPatch)  // larger ones: workspace path
+      bins[solve_bin_rows(patchN_[kk], F)].push_back((int)kk);
   for (int R = 0; R < (int)bins.size(); ++R) {
     if (bins[R].empty())
       continue;
@@ -382,6 +382,8 @@ SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
   a.status = status_.get();
   a.M = M_;
   a.opRow0 = -1;
+  a.phi0 = kPhi0[prm_.kind];
+  a.singular = counters_.get() + kWsSingular;
   return a;
 }
 
@@ -412,6 +414,7 @@ BlendArgs RbfPumPlan::blendArgs(int F) const {
 // source sums its patch entries in ascending patch order.
 void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
                                 hipStream_t s) {
+  require_register_patches("the transpose apply");
   if (F < 1)
     throw std::invalid_argument("transpose apply needs at least one field");
   if (N_ == 0)
@@ -479,6 +482,7 @@ void RbfPumPlan::applyState(const double *dVals, int F, double *dOut,
 // one blend with the derivative of the Shepard weights.
 void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
                                  double *dCoordGrad, hipStream_t s) {
+  require_register_patches("the target-coordinate gradient");
   if (F < 1)
     throw std::invalid_argument("target gradient needs at least one field");
   if (M_ == 0)
@@ -511,6 +515,7 @@ void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
 // (halves its memory and the bytes each apply reads; applied with FP64
 // accumulation).
 void RbfPumPlan::buildOperator(hipStream_t s, bool fp32) {
+  require_register_patches("the stored operator");
   releaseGraph();
   op_.release();
   op32_.release();
@@ -590,6 +595,7 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
   const int64_t total = totalSrc_;
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[0], s));
+  launch_ws_reset(counters_.get(), true, s);
   launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[1], s));
@@ -624,6 +630,11 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
       a.patches = B.dev[b].get();
       a.count = B.count[b];
       launch_solve_f64(B.R[b], prm_.kind, a, coef_.get(), s);
+    }
+    if (!wsHost_.empty()) {
+      if (f0 > 0)
+        launch_ws_reset(counters_.get(), false, s);
+      launch_workspace(a, s);
     }
     BlendArgs bl = blendArgs(nf);
     bl.out = dOut + (int64_t)f0 * M_;
@@ -754,6 +765,109 @@ void RbfPumPlan::patchInfo(int *pv, int *status, hipStream_t s) const {
     PACMAN_HIP_CHECK(hipMemcpyAsync(status, status_.get(), 4 * K_,
                                     hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Patches above kMaxPatch sources, largest first (ties: lower patch index),
+// and the per-slot workspaces of their persistent launch: a slot holds the
+// lower block triangle of the largest patch plus its Linv blocks (FP64 size;
+// FP32 storage uses the first half) and 4 + 4 F vectors. Slots: two per CU,
+// fewer if the workspaces would exceed the budget (PACMAN_AMD_WS_BUDGET_MB,
+// default 8192).
+void RbfPumPlan::plan_workspace(hipStream_t s) {
+  wsHost_.clear();
+  for (int64_t k = 0; k < K_; ++k)
+    if (patchN_[k] > kMaxPatch)
+      wsHost_.push_back((int)k);
+  if (wsHost_.empty())
+    return;
+  std::stable_sort(wsHost_.begin(), wsHost_.end(), [&](int x, int y) {
+    return patchN_[x] > patchN_[y];
+  });
+  const int nbMax = (maxN_ + solve_fields_per_launch() + kWsBlock - 1) /
+                    kWsBlock;
+  const int64_t slotBytes =
+      8 * (ws_slot_blocks(nbMax) + ws_slot_vec(nbMax));
+  int cu = 0;
+  int dev = 0;
+  PACMAN_HIP_CHECK(hipGetDevice(&dev));
+  PACMAN_HIP_CHECK(
+      hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+  double budgetMb = 8192.0;
+  if (const char *e = std::getenv("PACMAN_AMD_WS_BUDGET_MB"))
+    budgetMb = std::max(1.0, std::atof(e));
+  const int64_t bySize = std::max<int64_t>(
+      1, (int64_t)(budgetMb * 1048576.0) / std::max<int64_t>(slotBytes, 1));
+  wsSlots_ = (int)std::min<int64_t>(
+      {(int64_t)wsHost_.size(), 2 * (int64_t)std::max(cu, 1), bySize});
+  wsNbMax_ = nbMax;
+  wsDev_.alloc(wsHost_.size());
+  PACMAN_HIP_CHECK(hipMemcpyAsync(wsDev_.get(), wsHost_.data(),
+                                  4 * wsHost_.size(), hipMemcpyHostToDevice,
+                                  s));
+  wsFallback_.alloc(wsHost_.size());
+  wsWork_.alloc((size_t)wsSlots_ * ws_slot_blocks(nbMax));
+  wsVec_.alloc((size_t)wsSlots_ * ws_slot_vec(nbMax));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // wsHost_ copied
+}
+
+void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s) {
+  WsArgs w{};
+  w.a = a;
+  w.a.patches = wsDev_.get();
+  w.a.count = (int)wsHost_.size();
+  w.coef = coef_.get();
+  w.nbMax = wsNbMax_;
+  w.work = wsWork_.get();
+  w.slotBlocks = ws_slot_blocks(wsNbMax_);
+  w.linvOffset = ws_linv_offset(wsNbMax_);
+  w.vec = wsVec_.get();
+  w.slotVec = ws_slot_vec(wsNbMax_);
+  w.counters = counters_.get();
+  w.fallback = wsFallback_.get();
+  w.countPtr = nullptr;
+  w.queueIdx = kWsQueue;
+  w.maxIR = prm_.maxIR;
+  w.phi0 = kPhi0[prm_.kind];
+  const bool mixed = prm_.precision == 1;
+  launch_ws_solve(mixed, prm_.kind, w, wsSlots_, s);
+  if (mixed) {
+    // FP64 pass over the patches the refinement handed back (the list
+    // length is read on the device; an empty list ends every workgroup at
+    // once)
+    w.a.patches = wsFallback_.get();
+    w.countPtr = counters_.get() + kWsFallbacks;
+    w.queueIdx = kWsQueue2;
+    launch_ws_solve(false, prm_.kind, w, wsSlots_, s);
+  }
+}
+
+void RbfPumPlan::setPrecision(int precision, int maxIR) {
+  if (precision != 0 && precision != 1)
+    throw std::invalid_argument("precision must be 0 (FP64) or 1 (mixed)");
+  if (maxIR < 0 || maxIR > 100)
+    throw std::invalid_argument("refinement iteration limit out of range");
+  releaseGraph();
+  prm_.precision = precision;
+  prm_.maxIR = maxIR;
+}
+
+void RbfPumPlan::applyCounters(int *out, hipStream_t s) const {
+  int c[kWsCounters];
+  PACMAN_HIP_CHECK(hipMemcpyAsync(c, counters_.get(), sizeof(c),
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  out[0] = c[kWsSingular];
+  out[1] = c[kWsIrIters];
+  out[2] = c[kWsFallbackTotal];
+}
+
+void RbfPumPlan::require_register_patches(const char *what) const {
+  if (!wsHost_.empty())
+    throw std::runtime_error(
+        std::string(what) + " is available for patches of up to " +
+        std::to_string(kMaxPatch) + " source points; this plan has " +
+        std::to_string(wsHost_.size()) + " larger patches (largest " +
+        std::to_string(maxN_) + ")");
 }
 
 } // namespace pacman_amd

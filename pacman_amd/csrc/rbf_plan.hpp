@@ -18,6 +18,11 @@ struct PlanParams {
   int npc = 50;            // nodes per cluster (interpolator.hxx:39)
   double overlap = 0.15;   // relative overlap (interpolator.hxx:40)
   double support = 0.1;    // Wendland support radius; <= 0 means the patch radius
+  // 0: FP64 throughout; 1: mixed - patches solved on the workspace path
+  // store FP32 factors and are refined to FP64 accuracy (iterative
+  // refinement, FP64 fallback per patch).
+  int precision = 0;
+  int maxIR = 6;           // refinement iteration limit (mixed)
 };
 
 class RbfPumPlan {
@@ -64,6 +69,13 @@ public:
   const int64_t *latticeShape() const { return lat_.shape; }
   int64_t numPairs() const { return pairs_.P; }
   int maxPatchSize() const { return maxN_; }
+  // Patches above kMaxPatch sources (solved on the workspace path).
+  int64_t numLargePatches() const { return (int64_t)wsHost_.size(); }
+  void setPrecision(int precision, int maxIR);
+  // Counters of the last apply (synchronises s): 0 numerically singular
+  // patches, 1 refinement iterations (summed over patches), 2 patches that
+  // fell back from FP32 refinement to FP64.
+  void applyCounters(int *out, hipStream_t s) const;
 
   // Host copies for inspection and parity tests (original point indices,
   // each list sorted ascending).
@@ -105,6 +117,14 @@ private:
     std::vector<DevBuf<int>> dev;
   };
   std::vector<Bins> bins_;
+  // Workspace path: patches above kMaxPatch sources, largest first.
+  std::vector<int> wsHost_;
+  DevBuf<int> wsDev_, wsFallback_, counters_;
+  int wsNbMax_ = 0, wsSlots_ = 0;
+  DevBuf<double> wsWork_, wsVec_;  // wsWork_ also holds FP32 blocks
+  void plan_workspace(hipStream_t s);
+  void require_register_patches(const char *what) const;
+  void launch_workspace(const SolveArgs &a, hipStream_t s);
   std::vector<int> patchN_;  // sources per patch (host)
   const Bins &bins_for(int F, hipStream_t s);
   void prepare(int F, hipStream_t s);

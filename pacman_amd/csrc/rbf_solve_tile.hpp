@@ -436,7 +436,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
           dd = (c == i) ? v[c] : dd;
         dpan[buf][i] = dd;
         dinvp[buf][i] = i < kk ? fast_rcp(dd) : 1.0;
-        if (i < kk && dd == 0.0)
+        if (i < kk && !(dd > 1e-12 * a.phi0))
           spivot = 1;
       }
 #pragma unroll
@@ -693,6 +693,8 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 
   if (tid == 0 && a.status)
     a.status[patch] = spivot;
+  if (tid == 0 && spivot && a.singular)
+    atomicAdd(a.singular, 1);
 
   if constexpr (ADJ) {
     for (int t = tid; t < n; t += NT)

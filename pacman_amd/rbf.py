@@ -163,7 +163,8 @@ class Plan:
     def __init__(self, space_dimension, rbf_function, source_points,
                  target_points, nodes_per_cluster=DEFAULT_NODES_PER_CLUSTER,
                  relative_overlap=DEFAULT_RELATIVE_OVERLAP,
-                 support_radius=DEFAULT_SUPPORT_RADIUS, stream=None):
+                 support_radius=DEFAULT_SUPPORT_RADIUS, stream=None,
+                 precision="fp64", max_refinements=6):
         self._lib = _lib.load()
         self._h = ctypes.c_void_p()
         self.dim = int(space_dimension)
@@ -200,6 +201,39 @@ class Plan:
         if rc != 0:
             raise RuntimeError(_lib.last_error())
         self.on_device = bool(dev)
+        if precision != "fp64" or max_refinements != 6:
+            self.set_precision(precision, max_refinements)
+
+    _PRECISIONS = {"fp64": 0, "mixed": 1}
+
+    def set_precision(self, precision="fp64", max_refinements=6):
+        """Arithmetic of the patches above 192 points (the workspace solve):
+        "fp64", or "mixed" (FP32 assembly and factorisation, FP64 iterative
+        refinement with at most max_refinements steps, FP64 fallback per
+        patch). Smaller patches are solved in FP64 either way."""
+        if precision not in self._PRECISIONS:
+            raise ValueError('precision must be "fp64" or "mixed"')
+        rc = self._lib.pacman_rbf_plan_set_precision(
+            self._h, self._PRECISIONS[precision], int(max_refinements))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        self.precision = precision
+
+    def counters(self, stream=None) -> dict:
+        """Counters of the last apply (synchronises): patches found
+        numerically singular (duplicate sources; their values are not
+        meaningful), refinement steps summed over patches, patches that fell
+        back from FP32 refinement to FP64."""
+        c = (ctypes.c_int * 3)()
+        rc = self._lib.pacman_rbf_plan_apply_counters(
+            self._h, c, ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return {"singular": c[0], "refinements": c[1], "fallbacks": c[2]}
+
+    @property
+    def num_large_patches(self) -> int:
+        return int(self._lib.pacman_rbf_plan_num_large_patches(self._h))
 
     def __del__(self):
         h = getattr(self, "_h", None)
