@@ -119,6 +119,28 @@ int pacman_rbf_plan_centers(pacman_rbf_plan plan, double *centers,
 long long pacman_rbf_plan_csr_size(pacman_rbf_plan plan, int which);
 int pacman_rbf_plan_csr(pacman_rbf_plan plan, int which, long long *offsets,
                         long long *indices);
+/* A plan on density-aware patches (no reference counterpart: the reference
+ * derives one patch radius from FindRadius, clusters_radius.hpp:22-125,
+ * which on strongly clustered clouds gives patches of one to ~1e5 points and
+ * leaves sparse regions uncovered). A binary tree over Morton codes of the
+ * points halves every cell holding more than leafPoints sources; each leaf
+ * with targets is a patch centred on its cell, radius = max((1 +
+ * relativeOverlap) * half the cell diagonal, distance to the minPoints-th
+ * nearest source), patches holding more than maxPoints sources are halved
+ * again; Wendland support = supportScale * radius, PU weights Wendland C2
+ * over each patch's own radius. Every target lies inside some patch. The
+ * target-coordinate gradient is not available on such plans. */
+int pacman_rbf_plan_create_adaptive(
+    pacman_rbf_plan *plan, int spaceDimension, int rbfFunction,
+    int leafPoints, int minPoints, int maxPoints, double relativeOverlap,
+    double supportScale, const double *sourcePoints, long long nSourcePoints,
+    const double *targetPoints, long long nTargetPoints, int onDevice,
+    void *stream);
+
+/* radii[K]: the radius of every patch (the same value for all patches of a
+ * plan built by pacman_rbf_plan_create). */
+int pacman_rbf_plan_patch_radii(pacman_rbf_plan plan, double *radii);
+
 /* Patches holding more than 192 source points (no reference counterpart:
  * the reference solves every patch the same way). They are solved by a
  * persistent workspace kernel (blocked L D L^T in device memory, any size);

@@ -85,6 +85,14 @@ def lib():
             _dp]
         L.oracle_setup_target_grad.argtypes = [ctypes.c_void_p, _dp,
                                                ctypes.c_int, _dp, _dp]
+        L.oracle_adaptive_create.restype = ctypes.c_void_p
+        L.oracle_adaptive_create.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
+            ctypes.c_int64, _dp, ctypes.c_int64]
+        L.oracle_adaptive_patches.argtypes = [ctypes.c_void_p, _dp, _dp, _ip]
+        L.oracle_morton_code.restype = ctypes.c_uint64
+        L.oracle_morton_code.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.oracle_fe_nearest.argtypes = [
             ctypes.c_int, _dp, ctypes.c_int64, _dp, _dp, ctypes.c_int64,
             ctypes.c_int, _dp, _ip]
@@ -230,6 +238,44 @@ class Setup:
         if rc != 0:
             raise ValueError("oracle_setup_target_grad failed")
         return out
+
+
+class AdaptiveSetup(Setup):
+    """Oracle state of the density-aware patch builder (not in the
+    reference; rbf_pum_oracle.cpp, runAdaptiveSetup): patch centres, per-
+    patch radii and supports, CSR lists, weights, apply."""
+
+    def __init__(self, dim, rbf_function, source_points, target_points,
+                 leaf_points=40, min_points=20, max_points=600,
+                 overlap=0.15, support_scale=1.0):
+        self.dim = int(dim)
+        self.src = _f64(source_points).reshape(-1, self.dim)
+        self.tgt = _f64(target_points).reshape(-1, self.dim)
+        self.ns, self.nt = self.src.shape[0], self.tgt.shape[0]
+        self._h = lib().oracle_adaptive_create(
+            self.dim, int(rbf_function), int(leaf_points), int(min_points),
+            int(max_points), float(overlap), float(support_scale),
+            _ptr(self.src), self.ns, _ptr(self.tgt), self.nt)
+        if not self._h:
+            raise ValueError("oracle rejected the parameters")
+
+    def patches(self):
+        """(radius, support, depth) per patch."""
+        K = self.num_patches
+        r, d = np.zeros(K), np.zeros(K)
+        depth = np.zeros(K, np.int32)
+        lib().oracle_adaptive_patches(self._h, _ptr(r), _ptr(d),
+                                      _ptr(depth, _ip))
+        return r, d, depth
+
+
+def morton_code(x, dim, lo, ext) -> int:
+    """The builder's 16-bit-per-axis Morton code of one point."""
+    x = _f64(x)
+    lo = _f64(lo)
+    ext = _f64(ext)
+    return int(lib().oracle_morton_code(_ptr(x), int(dim), _ptr(lo),
+                                        _ptr(ext)))
 
 
 def rbf_interpolate(dim, rbf_function, source_points, source_values,

@@ -420,6 +420,9 @@ struct Setup {
   std::vector<i64> wOff, wIdx;        // CSR per target: centres within r
   std::vector<double> w;              // normalised PU weights
   double minMargin = std::numeric_limits<double>::infinity();
+  // adaptive patches only: radius and support of every patch
+  std::vector<double> patchR, patchDelta;
+  std::vector<int> patchDepth;        // tree depth of the patch's cell
 };
 
 std::vector<double> pad3(const double *p, i64 n, int dim) {
@@ -700,6 +703,259 @@ void runSetup(const Params &prm, const std::vector<double> &S, i64 ns,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adaptive patches. Not in the reference: its single FindRadius radius
+// (clusters_radius.hpp:22-125) gives, on strongly clustered clouds, patches
+// of one to ~1e5 points and leaves sparse regions uncovered. This builder
+// (the GPU one in rbf_adaptive.hip follows it step by step) adapts the
+// patches to the source density:
+//  1. box = bounding box of sources and targets; each point gets a Morton
+//     code of its coordinates quantised to 16 bits per axis,
+//     q = min(65535, trunc((x - lo) / ext * 65536)), bits interleaved from
+//     the most significant, axis 0 first;
+//  2. a binary tree over the code bits: a cell (code prefix of d bits) with
+//     more than `leaf` sources is halved along the axis of bit d; cells with
+//     neither sources nor targets are dropped; a leaf that holds at least
+//     one target becomes a patch;
+//  3. patch centre = centre of the cell; radius r = max(rc, dk) with
+//     rc = (1 + overlap) * half the cell diagonal (the ball covers the cell,
+//     so every target is inside some patch) and dk = distance to the
+//     nmin-th nearest source (every patch holds at least nmin sources);
+//  4. a patch whose ball holds more than nmax sources is halved again (its
+//     children with targets replace it) until none does;
+//  5. patches ordered by cell (prefix << (bits - d)); support delta_k =
+//     supportScale * r_k; members: sources within r_k, targets within
+//     r_k - 1e-14 (solver.hpp:62); PU weights psi(|y - c_k| / r_k) over the
+//     patches within r_k of each target, psi = Wendland C2 (as
+//     solver.hpp:128-169 with a per-patch radius).
+// ---------------------------------------------------------------------------
+constexpr int kMortonBits = 16;
+
+struct AdaptiveParams {
+  int leaf = 40, nmin = 20, nmax = 600;
+  double overlap = 0.15, supportScale = 1.0;
+};
+
+inline uint64_t quantize(double x, double lo, double ext) {
+  const double t = ((x - lo) / ext) * 65536.0;
+  if (!(t > 0.0))
+    return 0;
+  if (t >= 65535.0)
+    return 65535;
+  return (uint64_t)t;
+}
+
+// Morton prefix of depth d of the cell with integer coordinates j[a] (b_a =
+// (d + D - 1 - a) / D bits on axis a).
+inline uint64_t cellPrefix(const uint64_t *j, int D, int d) {
+  uint64_t p = 0;
+  for (int t = 0; t < d; ++t) {
+    const int a = t % D;
+    const int ba = (d + D - 1 - a) / D;
+    const int bit = ba - 1 - t / D;
+    p = (p << 1) | ((j[a] >> bit) & 1u);
+  }
+  return p;
+}
+
+inline uint64_t mortonCode(const double *x, int D, const double *lo,
+                           const double *ext) {
+  uint64_t q[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a)
+    q[a] = quantize(x[a], lo[a], ext[a]);
+  return cellPrefix(q, D, kMortonBits * D);
+}
+
+struct AdaptiveCell {
+  uint64_t p;
+  int d;
+};
+
+// Geometry of a cell: centre (padded to 3) and cover radius.
+void cellGeometry(const AdaptiveCell &c, int D, const double *lo,
+                  const double *ext, double overlap, double *cen,
+                  double *rc) {
+  double s = 0.0;
+  for (int a = 0; a < 3; ++a)
+    cen[a] = 0.0;
+  for (int a = 0; a < D; ++a) {
+    const int ba = (c.d + D - 1 - a) / D;
+    uint64_t ia = 0;  // axis-a bits of the prefix, most significant first
+    for (int t = a; t < c.d; t += D)
+      ia = (ia << 1) | ((c.p >> (c.d - 1 - t)) & 1u);
+    const double h = ext[a] * std::ldexp(1.0, -ba);
+    cen[a] = lo[a] + ((double)ia + 0.5) * h;
+    s = s + h * h;
+  }
+  *rc = (1.0 + overlap) * (0.5 * std::sqrt(s));
+}
+
+void runAdaptiveSetup(const Params &prm, const AdaptiveParams &ap,
+                      const std::vector<double> &S, i64 ns,
+                      const std::vector<double> &T, i64 nt, Setup &st) {
+  const int D = prm.dim, DB = kMortonBits * D;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
+  for (int a = 0; a < D; ++a) {
+    lo[a] = std::numeric_limits<double>::max();
+    hi[a] = -std::numeric_limits<double>::max();
+    for (i64 i = 0; i < ns; ++i) {
+      lo[a] = std::min(lo[a], S[3 * i + a]);
+      hi[a] = std::max(hi[a], S[3 * i + a]);
+    }
+    for (i64 i = 0; i < nt; ++i) {
+      lo[a] = std::min(lo[a], T[3 * i + a]);
+      hi[a] = std::max(hi[a], T[3 * i + a]);
+    }
+    ext[a] = hi[a] - lo[a];
+    if (!(ext[a] > 0.0))
+      ext[a] = 1.0;
+    st.lower[a] = lo[a];
+    st.upper[a] = hi[a];
+  }
+  std::vector<uint64_t> sc(ns), tc(nt);
+  for (i64 i = 0; i < ns; ++i)
+    sc[i] = mortonCode(&S[3 * i], D, lo, ext);
+  for (i64 i = 0; i < nt; ++i)
+    tc[i] = mortonCode(&T[3 * i], D, lo, ext);
+  std::vector<uint64_t> ss(sc), ts(tc);
+  std::sort(ss.begin(), ss.end());
+  std::sort(ts.begin(), ts.end());
+  auto countIn = [&](const std::vector<uint64_t> &v, const AdaptiveCell &c) {
+    const uint64_t a = c.p << (DB - c.d), b = (c.p + 1) << (DB - c.d);
+    return (i64)(std::lower_bound(v.begin(), v.end(), b) -
+                 std::lower_bound(v.begin(), v.end(), a));
+  };
+  // 2. the tree
+  std::vector<AdaptiveCell> leaves, stack{{0, 0}};
+  while (!stack.empty()) {
+    const AdaptiveCell c = stack.back();
+    stack.pop_back();
+    const i64 nsrc = countIn(ss, c), ntgt = countIn(ts, c);
+    if (nsrc > ap.leaf && c.d < DB) {
+      for (int h = 0; h < 2; ++h) {
+        const AdaptiveCell ch{2 * c.p + (uint64_t)h, c.d + 1};
+        if (countIn(ss, ch) > 0 || countIn(ts, ch) > 0)
+          stack.push_back(ch);
+      }
+    } else if (ntgt > 0) {
+      leaves.push_back(c);
+    }
+  }
+  // 3-4. radii; halve patches holding more than nmax sources
+  const i64 kq = std::min<i64>(ap.nmin, ns);
+  struct Patch {
+    AdaptiveCell c;
+    double cen[3], r;
+    i64 n;
+  };
+  std::vector<Patch> done;
+  std::vector<AdaptiveCell> todo = leaves;
+  while (!todo.empty()) {
+    std::vector<Patch> cur(todo.size());
+#pragma omp parallel for schedule(dynamic, 16)
+    for (i64 i = 0; i < (i64)todo.size(); ++i) {
+      Patch &P = cur[i];
+      P.c = todo[i];
+      double rc;
+      cellGeometry(P.c, D, lo, ext, ap.overlap, P.cen, &rc);
+      std::vector<double> d2(ns);
+      for (i64 j = 0; j < ns; ++j)
+        d2[j] = sqdiff(P.cen, &S[3 * j]);
+      std::nth_element(d2.begin(), d2.begin() + (kq - 1), d2.end());
+      const double dk = std::sqrt(d2[kq - 1]);
+      P.r = std::max(rc, dk);
+      i64 n = 0;
+      for (i64 j = 0; j < ns; ++j)
+        n += dist(P.cen, &S[3 * j]) <= P.r ? 1 : 0;
+      P.n = n;
+    }
+    todo.clear();
+    for (const Patch &P : cur) {
+      if (P.n > ap.nmax && P.c.d < DB) {
+        for (int h = 0; h < 2; ++h) {
+          const AdaptiveCell ch{2 * P.c.p + (uint64_t)h, P.c.d + 1};
+          if (countIn(ts, ch) > 0)
+            todo.push_back(ch);
+        }
+      } else {
+        done.push_back(P);
+      }
+    }
+  }
+  std::sort(done.begin(), done.end(), [&](const Patch &x, const Patch &y) {
+    return (x.c.p << (DB - x.c.d)) < (y.c.p << (DB - y.c.d));
+  });
+  const i64 K = (i64)done.size();
+  st.centers.assign((size_t)K * 3, 0.0);
+  st.patchR.resize(K);
+  st.patchDelta.resize(K);
+  st.patchDepth.resize(K);
+  st.centerCandidate.resize(K);
+  for (i64 k = 0; k < K; ++k) {
+    for (int a = 0; a < 3; ++a)
+      st.centers[3 * k + a] = done[k].cen[a];
+    st.patchR[k] = done[k].r;
+    st.patchDelta[k] = ap.supportScale * done[k].r;
+    st.patchDepth[k] = done[k].c.d;
+    st.centerCandidate[k] = (i64)(done[k].c.p << (DB - done[k].c.d));
+  }
+  // 5. membership and weights, brute force
+  std::vector<std::vector<i64>> sl(K), tl(K);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (i64 k = 0; k < K; ++k) {
+    const double *c = &st.centers[3 * k];
+    const double r = st.patchR[k];
+    for (i64 j = 0; j < ns; ++j)
+      if (dist(c, &S[3 * j]) <= r)
+        sl[k].push_back(j);
+    for (i64 j = 0; j < nt; ++j)
+      if (dist(c, &T[3 * j]) <= r - kEps)
+        tl[k].push_back(j);
+  }
+  st.srcOff.assign(K + 1, 0);
+  st.tgtOff.assign(K + 1, 0);
+  for (i64 k = 0; k < K; ++k) {
+    st.srcOff[k + 1] = st.srcOff[k] + (i64)sl[k].size();
+    st.tgtOff[k + 1] = st.tgtOff[k] + (i64)tl[k].size();
+  }
+  st.srcIdx.clear();
+  st.tgtIdx.clear();
+  for (i64 k = 0; k < K; ++k) {
+    st.srcIdx.insert(st.srcIdx.end(), sl[k].begin(), sl[k].end());
+    st.tgtIdx.insert(st.tgtIdx.end(), tl[k].begin(), tl[k].end());
+  }
+  std::vector<std::vector<i64>> wl(nt);
+  std::vector<std::vector<double>> wv(nt);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i64 t = 0; t < nt; ++t) {
+    const double *y = &T[3 * t];
+    double sum = 0.0;
+    for (i64 k = 0; k < K; ++k) {
+      const double d = dist(y, &st.centers[3 * k]);
+      if (d <= st.patchR[k]) {
+        const double w = wendland(1, d, 1.0 / st.patchR[k]);
+        wl[t].push_back(k);
+        wv[t].push_back(w);
+        sum += w;
+      }
+    }
+    for (double &w : wv[t])
+      w /= sum;
+  }
+  st.wOff.assign(nt + 1, 0);
+  for (i64 t = 0; t < nt; ++t)
+    st.wOff[t + 1] = st.wOff[t] + (i64)wl[t].size();
+  st.wIdx.resize(st.wOff[nt]);
+  st.w.resize(st.wOff[nt]);
+  for (i64 t = 0; t < nt; ++t) {
+    std::copy(wl[t].begin(), wl[t].end(), st.wIdx.begin() + st.wOff[t]);
+    std::copy(wv[t].begin(), wv[t].end(), st.w.begin() + st.wOff[t]);
+  }
+  st.radius = 0.0;
+  for (double r : st.patchR)
+    st.radius = std::max(st.radius, r);
+}
+
 // Per-patch solve (solver.hpp:217-286) for F fields; writes the local
 // interpolant at each member target into loc (indexed like tgtIdx).
 struct PatchOut {
@@ -719,9 +975,11 @@ void solvePatches(const Params &prm, const std::vector<double> &S,
   if (po.wantGrad)
     po.grad.assign(st.tgtIdx.size() * (size_t)F * 3, 0.0);
   po.pv.assign(K, 0);
-  const double dinv = 1.0 / delta;
 #pragma omp parallel for schedule(dynamic, 1)
   for (i64 k = kBegin; k < kEnd; ++k) {
+    // one support for all patches, or per patch (adaptive patches)
+    const double dinv =
+        1.0 / (st.patchDelta.empty() ? delta : st.patchDelta[k]);
     const i64 s0 = st.srcOff[k], n = st.srcOff[k + 1] - s0;
     const i64 t0 = st.tgtOff[k], m = st.tgtOff[k + 1] - t0;
     if (n == 0)
@@ -1062,6 +1320,48 @@ int oracle_setup_apply(void *h, const double *vals, int F, int64_t kBegin,
   if (pvOut)
     std::copy(po.pv.begin(), po.pv.end(), pvOut);
   return 0;
+}
+
+// Adaptive setup (the density-aware builder above). support: supportScale
+// (delta_k = supportScale * r_k).
+void *oracle_adaptive_create(int dim, int rbfFunction, int leaf, int nmin,
+                             int nmax, double overlap, double supportScale,
+                             const double *src, int64_t ns, const double *tgt,
+                             int64_t nt) {
+  if (dim < 1 || dim > 3 || kindOf(rbfFunction) < 0 || ns <= 0 || nt < 0 ||
+      leaf < 1 || nmin < 1 || nmax < nmin || !(overlap > 0 && overlap < 1) ||
+      !(supportScale > 0))
+    return nullptr;
+  auto *o = new OracleSetup;
+  o->prm.dim = dim;
+  o->prm.kind = kindOf(rbfFunction);
+  o->prm.support = -1.0;
+  o->S = pad3(src, ns, dim);
+  o->T = pad3(tgt, nt, dim);
+  o->ns = ns;
+  o->nt = nt;
+  AdaptiveParams ap;
+  ap.leaf = leaf;
+  ap.nmin = nmin;
+  ap.nmax = nmax;
+  ap.overlap = overlap;
+  ap.supportScale = supportScale;
+  runAdaptiveSetup(o->prm, ap, o->S, ns, o->T, nt, o->st);
+  return o;
+}
+
+// Per patch: radius, support, tree depth.
+void oracle_adaptive_patches(void *h, double *radius, double *support,
+                             int *depth) {
+  auto &st = static_cast<OracleSetup *>(h)->st;
+  std::copy(st.patchR.begin(), st.patchR.end(), radius);
+  std::copy(st.patchDelta.begin(), st.patchDelta.end(), support);
+  std::copy(st.patchDepth.begin(), st.patchDepth.end(), depth);
+}
+
+uint64_t oracle_morton_code(const double *x, int dim, const double *lo,
+                            const double *ext) {
+  return mortonCode(x, dim, lo, ext);
 }
 
 // Gradient with respect to the target coordinates (nt x dim) of

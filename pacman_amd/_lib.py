@@ -53,6 +53,11 @@ SIGNATURES = {
     "pacman_rbf_plan_num_pairs": (ctypes.c_longlong, [_vp]),
     "pacman_rbf_plan_max_patch_size": (ctypes.c_int, [_vp]),
     "pacman_rbf_plan_num_large_patches": (ctypes.c_longlong, [_vp]),
+    "pacman_rbf_plan_create_adaptive": (ctypes.c_int, [
+        ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp,
+        ctypes.c_longlong, _vp, ctypes.c_longlong, ctypes.c_int, _vp]),
+    "pacman_rbf_plan_patch_radii": (ctypes.c_int, [_vp, _dp]),
     "pacman_rbf_plan_set_precision": (ctypes.c_int, [_vp, ctypes.c_int,
                                                      ctypes.c_int]),
     "pacman_rbf_plan_apply_counters": (ctypes.c_int, [_vp, _ip, _vp]),

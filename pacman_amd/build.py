@@ -40,6 +40,7 @@ COMMON = [
 UNITS = {
     "rbf_geometry.hip": ["-ffp-contract=off"],
     "rbf_plan.hip": ["-ffp-contract=off"],
+    "rbf_adaptive.hip": ["-ffp-contract=off"],
     "capi.hip": ["-ffp-contract=off"],
     "fe_nearest.hip": ["-ffp-contract=off"],
     "mls.hip": ["-ffp-contract=off"],

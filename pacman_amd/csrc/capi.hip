@@ -835,6 +835,75 @@ int pacman_rbf_plan_max_patch_size(pacman_rbf_plan plan) {
   return plan && plan->plan ? plan->plan->maxPatchSize() : -1;
 }
 
+int pacman_rbf_plan_create_adaptive(
+    pacman_rbf_plan *plan, int spaceDimension, int rbfFunction,
+    int leafPoints, int minPoints, int maxPoints, double relativeOverlap,
+    double supportScale, const double *sourcePoints, long long nSourcePoints,
+    const double *targetPoints, long long nTargetPoints, int onDevice,
+    void *stream) {
+  try {
+    if (!plan)
+      throw std::invalid_argument("plan is null");
+    *plan = nullptr;
+    if (sourcePoints == nullptr && nSourcePoints > 0)
+      throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
+    if (targetPoints == nullptr && nTargetPoints > 0)
+      throw std::invalid_argument("targetPoints is null but nTargetPoints > 0");
+    if (spaceDimension < 1 || spaceDimension > 3)
+      throw std::runtime_error(
+          "The dimension of the points can only be: 1, 2 or 3.");
+    const int kind = kind_of(rbfFunction);
+    if (kind < 0)
+      throw std::invalid_argument("Unsupported RBF function selector: " +
+                                  std::to_string(rbfFunction));
+    hipStream_t s = (hipStream_t)stream;
+    PlanParams p;
+    p.dim = spaceDimension;
+    p.kind = kind;
+    p.overlap = relativeOverlap;
+    p.adaptive = true;
+    p.ap.leaf = leafPoints;
+    p.ap.nmin = minPoints;
+    p.ap.nmax = maxPoints;
+    p.ap.overlap = relativeOverlap;
+    p.ap.supportScale = supportScale;
+    auto h = std::make_unique<pacman_rbf_plan_s>();
+    if (onDevice) {
+      h->plan = std::make_unique<RbfPumPlan>(p, sourcePoints, nSourcePoints,
+                                             targetPoints, nTargetPoints, s);
+    } else {
+      DevBuf<double> ds((size_t)std::max<long long>(nSourcePoints, 0) *
+                        spaceDimension),
+          dt((size_t)std::max<long long>(nTargetPoints, 0) * spaceDimension);
+      if (nSourcePoints > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(ds.get(), sourcePoints, 8 * ds.n,
+                                        hipMemcpyHostToDevice, s));
+      if (nTargetPoints > 0)
+        PACMAN_HIP_CHECK(hipMemcpyAsync(dt.get(), targetPoints, 8 * dt.n,
+                                        hipMemcpyHostToDevice, s));
+      h->plan = std::make_unique<RbfPumPlan>(p, ds.get(), nSourcePoints,
+                                             dt.get(), nTargetPoints, s);
+    }
+    *plan = h.release();
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_create_adaptive", e.what());
+    return -1;
+  }
+}
+
+int pacman_rbf_plan_patch_radii(pacman_rbf_plan plan, double *radii) {
+  try {
+    if (!plan || !plan->plan || !radii)
+      throw std::invalid_argument("plan or output is null");
+    plan->plan->patchRadii(radii, nullptr);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_patch_radii", e.what());
+    return -1;
+  }
+}
+
 long long pacman_rbf_plan_num_large_patches(pacman_rbf_plan plan) {
   return plan && plan->plan ? (long long)plan->plan->numLargePatches() : -1;
 }

@@ -1,0 +1,791 @@
+// Density-aware ("adaptive") patches on the GPU. Not in the reference: its
+// single patch radius from FindRadius (src/rbf_pum/clustering/
+// clusters_radius.hpp:22-125) gives, on strongly clustered clouds, patches
+// of one to ~1e5 points and leaves sparse regions uncovered. The steps and
+// every floating-point expression follow the oracle's runAdaptiveSetup
+// (compiled here with -ffp-contract=off, so centres, radii and member lists
+// are bit-identical):
+//
+//  1. box of sources and targets; 16-bit-per-axis Morton codes (quantised
+//     coordinate q = min(65535, trunc((x - lo) / ext * 65536)), bits
+//     interleaved from the most significant, axis 0 first); a stable LSD
+//     radix sort (8-bit digits) orders each point set by code, which also
+//     gives the Morton-ordered SoA coordinates the solve reads;
+//  2. a binary tree over the code bits, one launch per depth: a cell with
+//     more than `leaf` sources is halved (the split position is a binary
+//     search in the sorted codes), cells with neither sources nor targets
+//     are dropped, leaves with targets become patches;
+//  3. per patch (one wave): centre = cell centre, rc = (1 + overlap) * half
+//     the cell diagonal; when fewer than nmin sources lie within rc the
+//     radius is the distance to the nmin-th nearest source (bisection on
+//     the bits of the squared distances over an exact candidate set);
+//  4. patches holding more than nmax sources are halved again;
+//  5. patches sorted by cell key; members: sources within r_k, targets
+//     within r_k (weights) and r_k - 1e-14 (members).
+//
+// Ball queries read the sorted codes: the 3^D cells around the centre at
+// the deepest depth whose cells are at least (1 + 1e-9) r wide on every
+// axis hold every point within r; each cell is one contiguous code range
+// (two binary searches, one lane each), scanned by the wave.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "rbf_internal.hpp"
+
+namespace pacman_amd {
+namespace {
+
+constexpr int kBits = 16;
+constexpr int kThreads = 256;
+
+struct Frame {
+  double lo[3], ext[3];
+  int D, DB;
+  double hmin[3 * kBits + 1];  // smallest cell edge at each depth
+};
+
+struct SortedSet {
+  const uint64_t *code;
+  const double *x, *y, *z;
+  int n;
+};
+
+__host__ __device__ inline uint64_t quantize(double x, double lo, double ext) {
+  const double t = ((x - lo) / ext) * 65536.0;
+  if (!(t > 0.0))
+    return 0;
+  if (t >= 65535.0)
+    return 65535;
+  return (uint64_t)t;
+}
+
+__host__ __device__ inline int axis_bits(int d, int D, int a) {
+  return (d + D - 1 - a) / D;
+}
+
+__host__ __device__ inline uint64_t cell_prefix(const uint64_t *j, int D,
+                                                int d) {
+  uint64_t p = 0;
+  for (int t = 0; t < d; ++t) {
+    const int a = t % D;
+    const int bit = axis_bits(d, D, a) - 1 - t / D;
+    p = (p << 1) | ((j[a] >> bit) & 1u);
+  }
+  return p;
+}
+
+__device__ inline int lower_bound_u64(const uint64_t *a, int lo, int hi,
+                                      uint64_t key) {
+  while (lo < hi) {
+    const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+    if (a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// ---- codes and the radix sort ---------------------------------------------
+
+__global__ void morton_codes_kernel(const double *pts, int64_t n, int dim,
+                                    Frame f, uint64_t *key, int *val) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  uint64_t q[3] = {0, 0, 0};
+  for (int a = 0; a < dim; ++a)
+    q[a] = quantize(pts[i * dim + a], f.lo[a], f.ext[a]);
+  key[i] = cell_prefix(q, dim, f.DB);
+  val[i] = (int)i;
+}
+
+constexpr int kRounds = 16;
+constexpr int kTile = kThreads * kRounds;
+
+__global__ __launch_bounds__(kThreads) void rs_hist_kernel(const uint64_t *key,
+                                                           int64_t n, int shift,
+                                                           int nblk, int *hist) {
+  __shared__ int h[256];
+  const int tid = threadIdx.x;
+  h[tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t i = base + (int64_t)r * kThreads + tid;
+    if (i < n)
+      atomicAdd(&h[(key[i] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)tid * nblk + blockIdx.x] = h[tid];
+}
+
+// Stable scatter: within a round the order is wave by wave, lane by lane
+// (elements in input order); rounds in order; blocks through the digit-major
+// scan of the histograms.
+__global__ __launch_bounds__(kThreads) void rs_scatter_kernel(
+    const uint64_t *kin, const int *vin, int64_t n, int shift, int nblk,
+    const int64_t *hoff, uint64_t *kout, int *vout) {
+  __shared__ int64_t base[256];
+  __shared__ int wcnt[kThreads / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  base[tid] = hoff[(int64_t)tid * nblk + blockIdx.x];
+  for (int q = tid; q < (kThreads / 64) * 256; q += kThreads)
+    wcnt[q >> 8][q & 255] = 0;
+  __syncthreads();
+  const int64_t b0 = (int64_t)blockIdx.x * kTile;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kRounds; ++r) {
+    const int64_t i = b0 + (int64_t)r * kThreads + tid;
+    const bool valid = i < n;
+    const uint64_t k = valid ? kin[i] : 0;
+    const int v = valid ? vin[i] : 0;
+    const int dig = valid ? (int)((k >> shift) & 255u) : 0;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool on = (dig >> b) & 1;
+      const uint64_t bb = __ballot(valid && on);
+      m &= on ? bb : ~bb;
+    }
+    const int rank = __popcll(m & below);
+    const int tot = __popcll(m);
+    if (valid && rank == tot - 1)
+      wcnt[wave][dig] = tot;
+    __syncthreads();
+    if (valid) {
+      int pre = 0;
+      for (int w = 0; w < wave; ++w)
+        pre += wcnt[w][dig];
+      const int64_t pos = base[dig] + pre + rank;
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    int add = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+      add += wcnt[w][tid];
+      wcnt[w][tid] = 0;
+    }
+    base[tid] += add;
+    __syncthreads();
+  }
+}
+
+// Sorts (key, val) by the low `bits` bits of key (a multiple of 16), stably;
+// the result is back in (k0, v0).
+void radix_sort_pairs(uint64_t *k0, int *v0, uint64_t *k1, int *v1, int64_t n,
+                      int bits, hipStream_t s) {
+  if (n <= 1)
+    return;
+  const int nblk = (int)((n + kTile - 1) / kTile);
+  DevBuf<int> hist((size_t)256 * nblk);
+  DevBuf<int64_t> hoff((size_t)256 * nblk + 1);
+  uint64_t *ka = k0, *kb = k1;
+  int *va = v0, *vb = v1;
+  for (int shift = 0; shift < bits; shift += 8) {
+    rs_hist_kernel<<<nblk, kThreads, 0, s>>>(ka, n, shift, nblk, hist.get());
+    PACMAN_HIP_CHECK(hipGetLastError());
+    exclusive_scan_i32_to_i64(hist.get(), hoff.get(), (int64_t)256 * nblk, s);
+    rs_scatter_kernel<<<nblk, kThreads, 0, s>>>(ka, va, n, shift, nblk,
+                                                hoff.get(), kb, vb);
+    PACMAN_HIP_CHECK(hipGetLastError());
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != k0) {  // odd number of passes
+    PACMAN_HIP_CHECK(hipMemcpyAsync(k0, ka, 8 * n, hipMemcpyDeviceToDevice, s));
+    PACMAN_HIP_CHECK(hipMemcpyAsync(v0, va, 4 * n, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+__global__ void gather_sorted_kernel(const double *pts, int64_t n, int dim,
+                                     const int *order, double *x, double *y,
+                                     double *z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const int64_t o = order[i];
+  x[i] = pts[o * dim];
+  y[i] = dim > 1 ? pts[o * dim + 1] : 0.0;
+  z[i] = dim > 2 ? pts[o * dim + 2] : 0.0;
+}
+
+// ---- the tree --------------------------------------------------------------
+
+struct TNode {
+  uint64_t p;
+  int d, slo, shi, tlo, thi, pad;
+};
+
+__global__ void tree_level_kernel(const TNode *in, int nIn, TNode *out,
+                                  int *counters, int maxOut, uint64_t *leafP,
+                                  int *leafD, int maxLeaf, SortedSet S,
+                                  SortedSet T, int leaf, int DB) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nIn)
+    return;
+  const TNode nd = in[i];
+  if (nd.shi - nd.slo > leaf && nd.d < DB) {
+    const uint64_t key = (2 * nd.p + 1) << (DB - nd.d - 1);
+    const int ms = lower_bound_u64(S.code, nd.slo, nd.shi, key);
+    const int mt = lower_bound_u64(T.code, nd.tlo, nd.thi, key);
+    const TNode c[2] = {{2 * nd.p, nd.d + 1, nd.slo, ms, nd.tlo, mt, 0},
+                        {2 * nd.p + 1, nd.d + 1, ms, nd.shi, mt, nd.thi, 0}};
+    for (int h = 0; h < 2; ++h)
+      if (c[h].shi > c[h].slo || c[h].thi > c[h].tlo) {
+        const int o = atomicAdd(&counters[0], 1);
+        if (o < maxOut)
+          out[o] = c[h];
+      }
+  } else if (nd.thi > nd.tlo) {
+    const int o = atomicAdd(&counters[1], 1);
+    if (o < maxLeaf) {
+      leafP[o] = nd.p;
+      leafD[o] = nd.d;
+    }
+  }
+}
+
+// ---- ball queries ------------------------------------------------------------
+
+__device__ inline void cell_geometry(uint64_t p, int d, const Frame &f,
+                                     double overlap, double *cen, double *rc) {
+  double s = 0.0;
+  cen[0] = cen[1] = cen[2] = 0.0;
+  for (int a = 0; a < f.D; ++a) {
+    const int ba = axis_bits(d, f.D, a);
+    uint64_t ia = 0;
+    for (int t = a; t < d; t += f.D)
+      ia = (ia << 1) | ((p >> (d - 1 - t)) & 1u);
+    const double h = f.ext[a] * ldexp(1.0, -ba);
+    cen[a] = f.lo[a] + ((double)ia + 0.5) * h;
+    s = s + h * h;
+  }
+  *rc = (1.0 + overlap) * (0.5 * sqrt(s));
+}
+
+__device__ inline int depth_for(const Frame &f, double r) {
+  int d = 0;
+  for (int dd = 1; dd <= f.DB; ++dd) {
+    if (f.hmin[dd] >= r * (1.0 + 1e-9))
+      d = dd;
+    else
+      break;
+  }
+  return d;
+}
+
+// The code ranges of the 3^D cells around c at depth d: lane l < 3^D holds
+// range l in [rs, re) (empty outside the box).
+struct Ranges {
+  int rs, re, nr;
+};
+
+__device__ inline Ranges neighbour_ranges(const SortedSet &S, const Frame &f,
+                                          const double *c, int d) {
+  const int lane = threadIdx.x & 63;
+  Ranges g;
+  g.nr = f.D == 1 ? 3 : f.D == 2 ? 9 : 27;
+  g.rs = g.re = 0;
+  if (lane < g.nr) {
+    uint64_t j[3] = {0, 0, 0};
+    bool ok = true;
+    int t = lane;
+    for (int a = 0; a < f.D; ++a) {
+      const int o = t % 3 - 1;
+      t /= 3;
+      const int ba = axis_bits(d, f.D, a);
+      const int64_t ia =
+          (int64_t)(quantize(c[a], f.lo[a], f.ext[a]) >> (kBits - ba)) + o;
+      if (ia < 0 || ia >= ((int64_t)1 << ba))
+        ok = false;
+      j[a] = (uint64_t)(ia < 0 ? 0 : ia);
+    }
+    if (ok) {
+      const uint64_t p = cell_prefix(j, f.D, d);
+      const int sh = f.DB - d;
+      g.rs = lower_bound_u64(S.code, 0, S.n, p << sh);
+      g.re = lower_bound_u64(S.code, g.rs, S.n, (p + 1) << sh);
+    }
+  }
+  return g;
+}
+
+// f(position, valid) over every candidate of the ranges, in range order
+// (runs packed into one wave pass while they fit in its 64 lanes).
+template <class Fn>
+__device__ inline void scan_ranges(const Ranges &g, Fn &&fn) {
+  const int lane = threadIdx.x & 63;
+  int r = 0;
+  int cs = __shfl(g.rs, 0), ce = __shfl(g.re, 0);
+  while (r < g.nr) {
+    if (ce - cs > 64) {
+      for (; ce - cs > 64; cs += 64)
+        fn(cs + lane, true);
+      continue;
+    }
+    int used = 0, p = -1;
+    while (r < g.nr && used + (ce - cs) <= 64) {
+      const int len = ce - cs;
+      if (lane >= used && lane < used + len)
+        p = cs + (lane - used);
+      used += len;
+      if (++r < g.nr) {
+        cs = __shfl(g.rs, r);
+        ce = __shfl(g.re, r);
+      }
+    }
+    fn(p < 0 ? 0 : p, p >= 0);
+  }
+}
+
+__device__ inline double dist3(double ax, double ay, double az, double bx,
+                               double by, double bz) {
+  return sqrt(sqdiff3(ax, ay, az, bx, by, bz));
+}
+
+__device__ inline int ball_count(const SortedSet &S, const Frame &f,
+                                 const double *c, double r) {
+  const Ranges g = neighbour_ranges(S, f, c, depth_for(f, r));
+  int cnt = 0;
+  scan_ranges(g, [&](int p, bool valid) {
+    const bool hit =
+        valid && dist3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= r;
+    cnt += __popcll(__ballot(hit));
+  });
+  return cnt;
+}
+
+// Distance to the kq-th nearest point of S from c (kq <= S.n).
+__device__ inline double knn_distance(const SortedSet &S, const Frame &f,
+                                      const double *c, int kq, int d) {
+  Ranges g;
+  double lim2 = 0.0;
+  for (;; --d) {
+    g = neighbour_ranges(S, f, c, d);
+    if (d == 0)
+      break;
+    const double lim = f.hmin[d] / (1.0 + 1e-9);
+    lim2 = lim * lim;
+    int cnt = 0;
+    scan_ranges(g, [&](int p, bool valid) {
+      const bool hit =
+          valid && sqdiff3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= lim2;
+      cnt += __popcll(__ballot(hit));
+    });
+    if (cnt >= kq)
+      break;
+  }
+  // smallest t with #{d2 <= t} >= kq, bisected on the bits of t
+  uint64_t lo = 0,
+           hi = d == 0 ? 0x7ff0000000000000ull : (uint64_t)__double_as_longlong(lim2);
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    const double t = __longlong_as_double((long long)mid);
+    int cnt = 0;
+    scan_ranges(g, [&](int p, bool valid) {
+      const bool hit =
+          valid && sqdiff3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= t;
+      cnt += __popcll(__ballot(hit));
+    });
+    if (cnt >= kq)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return sqrt(__longlong_as_double((long long)hi));
+}
+
+// One wave per cell: radius, source count; cells within nmax are appended
+// to the done list, larger ones are halved (children with targets go to the
+// next work list).
+struct CellEvalArgs {
+  const uint64_t *cellP;
+  const int *cellD;
+  int count;
+  SortedSet S, T;
+  Frame f;
+  double overlap;
+  int kq, nmax;
+  uint64_t *nextP;
+  int *nextD;
+  int maxNext;
+  uint64_t *doneKey;
+  int *doneD;
+  double *doneC, *doneR;  // doneC: 3 per entry
+  int maxDone;
+  int *counters;          // [0] next, [1] done
+};
+
+__global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
+  const int i = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (i >= e.count)
+    return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t p = e.cellP[i];
+  const int d = e.cellD[i];
+  double c[3], rc;
+  cell_geometry(p, d, e.f, e.overlap, c, &rc);
+  int n = ball_count(e.S, e.f, c, rc);
+  double r = rc;
+  if (n < e.kq) {
+    r = knn_distance(e.S, e.f, c, e.kq, depth_for(e.f, rc));
+    if (r < rc)  // max(rc, dk) exactly
+      r = rc;
+    n = ball_count(e.S, e.f, c, r);
+  }
+  if (lane != 0)
+    return;
+  if (n > e.nmax && d < e.f.DB) {
+    for (int h = 0; h < 2; ++h) {
+      const uint64_t cp = 2 * p + (uint64_t)h;
+      const int sh = e.f.DB - d - 1;
+      const int a = lower_bound_u64(e.T.code, 0, e.T.n, cp << sh);
+      const int b = lower_bound_u64(e.T.code, a, e.T.n, (cp + 1) << sh);
+      if (b > a) {
+        const int o = atomicAdd(&e.counters[0], 1);
+        if (o < e.maxNext) {
+          e.nextP[o] = cp;
+          e.nextD[o] = d + 1;
+        }
+      }
+    }
+  } else {
+    const int o = atomicAdd(&e.counters[1], 1);
+    if (o < e.maxDone) {
+      e.doneKey[o] = p << (e.f.DB - d);
+      e.doneD[o] = d;
+      e.doneC[3 * o] = c[0];
+      e.doneC[3 * o + 1] = c[1];
+      e.doneC[3 * o + 2] = c[2];
+      e.doneR[o] = r;
+    }
+  }
+}
+
+// Final patch arrays in key order.
+__global__ void patch_arrays_kernel(const int *order, int K, const int *doneD,
+                                    const double *doneC, const double *doneR,
+                                    double supportScale, double *cx,
+                                    double *cy, double *cz, double *r,
+                                    double *rinv, double *dinv, int *depth) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K)
+    return;
+  const int o = order[k];
+  cx[k] = doneC[3 * o];
+  cy[k] = doneC[3 * o + 1];
+  cz[k] = doneC[3 * o + 2];
+  const double rr = doneR[o];
+  r[k] = rr;
+  rinv[k] = 1.0 / rr;
+  dinv[k] = 1.0 / (supportScale * rr);
+  depth[k] = doneD[o];
+}
+
+// Membership: count pass (fill = false) and fill pass. Sources: positions
+// within r_k. Targets: positions within r_k with the member flag (within
+// r_k - 1e-14) in the top bit, and the patch of every pair.
+struct MemberArgs {
+  SortedSet P;
+  Frame f;
+  const double *cx, *cy, *cz, *r;
+  int K;
+  bool targets;
+  int *cnt;
+  const int64_t *off;
+  int *out, *pairPatch;
+};
+
+__global__ __launch_bounds__(kThreads) void member_kernel(MemberArgs m,
+                                                          bool fill) {
+  const int k = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (k >= m.K)
+    return;
+  const int lane = threadIdx.x & 63;
+  const double c[3] = {m.cx[k], m.cy[k], m.cz[k]};
+  const double rad = m.r[k], radMember = rad - kEps;
+  const Ranges g = neighbour_ranges(m.P, m.f, c, depth_for(m.f, rad));
+  int64_t w = fill ? m.off[k] : 0;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  scan_ranges(g, [&](int p, bool valid) {
+    double dd = 0.0;
+    if (valid)
+      dd = dist3(c[0], c[1], c[2], m.P.x[p], m.P.y[p], m.P.z[p]);
+    const bool hit = valid && dd <= rad;
+    const uint64_t bal = __ballot(hit);
+    if (fill && hit) {
+      const int64_t o = w + __popcll(bal & below);
+      if (m.targets) {
+        const unsigned member = dd <= radMember ? 1u : 0u;
+        m.out[o] = (int)((unsigned)p | (member << 31));
+        m.pairPatch[o] = k;
+      } else {
+        m.out[o] = p;
+      }
+    }
+    w += __popcll(bal);
+  });
+  if (!fill && lane == 0)
+    m.cnt[k] = (int)w;
+}
+
+unsigned blocks(int64_t n, int per) {
+  return (unsigned)std::max<int64_t>(1, (n + per - 1) / per);
+}
+
+} // namespace
+
+void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
+                    int64_t N, const double *dTgt, int64_t M,
+                    AdaptivePatches &out, hipStream_t s) {
+  if (ap.leaf < 1 || ap.nmin < 1 || ap.nmax < ap.nmin)
+    throw std::invalid_argument(
+        "adaptive patches need leafPoints >= 1 and 1 <= minPoints <= maxPoints");
+  if (!(ap.overlap > 0.0 && ap.overlap < 1.0))
+    throw std::invalid_argument("relativeOverlap must lie in (0, 1)");
+  if (!(ap.supportScale > 0.0))
+    throw std::invalid_argument("supportScale must be positive");
+  // 1. frame: box of both point sets
+  double slo[3], shi[3], tlo[3], thi[3];
+  bbox_device(dSrc, N, dim, slo, shi, s);
+  bbox_device(dTgt, M, dim, tlo, thi, s);
+  Frame f{};
+  f.D = dim;
+  f.DB = kBits * dim;
+  for (int a = 0; a < 3; ++a) {
+    f.lo[a] = 0.0;
+    f.ext[a] = 1.0;
+  }
+  for (int a = 0; a < dim; ++a) {
+    const double lo = M > 0 ? std::min(slo[a], tlo[a]) : slo[a];
+    const double hi = M > 0 ? std::max(shi[a], thi[a]) : shi[a];
+    f.lo[a] = lo;
+    f.ext[a] = hi - lo;
+    if (!(f.ext[a] > 0.0))
+      f.ext[a] = 1.0;
+    out.lo[a] = lo;
+    out.hi[a] = hi;
+  }
+  for (int d = 0; d <= f.DB; ++d) {
+    double h = DBL_MAX;
+    for (int a = 0; a < dim; ++a)
+      h = std::min(h, f.ext[a] * std::ldexp(1.0, -axis_bits(d, dim, a)));
+    f.hmin[d] = h;
+  }
+  const int bits = f.DB;  // 16, 32 or 48: an even number of 8-bit passes
+  // sorted codes and Morton-ordered SoA coordinates of both sets
+  auto sort_set = [&](const double *pts, int64_t n, DevBuf<uint64_t> &code,
+                      DeviceGrid &g) {
+    const int64_t n1 = std::max<int64_t>(n, 1);
+    code.alloc(n1);
+    DevBuf<uint64_t> k1(n1);
+    DevBuf<int> v1(n1);
+    g.perm.alloc(n1);
+    g.x.alloc(n1);
+    g.y.alloc(n1);
+    g.z.alloc(n1);
+    if (n > 0) {
+      morton_codes_kernel<<<blocks(n, kThreads), kThreads, 0, s>>>(
+          pts, n, dim, f, code.get(), g.perm.get());
+      PACMAN_HIP_CHECK(hipGetLastError());
+      radix_sort_pairs(code.get(), g.perm.get(), k1.get(), v1.get(), n, bits,
+                       s);
+      gather_sorted_kernel<<<blocks(n, kThreads), kThreads, 0, s>>>(
+          pts, n, dim, g.perm.get(), g.x.get(), g.y.get(), g.z.get());
+      PACMAN_HIP_CHECK(hipGetLastError());
+    }
+    g.view = GridView{};
+    g.view.n = (int)n;
+    g.view.x = g.x.get();
+    g.view.y = g.y.get();
+    g.view.z = g.z.get();
+    g.view.perm = g.perm.get();
+    g.view.cell_start = nullptr;
+    g.ncell = 0;
+  };
+  DevBuf<uint64_t> scode, tcode;
+  sort_set(dSrc, N, scode, out.src);
+  sort_set(dTgt, M, tcode, out.tgt);
+  const SortedSet S{scode.get(), out.src.x.get(), out.src.y.get(),
+                    out.src.z.get(), (int)N};
+  const SortedSet T{tcode.get(), out.tgt.x.get(), out.tgt.y.get(),
+                    out.tgt.z.get(), (int)M};
+
+  // 2. the tree, one launch per depth
+  const int64_t maxLeaf = std::max<int64_t>(M, 1);
+  DevBuf<uint64_t> leafP(maxLeaf);
+  DevBuf<int> leafD(maxLeaf), ctr(2);
+  std::vector<TNode> root{{0, 0, 0, (int)N, 0, (int)M, 0}};
+  DevBuf<TNode> level(1);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(level.get(), root.data(), sizeof(TNode),
+                                  hipMemcpyHostToDevice, s));
+  int nLevel = 1, nLeaves = 0;
+  for (int d = 0; d <= f.DB && nLevel > 0; ++d) {
+    const int maxOut = 2 * nLevel;
+    DevBuf<TNode> next(maxOut);
+    int h[2] = {0, nLeaves};
+    PACMAN_HIP_CHECK(
+        hipMemcpyAsync(ctr.get(), h, 8, hipMemcpyHostToDevice, s));
+    tree_level_kernel<<<blocks(nLevel, kThreads), kThreads, 0, s>>>(
+        level.get(), nLevel, next.get(), ctr.get(), maxOut, leafP.get(),
+        leafD.get(), (int)maxLeaf, S, T, ap.leaf, f.DB);
+    PACMAN_HIP_CHECK(hipGetLastError());
+    PACMAN_HIP_CHECK(
+        hipMemcpyAsync(h, ctr.get(), 8, hipMemcpyDeviceToHost, s));
+    PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+    if (h[1] > maxLeaf)
+      throw std::runtime_error("adaptive patches: leaf list overflow");
+    nLevel = h[0];
+    nLeaves = h[1];
+    level = std::move(next);
+  }
+
+  // 3-4. radii; halve patches above nmax until none is
+  const int kq = (int)std::min<int64_t>(ap.nmin, N);
+  const int64_t maxDone = std::max<int64_t>(M, 1);
+  DevBuf<uint64_t> doneKey(maxDone);
+  DevBuf<int> doneD(maxDone);
+  DevBuf<double> doneC(3 * maxDone), doneR(maxDone);
+  DevBuf<uint64_t> curP = std::move(leafP);
+  DevBuf<int> curD = std::move(leafD);
+  int nCur = nLeaves, nDone = 0;
+  while (nCur > 0) {
+    const int maxNext = 2 * nCur;
+    DevBuf<uint64_t> nextP(maxNext);
+    DevBuf<int> nextD(maxNext);
+    int h[2] = {0, nDone};
+    PACMAN_HIP_CHECK(
+        hipMemcpyAsync(ctr.get(), h, 8, hipMemcpyHostToDevice, s));
+    CellEvalArgs e{};
+    e.cellP = curP.get();
+    e.cellD = curD.get();
+    e.count = nCur;
+    e.S = S;
+    e.T = T;
+    e.f = f;
+    e.overlap = ap.overlap;
+    e.kq = kq;
+    e.nmax = ap.nmax;
+    e.nextP = nextP.get();
+    e.nextD = nextD.get();
+    e.maxNext = maxNext;
+    e.doneKey = doneKey.get();
+    e.doneD = doneD.get();
+    e.doneC = doneC.get();
+    e.doneR = doneR.get();
+    e.maxDone = (int)maxDone;
+    e.counters = ctr.get();
+    cell_eval_kernel<<<blocks(nCur, kThreads / 64), kThreads, 0, s>>>(e);
+    PACMAN_HIP_CHECK(hipGetLastError());
+    PACMAN_HIP_CHECK(
+        hipMemcpyAsync(h, ctr.get(), 8, hipMemcpyDeviceToHost, s));
+    PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+    if (h[1] > maxDone)
+      throw std::runtime_error("adaptive patches: patch list overflow");
+    nCur = h[0];
+    nDone = h[1];
+    curP = std::move(nextP);
+    curD = std::move(nextD);
+  }
+
+  // 5. patches in key order
+  const int K = nDone;
+  out.K = K;
+  const int64_t K1 = std::max(K, 1);
+  DevBuf<int> order(K1), ordTmp(K1);
+  DevBuf<uint64_t> keyTmp(K1);
+  {
+    std::vector<int> iota(K1);
+    for (int k = 0; k < K; ++k)
+      iota[k] = k;
+    PACMAN_HIP_CHECK(hipMemcpyAsync(order.get(), iota.data(), 4 * K1,
+                                    hipMemcpyHostToDevice, s));
+    radix_sort_pairs(doneKey.get(), order.get(), keyTmp.get(), ordTmp.get(),
+                     K, bits, s);
+    PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // iota copied
+  }
+  out.cx.alloc(K1);
+  out.cy.alloc(K1);
+  out.cz.alloc(K1);
+  out.r.alloc(K1);
+  out.rinv.alloc(K1);
+  out.dinv.alloc(K1);
+  out.key.alloc(K1);
+  out.depth.alloc(K1);
+  // doneKey is now sorted; the other entries are in append order, reached
+  // through `order`
+  if (K > 0) {
+    patch_arrays_kernel<<<blocks(K, kThreads), kThreads, 0, s>>>(
+        order.get(), K, doneD.get(), doneC.get(), doneR.get(),
+        ap.supportScale, out.cx.get(), out.cy.get(), out.cz.get(),
+        out.r.get(), out.rinv.get(), out.dinv.get(), out.depth.get());
+    PACMAN_HIP_CHECK(hipGetLastError());
+    PACMAN_HIP_CHECK(hipMemcpyAsync(out.key.get(), doneKey.get(), 8 * K,
+                                    hipMemcpyDeviceToDevice, s));
+  }
+
+  // membership
+  MemberArgs m{};
+  m.f = f;
+  m.cx = out.cx.get();
+  m.cy = out.cy.get();
+  m.cz = out.cz.get();
+  m.r = out.r.get();
+  m.K = K;
+  DevBuf<int> cnt(K1);
+  const unsigned wb = blocks(K, kThreads / 64);
+  // sources
+  m.P = S;
+  m.targets = false;
+  m.cnt = cnt.get();
+  out.srcOff.alloc(K + 1);
+  if (K > 0) {
+    member_kernel<<<wb, kThreads, 0, s>>>(m, false);
+    PACMAN_HIP_CHECK(hipGetLastError());
+  }
+  const int64_t tot = exclusive_scan_i32_to_i64(cnt.get(), out.srcOff.get(), K, s);
+  if (tot > INT_MAX)
+    throw std::runtime_error("patch source lists exceed 2^31 entries");
+  out.srcIdx.alloc(std::max<int64_t>(tot, 1));
+  m.off = out.srcOff.get();
+  m.out = out.srcIdx.get();
+  if (K > 0) {
+    member_kernel<<<wb, kThreads, 0, s>>>(m, true);
+    PACMAN_HIP_CHECK(hipGetLastError());
+  }
+  // targets
+  m.P = T;
+  m.targets = true;
+  PairSet &ps = out.pairs;
+  ps.off.alloc(K + 1);
+  if (K > 0) {
+    member_kernel<<<wb, kThreads, 0, s>>>(m, false);
+    PACMAN_HIP_CHECK(hipGetLastError());
+  }
+  const int64_t P = exclusive_scan_i32_to_i64(cnt.get(), ps.off.get(), K, s);
+  if (P > INT_MAX)
+    throw std::runtime_error("patch/target pairs exceed 2^31 entries");
+  ps.P = P;
+  ps.tgt.alloc(std::max<int64_t>(P, 1));
+  DevBuf<int> pairPatch(std::max<int64_t>(P, 1));
+  m.off = ps.off.get();
+  m.out = ps.tgt.get();
+  m.pairPatch = pairPatch.get();
+  if (K > 0) {
+    member_kernel<<<wb, kThreads, 0, s>>>(m, true);
+    PACMAN_HIP_CHECK(hipGetLastError());
+  }
+  build_target_slots(ps, pairPatch.get(), M, s);
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // temporaries in use above
+}
+
+} // namespace pacman_amd

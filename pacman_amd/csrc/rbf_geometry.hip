@@ -1018,16 +1018,27 @@ void patch_targets(const GridView &tgt, const double *cx, const double *cy,
   ps.P = P;
   const int64_t P1 = std::max<int64_t>(P, 1);
   ps.tgt.alloc(P1);
-  ps.slot.alloc(P1);
-  ps.slotPatch.alloc(P1);
-  ps.slotMember.alloc(P1);
-  DevBuf<int> pairPatch(P1), tlist(P1), tpatch(P1);
+  DevBuf<int> pairPatch(P1);
   if (K > 0) {
     member_compact_kernel<<<wb, kBlock, 0, s>>>(
         tgt, cx, cy, cz, K, r, r - kEps, true, kMemberCap, scratch.get(),
         cnt.get(), ps.off.get(), ps.tgt.get(), pairPatch.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
+  build_target_slots(ps, pairPatch.get(), M, s);
+  // temporaries are released while the kernels may still run (scan_device)
+}
+
+// Per-target transpose of the pairs (ps.off, ps.tgt, ps.P set; pairPatch[q]
+// = patch of pair q): slots in ascending patch order.
+void build_target_slots(PairSet &ps, const int *pairPatch, int64_t M,
+                        hipStream_t s) {
+  const int64_t P = ps.P;
+  const int64_t P1 = std::max<int64_t>(P, 1);
+  ps.slot.alloc(P1);
+  ps.slotPatch.alloc(P1);
+  ps.slotMember.alloc(P1);
+  DevBuf<int> tlist(P1), tpatch(P1);
   DevBuf<int> tcnt(std::max<int64_t>(M, 1)), tfill(std::max<int64_t>(M, 1));
   fill_i32(tcnt.get(), M, 0, s);
   fill_i32(tfill.get(), M, 0, s);
@@ -1040,7 +1051,7 @@ void patch_targets(const GridView &tgt, const double *cx, const double *cy,
   exclusive_scan_i32_to_i64(tcnt.get(), ps.toff.get(), M, s);
   if (P > 0) {
     pair_scatter_kernel<<<blocks_for(P, kBlock), kBlock, 0, s>>>(
-        ps.tgt.get(), pairPatch.get(), P, ps.toff.get(), tfill.get(),
+        ps.tgt.get(), pairPatch, P, ps.toff.get(), tfill.get(),
         tlist.get(), tpatch.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }

@@ -136,6 +136,33 @@ struct PairSet {
 void patch_targets(const GridView &tgt, const double *cx, const double *cy,
                    const double *cz, int64_t K, double r, PairSet &ps,
                    hipStream_t s);
+// The per-target slots of a pair set whose off, tgt and P are filled
+// (pairPatch[q] = patch of pair q).
+void build_target_slots(PairSet &ps, const int *pairPatch, int64_t M,
+                        hipStream_t s);
+
+// ---- adaptive patches (rbf_adaptive.hip) ----------------------------------
+struct AdaptiveParams {
+  int leaf = 40;             // sources per tree leaf before it is halved
+  int nmin = 20;             // sources every patch holds at least
+  int nmax = 600;            // patches above this are halved again
+  double overlap = 0.15;     // patch radius = (1 + overlap) half-diagonal
+  double supportScale = 1.0; // Wendland support = supportScale * radius
+};
+struct AdaptivePatches {
+  DeviceGrid src, tgt;       // Morton-ordered SoA point sets (no cells)
+  DevBuf<double> cx, cy, cz, r, rinv, dinv;
+  DevBuf<int64_t> key;       // cell key (prefix << (bits - depth))
+  DevBuf<int> depth;
+  DevBuf<int64_t> srcOff;
+  DevBuf<int> srcIdx;        // positions in src
+  PairSet pairs;             // positions in tgt
+  int64_t K = 0;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
+                    int64_t N, const double *dTgt, int64_t M,
+                    AdaptivePatches &out, hipStream_t s);
 
 // ---- solve (rbf_solve.hip) ------------------------------------------------
 struct SolveArgs {
@@ -168,8 +195,10 @@ struct SolveArgs {
   // >= 0, field f's upstream value is the Shepard weight at pair opRow0 + f
   // of each patch and 0 at its other pairs (gSorted is not read).
   int opRow0;
-  // Per-patch 1 / support radius (adaptive patches); null: dinv for all.
+  // Per-patch 1 / support radius and 1 / patch radius (adaptive patches);
+  // null: dinv and rinv for all.
   const double *patchDinv;
+  const double *patchRinv;
   // Pivots d <= 1e-12 phi(0) (or not finite) mark a numerically singular
   // patch; each one found adds 1 to *singular (when set).
   double phi0;
@@ -275,6 +304,7 @@ struct BlendArgs {
   int64_t M;
   int F;
   double rinv;
+  const double *patchRinv;  // per-patch 1 / r (adaptive patches) or null
   double *out;              // F x M, original order
 };
 void launch_blend(const BlendArgs &a, hipStream_t s);

@@ -37,7 +37,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
     throw std::invalid_argument("RBF-PUM needs at least one source point");
   if (M < 0)
     throw std::invalid_argument("negative number of target points");
-  if (p.npc <= 0)
+  if (p.npc <= 0 && !p.adaptive)
     throw std::invalid_argument("nodesPerCluster must be positive");
   if (!(p.overlap > 0.0 && p.overlap < 1.0))
     throw std::invalid_argument("relativeOverlap must lie in (0, 1)");
@@ -45,7 +45,90 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
     throw std::invalid_argument("point counts above 2^30 are not supported");
 
   const auto tSetup = std::chrono::steady_clock::now();
-  auto tPhase = tSetup;
+  if (p.adaptive)
+    setup_adaptive(dSrc, dTgt, s);
+  else
+    setup_reference(dSrc, dTgt, s, tSetup);
+  auto tPhase = std::chrono::steady_clock::now();
+
+  // Size bins by register tile.
+  std::vector<int64_t> off(K_ + 1);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(off.data(), srcOff_.get(), 8 * (K_ + 1),
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  maxN_ = 0;
+  for (int64_t kk = 0; kk < K_; ++kk)
+    maxN_ = std::max(maxN_, (int)(off[kk + 1] - off[kk]));
+  patchN_.resize(K_);
+  for (int64_t kk = 0; kk < K_; ++kk)
+    patchN_[kk] = (int)(off[kk + 1] - off[kk]);
+  totalSrc_ = off[K_];
+  counters_.alloc(kWsCounters);
+  launch_ws_reset(counters_.get(), true, s);
+  plan_workspace(s);
+  bins_.resize(solve_fields_per_launch());
+  bins_for(1, s);
+  pv_.alloc(std::max<int64_t>(K_, 1));
+  status_.alloc(std::max<int64_t>(K_, 1));
+  for (auto &e : ev_)
+    PACMAN_HIP_CHECK(hipEventCreate(&e));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  setupMs_[3] += ms_since(tPhase);
+  setupMs_[4] = ms_since(tSetup);
+}
+
+// Density-aware patches (rbf_adaptive.hip): phases 0 (point sets), 2
+// (patches) and 3 (membership) share one host-timed interval, reported as
+// phase 2; radius and lattice phases are zero.
+void RbfPumPlan::setup_adaptive(const double *dSrc, const double *dTgt,
+                                hipStream_t s) {
+  const auto t0 = std::chrono::steady_clock::now();
+  AdaptivePatches ad;
+  build_adaptive(prm_.ap, prm_.dim, dSrc, N_, dTgt, M_, ad, s);
+  for (int a = 0; a < 3; ++a) {
+    lo_[a] = ad.lo[a];
+    hi_[a] = ad.hi[a];
+    tlo_[a] = ad.lo[a];
+    thi_[a] = ad.hi[a];
+  }
+  gs_ = std::move(ad.src);
+  gt_ = std::move(ad.tgt);
+  cx_ = std::move(ad.cx);
+  cy_ = std::move(ad.cy);
+  cz_ = std::move(ad.cz);
+  cand_ = std::move(ad.key);
+  patchR_ = std::move(ad.r);
+  patchRinv_ = std::move(ad.rinv);
+  patchDinv_ = std::move(ad.dinv);
+  srcOff_ = std::move(ad.srcOff);
+  srcIdx_ = std::move(ad.srcIdx);
+  pairs_ = std::move(ad.pairs);
+  K_ = ad.K;
+  lat_ = Lattice{};
+  lat_.dim = prm_.dim;
+  lat_.nb = K_;
+  for (int a = 0; a < 3; ++a)
+    lat_.shape[a] = 0;
+  // reported radius and support: the largest patch's
+  std::vector<double> r(std::max<int64_t>(K_, 1), 0.0);
+  if (K_ > 0)
+    PACMAN_HIP_CHECK(hipMemcpyAsync(r.data(), patchR_.get(), 8 * K_,
+                                    hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  r_ = *std::max_element(r.begin(), r.end());
+  delta_ = prm_.ap.supportScale * r_;
+  setupMs_[2] = ms_since(t0);
+}
+
+// The reference's patch construction: FindRadius, CreateClusters and the
+// membership queries on the uniform cell grids.
+void RbfPumPlan::setup_reference(const double *dSrc, const double *dTgt,
+                                 hipStream_t s,
+                                 std::chrono::steady_clock::time_point t0) {
+  const PlanParams &p = prm_;
+  const int D = p.dim;
+  const int64_t N = N_, M = M_;
+  auto tPhase = t0;
   // Bounding boxes (the source box is the BVH bounds of the reference).
   bbox_device(dSrc, N, D, lo_, hi_, s);
   bbox_device(dTgt, M, D, tlo_, thi_, s);
@@ -141,31 +224,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   patch_sources(gs_.view, cx_.get(), cy_.get(), cz_.get(), K_, r_, srcOff_,
                 srcIdx_, s);
   patch_targets(gt_.view, cx_.get(), cy_.get(), cz_.get(), K_, r_, pairs_, s);
-
-  // Size bins by register tile.
-  std::vector<int64_t> off(K_ + 1);
-  PACMAN_HIP_CHECK(hipMemcpyAsync(off.data(), srcOff_.get(), 8 * (K_ + 1),
-                                  hipMemcpyDeviceToHost, s));
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
-  maxN_ = 0;
-  for (int64_t kk = 0; kk < K_; ++kk)
-    maxN_ = std::max(maxN_, (int)(off[kk + 1] - off[kk]));
-  patchN_.resize(K_);
-  for (int64_t kk = 0; kk < K_; ++kk)
-    patchN_[kk] = (int)(off[kk + 1] - off[kk]);
-  totalSrc_ = off[K_];
-  counters_.alloc(kWsCounters);
-  launch_ws_reset(counters_.get(), true, s);
-  plan_workspace(s);
-  bins_.resize(solve_fields_per_launch());
-  bins_for(1, s);
-  pv_.alloc(std::max<int64_t>(K_, 1));
-  status_.alloc(std::max<int64_t>(K_, 1));
-  for (auto &e : ev_)
-    PACMAN_HIP_CHECK(hipEventCreate(&e));
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   setupMs_[3] = ms_since(tPhase);
-  setupMs_[4] = ms_since(tSetup);
 }
 
 // Patches grouped by tile blocks R = solve_bin_rows(n, F), one launch each.
@@ -384,6 +443,8 @@ SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
   a.opRow0 = -1;
   a.phi0 = kPhi0[prm_.kind];
   a.singular = counters_.get() + kWsSingular;
+  a.patchDinv = patchDinv_.n ? patchDinv_.get() : nullptr;
+  a.patchRinv = patchRinv_.n ? patchRinv_.get() : nullptr;
   return a;
 }
 
@@ -405,6 +466,7 @@ BlendArgs RbfPumPlan::blendArgs(int F) const {
   bl.M = M_;
   bl.F = F;
   bl.rinv = 1.0 / r_;  // interpolator.hpp:91
+  bl.patchRinv = patchRinv_.n ? patchRinv_.get() : nullptr;
   return bl;
 }
 
@@ -483,6 +545,9 @@ void RbfPumPlan::applyState(const double *dVals, int F, double *dOut,
 void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
                                  double *dCoordGrad, hipStream_t s) {
   require_register_patches("the target-coordinate gradient");
+  if (prm_.adaptive)
+    throw std::runtime_error("the target-coordinate gradient is available "
+                             "for plans with one patch radius (not adaptive)");
   if (F < 1)
     throw std::invalid_argument("target gradient needs at least one field");
   if (M_ == 0)
@@ -859,6 +924,19 @@ void RbfPumPlan::applyCounters(int *out, hipStream_t s) const {
   out[0] = c[kWsSingular];
   out[1] = c[kWsIrIters];
   out[2] = c[kWsFallbackTotal];
+}
+
+void RbfPumPlan::patchRadii(double *r, hipStream_t s) const {
+  if (K_ == 0)
+    return;
+  if (!patchR_.n) {
+    for (int64_t k = 0; k < K_; ++k)
+      r[k] = r_;
+    return;
+  }
+  PACMAN_HIP_CHECK(hipMemcpyAsync(r, patchR_.get(), 8 * K_,
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void RbfPumPlan::require_register_patches(const char *what) const {

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <chrono>
 #include <vector>
 
 #include "rbf_internal.hpp"
@@ -23,6 +24,10 @@ struct PlanParams {
   // refinement, FP64 fallback per patch).
   int precision = 0;
   int maxIR = 6;           // refinement iteration limit (mixed)
+  // Patches from the density-aware builder (rbf_adaptive.hip) instead of the
+  // reference's FindRadius / CreateClusters (npc is then unused).
+  bool adaptive = false;
+  AdaptiveParams ap;
 };
 
 class RbfPumPlan {
@@ -76,6 +81,10 @@ public:
   // patches, 1 refinement iterations (summed over patches), 2 patches that
   // fell back from FP32 refinement to FP64.
   void applyCounters(int *out, hipStream_t s) const;
+  bool adaptive() const { return prm_.adaptive; }
+  // Per-patch radii (host copy, K entries; adaptive plans: each patch's
+  // own, otherwise the one radius).
+  void patchRadii(double *r, hipStream_t s) const;
 
   // Host copies for inspection and parity tests (original point indices,
   // each list sorted ascending).
@@ -123,6 +132,10 @@ private:
   int wsNbMax_ = 0, wsSlots_ = 0;
   DevBuf<double> wsWork_, wsVec_;  // wsWork_ also holds FP32 blocks
   void plan_workspace(hipStream_t s);
+  void setup_reference(const double *dSrc, const double *dTgt, hipStream_t s,
+                       std::chrono::steady_clock::time_point t0);
+  void setup_adaptive(const double *dSrc, const double *dTgt, hipStream_t s);
+  DevBuf<double> patchR_, patchRinv_, patchDinv_;  // adaptive plans only
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, hipStream_t s);
   std::vector<int> patchN_;  // sources per patch (host)

@@ -39,6 +39,11 @@ namespace {
 
 using namespace solve_detail;
 
+// 1 / PU radius of patch k: per patch (adaptive patches) or one for all.
+template <class A> __device__ __forceinline__ double rinv_of(const A &a, int k) {
+  return a.patchRinv ? a.patchRinv[k] : a.rinv;
+}
+
 // ---------------------------------------------------------------------------
 // Polynomial fit
 // ---------------------------------------------------------------------------
@@ -311,7 +316,7 @@ __global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
   const int n = (int)(a.srcOff[patch + 1] - s0);
   const int F = a.F;
   const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
-  const double sc = a.rinv;
+  const double sc = a.patchRinv ? a.patchRinv[patch] : a.rinv;
   int pv, ca[4];
   double L[4][4];
   poly_basis(a, patch, lane, c, sc, pv, ca, L);
@@ -399,14 +404,14 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
       const int k = a.slotPatch[s0 + j];
       // DistanceNoCheck(target, center), solver.hpp:158-161
       const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-      ph[j] = wendland<1, double, true>(d, a.rinv);
+      ph[j] = wendland<1, double, true>(d, rinv_of(a, k));
       sum += ph[j];
     }
   }
   for (int64_t s = s0 + kSlots; s < s1; ++s) {
     const int k = a.slotPatch[s];
     const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-    sum += wendland<1, double, true>(d, a.rinv);
+    sum += wendland<1, double, true>(d, rinv_of(a, k));
   }
   const int64_t o = a.tperm[t];
   constexpr int kB = 8;
@@ -430,7 +435,7 @@ __global__ __launch_bounds__(256) void blend_kernel(BlendArgs a) {
         continue;
       const int k = a.slotPatch[s];
       const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-      const double w = wendland<1, double, true>(d, a.rinv) / sum;
+      const double w = wendland<1, double, true>(d, rinv_of(a, k)) / sum;
 #pragma unroll
       for (int f = 0; f < kB; ++f)
         if (f0 + f < a.F)
@@ -508,7 +513,7 @@ __global__ __launch_bounds__(256) void adjoint_poly_kernel(
   const int mall = (int)(a.pairOff[patch + 1] - p0);
   const int F = a.F;
   const double c[3] = {a.cx[patch], a.cy[patch], a.cz[patch]};
-  const double sc = a.rinv;
+  const double sc = a.patchRinv ? a.patchRinv[patch] : a.rinv;
   int pv, ca[4];
   double L[4][4];
   poly_basis(a, patch, lane, c, sc, pv, ca, L);
@@ -608,14 +613,14 @@ __global__ __launch_bounds__(256) void slot_weights_kernel(BlendArgs a,
     if (s0 + j < s1) {
       const int k = a.slotPatch[s0 + j];
       const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-      ph[j] = wendland<1, double, true>(d, a.rinv);
+      ph[j] = wendland<1, double, true>(d, rinv_of(a, k));
       sum += ph[j];
     }
   }
   for (int64_t s = s0 + kSlots; s < s1; ++s) {
     const int k = a.slotPatch[s];
     const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-    sum += wendland<1, double, true>(d, a.rinv);
+    sum += wendland<1, double, true>(d, rinv_of(a, k));
   }
 #pragma unroll
   for (int j = 0; j < kSlots; ++j)
@@ -624,7 +629,7 @@ __global__ __launch_bounds__(256) void slot_weights_kernel(BlendArgs a,
   for (int64_t s = s0 + kSlots; s < s1; ++s) {
     const int k = a.slotPatch[s];
     const double d = dist_fast(yx, yy, yz, a.cx[k], a.cy[k], a.cz[k]);
-    wslot[s] = a.slotMember[s] ? wendland<1, double, true>(d, a.rinv) / sum
+    wslot[s] = a.slotMember[s] ? wendland<1, double, true>(d, rinv_of(a, k)) / sum
                                : 0.0;
   }
 }

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane >> 4, lc = lane & 15;
   const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
-  const double dinv = a.dinv;
+  const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
 
   if (!ADJ && tid < kFMax * 4)
     scoef[tid >> 2][tid & 3] =

@@ -164,7 +164,7 @@ class Plan:
                  target_points, nodes_per_cluster=DEFAULT_NODES_PER_CLUSTER,
                  relative_overlap=DEFAULT_RELATIVE_OVERLAP,
                  support_radius=DEFAULT_SUPPORT_RADIUS, stream=None,
-                 precision="fp64", max_refinements=6):
+                 precision="fp64", max_refinements=6, _adaptive=None):
         self._lib = _lib.load()
         self._h = ctypes.c_void_p()
         self.dim = int(space_dimension)
@@ -193,11 +193,19 @@ class Plan:
         if dev != devt:
             raise ValueError("source and target points must both be on the "
                              "host or both on the device")
-        rc = self._lib.pacman_rbf_plan_create(
-            ctypes.byref(self._h), self.dim, int(rbf_function),
-            int(nodes_per_cluster), float(relative_overlap),
-            float(support_radius), ps, self.num_sources, pt,
-            self.num_targets, dev, ctypes.c_void_p(stream or 0))
+        if _adaptive is None:
+            rc = self._lib.pacman_rbf_plan_create(
+                ctypes.byref(self._h), self.dim, int(rbf_function),
+                int(nodes_per_cluster), float(relative_overlap),
+                float(support_radius), ps, self.num_sources, pt,
+                self.num_targets, dev, ctypes.c_void_p(stream or 0))
+        else:
+            leaf, nmin, nmax, scale = _adaptive
+            rc = self._lib.pacman_rbf_plan_create_adaptive(
+                ctypes.byref(self._h), self.dim, int(rbf_function), int(leaf),
+                int(nmin), int(nmax), float(relative_overlap), float(scale),
+                ps, self.num_sources, pt, self.num_targets, dev,
+                ctypes.c_void_p(stream or 0))
         if rc != 0:
             raise RuntimeError(_lib.last_error())
         self.on_device = bool(dev)
@@ -205,6 +213,31 @@ class Plan:
             self.set_precision(precision, max_refinements)
 
     _PRECISIONS = {"fp64": 0, "mixed": 1}
+
+    @classmethod
+    def adaptive(cls, space_dimension, rbf_function, source_points,
+                 target_points, leaf_points=40, min_points=20, max_points=600,
+                 relative_overlap=DEFAULT_RELATIVE_OVERLAP, support_scale=1.0,
+                 stream=None, precision="fp64", max_refinements=6):
+        """Plan on density-aware patches (no reference counterpart): a
+        binary tree over Morton codes halves cells holding more than
+        leaf_points sources; every leaf with targets is a patch centred on
+        its cell with radius max((1 + overlap) half-diagonal, distance to
+        the min_points-th nearest source); patches above max_points sources
+        are halved again; support = support_scale * radius."""
+        return cls(space_dimension, rbf_function, source_points,
+                   target_points, 1, relative_overlap, -1.0, stream,
+                   precision, max_refinements,
+                   _adaptive=(leaf_points, min_points, max_points,
+                              support_scale))
+
+    def patch_radii(self):
+        r = np.zeros(self.num_patches)
+        rc = self._lib.pacman_rbf_plan_patch_radii(
+            self._h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return r
 
     def set_precision(self, precision="fp64", max_refinements=6):
         """Arithmetic of the patches above 192 points (the workspace solve):

@@ -73,3 +73,40 @@ def config(name: str):
     if name == "c5":      # 8M jittered -> 8M uniform, npc 150
         return jittered_grid(200, 9), uniform(8_000_000, 10), 150, -1.0
     raise KeyError(name)
+
+
+def gaussian_mixture(n: int, seed: int, components: int = 12,
+                     sigma=(0.02, 0.15)) -> np.ndarray:
+    """n points from a Gaussian mixture in [0, 1]^3: means U[0, 1]^3,
+    isotropic sigma ~ U[sigma], weights ~ Dirichlet(1); points outside the
+    cube are drawn again (component and position) until inside.
+    Draw order: means, sigmas, weights, then rounds of (components,
+    normals) for the points still outstanding."""
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(0.0, 1.0, (components, 3))
+    sig = rng.uniform(sigma[0], sigma[1], components)
+    w = rng.dirichlet(np.ones(components))
+    out = np.empty((n, 3))
+    todo = np.arange(n)
+    while len(todo):
+        c = rng.choice(components, size=len(todo), p=w)
+        p = means[c] + sig[c][:, None] * rng.normal(size=(len(todo), 3))
+        ok = np.all((p >= 0.0) & (p <= 1.0), axis=1)
+        out[todo[ok]] = p[ok]
+        todo = todo[~ok]
+    return np.ascontiguousarray(out)
+
+
+def mixture_fields(P: np.ndarray, nfields: int = 8, seed: int = 12):
+    """franke_3d followed by nfields - 1 seeded sin(k . x + phi) fields."""
+    F = np.empty((nfields, len(P)))
+    F[0] = franke_3d(*P.T)
+    if nfields > 1:
+        F[1:] = trig_fields(P, nfields - 1, seed)
+    return F
+
+
+# The uneven-cloud workload: 4.1M mixture sources (seed 4) -> 4.1M uniform
+# targets (seed 5); patches from the adaptive builder.
+def mixture_config(n: int = 4_096_000):
+    return gaussian_mixture(n, 4), uniform(n, 5)

@@ -1,0 +1,116 @@
+"""Density-aware patches (Plan.adaptive) on the GPU against the oracle.
+
+The oracle's builder (oracle.AdaptiveSetup) is pinned by a numpy brute-force
+restatement in test_oracle.py. Here the GPU builder must reproduce it
+bit for bit: patch centres, radii and cell keys (`centers()` returns the
+cell key as the candidate index), and the three CSR lists as sets. The
+transfer is compared with the oracle's at rel-L2 1e-10, in FP64 and in mixed
+precision (FP32 workspace factors refined in FP64; patches of up to 192
+points are FP64 either way). Support = the patch radius: the local matrices
+of these clouds have cond(A) <= ~1e4, so FP64 round-off stays near 1e-13.
+"""
+
+import numpy as np
+import pytest
+
+import pacman_amd as pacman
+from pacman_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _clouds():
+    rng = np.random.default_rng(21)
+    return {
+        "3d-mixture": (W.gaussian_mixture(40000, 4), W.uniform(30000, 5),
+                       40, 20, 600),
+        "3d-cap": (W.gaussian_mixture(30000, 9), W.uniform(20000, 8), 60,
+                   15, 250),
+        "2d": (rng.normal(0.5, 0.12, (20000, 2)),
+               rng.uniform(0, 1, (15000, 2)), 30, 12, 400),
+        "1d": (rng.uniform(0, 1, (5000, 1)) ** 3,
+               rng.uniform(0, 1, (3000, 1)), 10, 6, 60),
+    }
+
+
+CLOUDS = _clouds()
+
+
+@pytest.mark.parametrize("name", sorted(CLOUDS))
+def test_adaptive_setup_bit_exact(name, oracle_mod):
+    S, T, leaf, nmin, nmax = CLOUDS[name]
+    dim = S.shape[1]
+    o = oracle_mod.AdaptiveSetup(dim, 0x11, S, T, leaf, nmin, nmax)
+    plan = pacman.rbf.Plan.adaptive(dim, 0x11, S, T, leaf, nmin, nmax)
+    assert plan.num_patches == o.num_patches
+    C, key = plan.centers()
+    oC, okey = o.centers()
+    np.testing.assert_array_equal(key, okey)
+    np.testing.assert_array_equal(C, oC)
+    r = plan.patch_radii()
+    orad, _, _ = o.patches()
+    np.testing.assert_array_equal(r, orad)
+    for which in (0, 1, 2):
+        a, b = plan.csr(which), o.csr(which)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+    n = np.diff(plan.csr(0)[0])
+    print(f"{name}: K={plan.num_patches} n in [{n.min()}, {n.max()}] "
+          f"large={plan.num_large_patches} radii [{r.min():.3g}, "
+          f"{r.max():.3g}]")
+
+
+@pytest.mark.parametrize("name,rbf", [("3d-mixture", 0x11),
+                                      ("3d-mixture", 0x10),
+                                      ("3d-cap", 0x12), ("2d", 0x13),
+                                      ("1d", 0x14)])
+def test_adaptive_transfer_matches_oracle(name, rbf, oracle_mod):
+    S, T, leaf, nmin, nmax = CLOUDS[name]
+    dim = S.shape[1]
+    V = np.stack([np.sin(3 * S[:, 0] + 1.0), np.cos(2 * S.sum(1))])
+    ref = oracle_mod.AdaptiveSetup(dim, rbf, S, T, leaf, nmin, nmax).apply(V)
+    plan = pacman.rbf.Plan.adaptive(dim, rbf, S, T, leaf, nmin, nmax)
+    got = plan.apply(V)
+    err = rel_l2(got, ref)
+    plan.set_precision("mixed")
+    mixed = plan.apply(V)
+    c = plan.counters()
+    errm = rel_l2(mixed, ref)
+    print(f"{name} rbf={hex(rbf)}: fp64 rel-L2 vs oracle {err:.2e}, mixed "
+          f"{errm:.2e} (tolerance 1e-10), large={plan.num_large_patches}, "
+          f"counters {c}")
+    assert err < 1e-10 and errm < 1e-10
+    assert c["singular"] == 0
+
+
+def test_adaptive_covers_every_target_and_reproduces_linear():
+    S, T, leaf, nmin, nmax = CLOUDS["3d-mixture"]
+    plan = pacman.rbf.Plan.adaptive(3, 0x11, S, T, leaf, nmin, nmax)
+    woff, _ = plan.csr(2)
+    assert np.all(np.diff(woff) > 0)
+    g = np.array([0.7, -1.3, 2.1])
+    out = plan.apply(0.25 + S @ g)
+    err = np.abs(out - (0.25 + T @ g)).max()
+    print(f"linear field max abs error {err:.2e}")
+    assert err < 1e-10
+
+
+def test_adaptive_support_scale_and_device_tensors(oracle_mod):
+    import torch
+    S, T, leaf, nmin, nmax = CLOUDS["3d-cap"]
+    dev = torch.device("cuda:0")
+    f = W.franke_3d(*S.T)
+    ref = oracle_mod.AdaptiveSetup(3, 0x11, S, T, leaf, nmin, nmax, 0.15,
+                                   0.7).apply(f)
+    plan = pacman.rbf.Plan.adaptive(3, 0x11, torch.from_numpy(S).to(dev),
+                                    torch.from_numpy(T).to(dev), leaf, nmin,
+                                    nmax, support_scale=0.7)
+    got = plan.apply(torch.from_numpy(f).to(dev)).cpu().numpy()
+    err = rel_l2(got, ref)
+    print(f"support scale 0.7: rel-L2 vs oracle {err:.2e}")
+    assert err < 1e-10
+    assert plan.support == pytest.approx(0.7 * plan.radius)

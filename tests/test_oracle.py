@@ -643,3 +643,192 @@ def test_mls_reference_same_grid(oracle_mod, dim):
     # neighbourhoods
     _, ref = _mls_numpy(G, G[::7], f, dim)
     assert np.abs(got[::7] - ref).max() < 1e-10
+
+
+# ---- adaptive patches (density-aware builder; not in the reference) --------
+
+def _quantize(x, lo, ext):
+    t = ((x - lo) / ext) * 65536.0
+    q = np.where(t > 0.0, t, 0.0)
+    q = np.where(q >= 65535.0, 65535.0, q)
+    return q.astype(np.uint64)      # truncation, as the C cast
+
+
+def _prefix(j, D, d):
+    """Morton prefix of depth d of integer cell coordinates j (n, D)."""
+    p = np.zeros(len(j), np.uint64)
+    for t in range(d):
+        a = t % D
+        ba = (d + D - 1 - a) // D
+        bit = np.uint64(ba - 1 - t // D)
+        p = (p << np.uint64(1)) | ((j[:, a] >> bit) & np.uint64(1))
+    return p
+
+
+def adaptive_numpy(S, T, dim, leaf, nmin, nmax, ov=0.15, variant=None):
+    """The adaptive builder restated by brute force in numpy: Morton codes of
+    16-bit-quantised coordinates over the box of both sets, binary tree over
+    the code bits (split cells with > leaf sources; keep cells with sources
+    or targets; leaves with targets are patches), centre = cell centre,
+    r = max((1 + ov) * half diagonal, distance to the nmin-th nearest
+    source), halve patches holding > nmax sources (children with targets),
+    order by cell. `variant` plants a mistake (tests only)."""
+    D, DB = dim, 16 * dim
+    lo = np.minimum(S.min(0), T.min(0))
+    ext = np.maximum(S.max(0), T.max(0)) - lo
+    ext = np.where(ext > 0, ext, 1.0)
+    qs = np.stack([_quantize(S[:, a], lo[a], ext[a]) for a in range(D)], 1)
+    qt = np.stack([_quantize(T[:, a], lo[a], ext[a]) for a in range(D)], 1)
+    cs, ct = np.sort(_prefix(qs, D, DB)), np.sort(_prefix(qt, D, DB))
+
+    def count(codes, p, d):
+        a, b = p << (DB - d), (p + 1) << (DB - d)
+        return int(np.searchsorted(codes, b) - np.searchsorted(codes, a))
+
+    leaves, stack = [], [(0, 0)]
+    while stack:
+        p, d = stack.pop()
+        if count(cs, p, d) > leaf and d < DB:
+            for h in (0, 1):
+                c = (2 * p + h, d + 1)
+                if count(cs, *c) or count(ct, *c):
+                    stack.append(c)
+        elif count(ct, p, d):
+            leaves.append((p, d))
+    S3 = pad3(S)
+    kq = min(nmin - (1 if variant == "knn_off_by_one" else 0), len(S))
+    done, todo = [], leaves
+    while todo:
+        nxt = []
+        for p, d in todo:
+            cen, s = np.zeros(3), 0.0
+            for a in range(D):
+                ba = (d + D - 1 - a) // D
+                ia = 0
+                for t in range(a, d, D):
+                    ia = (ia << 1) | ((p >> (d - 1 - t)) & 1)
+                h = ext[a] * 2.0 ** -ba
+                cen[a] = lo[a] + (float(ia) + 0.5) * h
+                s = s + h * h
+            grow = 1.0 if variant == "no_overlap" else 1.0 + ov
+            rc = grow * (0.5 * np.sqrt(s))
+            d2 = _sq(cen, S3)
+            dk = np.sqrt(np.partition(d2, kq - 1)[kq - 1])
+            r = max(rc, dk)
+            n = int(np.sum(np.sqrt(d2) <= r))
+            if n > nmax and d < DB and variant != "no_split":
+                nxt += [c for c in ((2 * p, d + 1), (2 * p + 1, d + 1))
+                        if count(ct, *c)]
+            else:
+                done.append(((p << (DB - d)), cen, r))
+        todo = nxt
+    done.sort(key=lambda x: x[0])
+    return (np.array([x[1] for x in done]), np.array([x[2] for x in done]),
+            np.array([x[0] for x in done], np.uint64))
+
+
+def _adaptive_clouds():
+    rng = np.random.default_rng(5)
+    return [
+        ("3d-mixture", W.gaussian_mixture(6000, 4), W.uniform(3000, 5), 40,
+         20, 300),
+        ("3d-dense-cap", W.gaussian_mixture(5000, 9), W.uniform(2000, 8), 60,
+         15, 150),
+        ("2d", rng.normal(0.5, 0.12, (3000, 2)), rng.uniform(0, 1, (1500, 2)),
+         30, 12, 200),
+        ("1d", rng.uniform(0, 1, (800, 1)) ** 3, rng.uniform(0, 1, (400, 1)),
+         10, 6, 60),
+    ]
+
+
+def test_adaptive_builder_bruteforce(oracle_mod):
+    """Centres, radii and the cell order of the oracle's adaptive patches
+    equal the numpy restatement exactly; members and weights by brute
+    force; each planted mistake changes the patches on some cloud."""
+    misses = {"no_overlap": 0, "knn_off_by_one": 0, "no_split": 0}
+    for name, S, T, leaf, nmin, nmax in _adaptive_clouds():
+        dim = S.shape[1]
+        o = oracle_mod.AdaptiveSetup(dim, 0x11, S, T, leaf, nmin, nmax)
+        C, cand = o.centers()
+        r, delta, _ = o.patches()
+        refC, refR, refKey = adaptive_numpy(S, T, dim, leaf, nmin, nmax)
+        np.testing.assert_array_equal(cand.astype(np.uint64), refKey,
+                                      err_msg=name)
+        np.testing.assert_array_equal(C[:, :dim], refC[:, :dim], err_msg=name)
+        np.testing.assert_array_equal(r, refR, err_msg=name)
+        np.testing.assert_array_equal(delta, r)       # support scale 1
+        n = np.diff(o.csr(0)[0])
+        assert n.min() >= min(nmin, len(S)) and n.max() <= nmax, name
+        # members: sources within r_k, targets within r_k - 1e-14
+        for which, P, eps in ((0, S, 0.0), (1, T, 1e-14)):
+            off, idx = o.csr(which)
+            for k in range(0, len(C), max(1, len(C) // 40)):
+                d = np.sqrt(_sq(C[k], pad3(P)))
+                np.testing.assert_array_equal(idx[off[k]:off[k + 1]],
+                                              np.nonzero(d <= r[k] - eps)[0])
+        # every target is covered; weights psi(|y - c_k| / r_k) / sum
+        off, idx = o.csr(2)
+        assert np.all(np.diff(off) > 0), name
+        w = o.weights()
+        for t in range(0, len(T), max(1, len(T) // 60)):
+            ks = idx[off[t]:off[t + 1]]
+            d = np.sqrt(_sq(pad3(T[t:t + 1])[0], pad3(C[ks, :dim])))
+            np.testing.assert_array_equal(ks, np.nonzero(
+                np.sqrt(_sq(pad3(T[t:t + 1])[0], pad3(C[:, :dim])))
+                <= r)[0])
+            v = _psi_c2(d * (1.0 / r[ks]))
+            np.testing.assert_allclose(w[off[t]:off[t + 1]], v / v.sum(),
+                                       rtol=1e-14, atol=1e-16)
+        for v in misses:
+            bc, br, bk = adaptive_numpy(S, T, dim, leaf, nmin, nmax, variant=v)
+            if (len(bk) != len(refKey) or np.any(bk != refKey)
+                    or np.any(br != refR)):
+                misses[v] += 1
+    assert all(m > 0 for m in misses.values()), misses
+
+
+def test_adaptive_morton_code_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(3)
+    for dim in (1, 2, 3):
+        P = rng.uniform(-1, 2, (50, dim))
+        lo, ext = P.min(0), P.max(0) - P.min(0)
+        q = np.stack([_quantize(P[:, a], lo[a], ext[a])
+                      for a in range(dim)], 1)
+        ref = _prefix(q, dim, 16 * dim)
+        got = [oracle_mod.morton_code(p, dim, lo, ext) for p in P]
+        assert [int(x) for x in ref] == got
+
+
+def test_adaptive_operator_matches_dense_numpy(oracle_mod):
+    """The adaptive transfer (per-patch support and PU radius) against a
+    dense numpy evaluation of sum_k w_k [E_k A_k^-1 (b - Q beta) + V beta]
+    on the oracle's own lists, and exact reproduction of linear fields."""
+    S = W.gaussian_mixture(3000, 4)
+    T = W.uniform(1500, 5)
+    o = oracle_mod.AdaptiveSetup(3, 0x13, S, T, 40, 20, 300, 0.15, 0.8)
+    f = W.franke_3d(*S.T)
+    got = o.apply(f)
+    C, _ = o.centers()
+    r, delta, _ = o.patches()
+    soff, sidx = o.csr(0)
+    toff, tidx = o.csr(1)
+    woff, widx = o.csr(2)
+    w = o.weights()
+    phi = np.vectorize(lambda p: oracle_mod.wendland(0x13, p, 1.0))
+    ref = np.zeros(len(T))
+    for k in range(len(C)):
+        src, tgt = sidx[soff[k]:soff[k + 1]], tidx[toff[k]:toff[k + 1]]
+        X, Y = S[src], T[tgt]
+        Q = np.hstack([np.ones((len(X), 1)), X])
+        beta = np.linalg.lstsq(Q, f[src], rcond=None)[0]
+        A = phi(np.sqrt(((X[:, None] - X[None]) ** 2).sum(-1)) / delta[k])
+        alpha = scipy.linalg.solve(A, f[src] - Q @ beta)
+        E = phi(np.sqrt(((Y[:, None] - X[None]) ** 2).sum(-1)) / delta[k])
+        loc = E @ alpha + np.hstack([np.ones((len(Y), 1)), Y]) @ beta
+        for j, t in enumerate(tgt):
+            e = woff[t] + int(np.nonzero(widx[woff[t]:woff[t + 1]] == k)[0][0])
+            ref[t] += w[e] * loc[j]
+    assert rel_l2(got, ref) < 1e-11
+    lin = 1.0 + S @ np.array([0.5, -1.0, 2.0])
+    assert np.abs(o.apply(lin) - (1.0 + T @ np.array([0.5, -1.0, 2.0]))
+                  ).max() < 1e-11
