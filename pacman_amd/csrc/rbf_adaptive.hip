@@ -283,26 +283,86 @@ __device__ inline int depth_for(const Frame &f, double r) {
   return d;
 }
 
-// The code ranges of the 3^D cells around c at depth d: lane l < 3^D holds
-// range l in [rs, re) (empty outside the box).
-struct Ranges {
-  int rs, re, nr;
+__device__ inline double dist3(double ax, double ay, double az, double bx,
+                               double by, double bz) {
+  return sqrt(sqdiff3(ax, ay, az, bx, by, bz));
+}
+
+// First position in [lo, hi) whose code is >= key, by the whole wave: 64
+// probes per round narrow the range 64-fold (a few rounds for millions of
+// points instead of ~22 dependent loads).
+__device__ inline int wave_lower_bound(const uint64_t *code, int lo, int hi,
+                                       uint64_t key) {
+  const int lane = threadIdx.x & 63;
+  while (hi - lo > 64) {
+    const int step = (hi - lo + 63) / 64;
+    const int pos = lo + lane * step;
+    const bool less = pos < hi && code[pos] < key;
+    const int cnt = __popcll(__ballot(less));
+    if (cnt == 0)
+      return lo;
+    const int nlo = lo + (cnt - 1) * step + 1;
+    const int nhi = min(hi, lo + cnt * step);
+    lo = nlo;
+    hi = nhi;
+  }
+  const int pos = lo + lane;
+  const bool less = pos < hi && code[pos] < key;
+  return lo + __popcll(__ballot(less));
+}
+
+// Squared distance from c to the box of the cell (prefix p, depth d), the
+// box widened by 1e-12 of the frame on each side (points are assigned to
+// cells through rounded quantisation).
+__device__ inline double cell_box_dist2(uint64_t p, int d, const Frame &f,
+                                        const double *c) {
+  double s = 0.0;
+  for (int a = 0; a < f.D; ++a) {
+    const int ba = axis_bits(d, f.D, a);
+    uint64_t ia = 0;
+    for (int t = a; t < d; t += f.D)
+      ia = (ia << 1) | ((p >> (d - 1 - t)) & 1u);
+    const double h = f.ext[a] * ldexp(1.0, -ba);
+    const double pad = 1e-12 * (f.ext[a] + fabs(f.lo[a]));
+    const double blo = f.lo[a] + (double)ia * h - pad;
+    const double bhi = f.lo[a] + (double)(ia + 1) * h + pad;
+    const double e = fmax(fmax(blo - c[a], c[a] - bhi), 0.0);
+    s += e * e;
+  }
+  return s;
+}
+
+constexpr int kRefine = 256;     // longer code ranges are halved first
+constexpr int kStack = 128;      // pending ranges per wave
+struct ScanNode {
+  uint64_t p;
+  int d, rs, re;
 };
 
-__device__ inline Ranges neighbour_ranges(const SortedSet &S, const Frame &f,
-                                          const double *c, int d) {
+// fn(position, valid) over a superset of the points of S within r of c (each
+// candidate once; ranges in a fixed order, so the order is deterministic):
+// the 3^D cells around c at the deepest depth whose cells are >= r wide;
+// a range longer than kRefine is halved along the next code bit and the
+// halves whose box lies farther than r from c are dropped.
+template <class Fn>
+__device__ inline void ball_scan(const SortedSet &S, const Frame &f,
+                                 const double *c, double r, ScanNode *stk,
+                                 Fn &&fn) {
   const int lane = threadIdx.x & 63;
-  Ranges g;
-  g.nr = f.D == 1 ? 3 : f.D == 2 ? 9 : 27;
-  g.rs = g.re = 0;
-  if (lane < g.nr) {
+  const int d0 = depth_for(f, r);
+  const int nr = f.D == 1 ? 3 : f.D == 2 ? 9 : 27;
+  const double r2 = r * r * (1.0 + 1e-9) + 1e-300;
+  // the neighbour cells, one lane each
+  int rs = 0, re = 0;
+  uint64_t pref = 0;
+  if (lane < nr) {
     uint64_t j[3] = {0, 0, 0};
     bool ok = true;
     int t = lane;
     for (int a = 0; a < f.D; ++a) {
       const int o = t % 3 - 1;
       t /= 3;
-      const int ba = axis_bits(d, f.D, a);
+      const int ba = axis_bits(d0, f.D, a);
       const int64_t ia =
           (int64_t)(quantize(c[a], f.lo[a], f.ext[a]) >> (kBits - ba)) + o;
       if (ia < 0 || ia >= ((int64_t)1 << ba))
@@ -310,53 +370,62 @@ __device__ inline Ranges neighbour_ranges(const SortedSet &S, const Frame &f,
       j[a] = (uint64_t)(ia < 0 ? 0 : ia);
     }
     if (ok) {
-      const uint64_t p = cell_prefix(j, f.D, d);
-      const int sh = f.DB - d;
-      g.rs = lower_bound_u64(S.code, 0, S.n, p << sh);
-      g.re = lower_bound_u64(S.code, g.rs, S.n, (p + 1) << sh);
+      pref = cell_prefix(j, f.D, d0);
+      const int sh = f.DB - d0;
+      rs = lower_bound_u64(S.code, 0, S.n, pref << sh);
+      re = lower_bound_u64(S.code, rs, S.n, (pref + 1) << sh);
     }
   }
-  return g;
-}
-
-// f(position, valid) over every candidate of the ranges, in range order
-// (runs packed into one wave pass while they fit in its 64 lanes).
-template <class Fn>
-__device__ inline void scan_ranges(const Ranges &g, Fn &&fn) {
-  const int lane = threadIdx.x & 63;
-  int r = 0;
-  int cs = __shfl(g.rs, 0), ce = __shfl(g.re, 0);
-  while (r < g.nr) {
-    if (ce - cs > 64) {
-      for (; ce - cs > 64; cs += 64)
-        fn(cs + lane, true);
+  // push the non-empty ones in reverse lane order (popped in lane order)
+  const bool keep = lane < nr && re > rs;
+  const uint64_t km = __ballot(keep);
+  const int nk = __popcll(km);
+  if (keep) {
+    const int above = __popcll(km >> lane) - 1;  // kept lanes above this one
+    stk[above] = ScanNode{pref, d0, rs, re};
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int top = nk;
+  while (top > 0) {
+    const ScanNode nd = stk[top - 1];
+    --top;
+    if (nd.re - nd.rs > kRefine && nd.d < f.DB && top + 2 <= kStack) {
+      const uint64_t key = (2 * nd.p + 1) << (f.DB - nd.d - 1);
+      const int mid = wave_lower_bound(S.code, nd.rs, nd.re, key);
+      const ScanNode ch[2] = {{2 * nd.p, nd.d + 1, nd.rs, mid},
+                              {2 * nd.p + 1, nd.d + 1, mid, nd.re}};
+      bool use[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        use[h] = ch[h].re > ch[h].rs &&
+                 cell_box_dist2(ch[h].p, ch[h].d, f, c) <= r2;
+      // every lane has read the popped entry before lane 0 reuses its slot
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) {
+        int t = top;
+        if (use[1])  // child 0 on top: popped first
+          stk[t++] = ch[1];
+        if (use[0])
+          stk[t++] = ch[0];
+      }
+      top += (use[0] ? 1 : 0) + (use[1] ? 1 : 0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       continue;
     }
-    int used = 0, p = -1;
-    while (r < g.nr && used + (ce - cs) <= 64) {
-      const int len = ce - cs;
-      if (lane >= used && lane < used + len)
-        p = cs + (lane - used);
-      used += len;
-      if (++r < g.nr) {
-        cs = __shfl(g.rs, r);
-        ce = __shfl(g.re, r);
-      }
-    }
-    fn(p < 0 ? 0 : p, p >= 0);
+    for (int b = nd.rs; b < nd.re; b += 64)
+      fn(b + lane, b + lane < nd.re);
   }
-}
-
-__device__ inline double dist3(double ax, double ay, double az, double bx,
-                               double by, double bz) {
-  return sqrt(sqdiff3(ax, ay, az, bx, by, bz));
 }
 
 __device__ inline int ball_count(const SortedSet &S, const Frame &f,
-                                 const double *c, double r) {
-  const Ranges g = neighbour_ranges(S, f, c, depth_for(f, r));
+                                 const double *c, double r, ScanNode *stk) {
   int cnt = 0;
-  scan_ranges(g, [&](int p, bool valid) {
+  ball_scan(S, f, c, r, stk, [&](int p, bool valid) {
     const bool hit =
         valid && dist3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= r;
     cnt += __popcll(__ballot(hit));
@@ -364,39 +433,33 @@ __device__ inline int ball_count(const SortedSet &S, const Frame &f,
   return cnt;
 }
 
-// Distance to the kq-th nearest point of S from c (kq <= S.n).
+// #{p : sqdiff(c, p) <= t} (candidates within sqrt(t)).
+__device__ inline int count_d2(const SortedSet &S, const Frame &f,
+                               const double *c, double t, ScanNode *stk) {
+  int cnt = 0;
+  ball_scan(S, f, c, sqrt(t), stk, [&](int p, bool valid) {
+    const bool hit =
+        valid && sqdiff3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= t;
+    cnt += __popcll(__ballot(hit));
+  });
+  return cnt;
+}
+
+// Distance to the kq-th nearest point of S from c (kq <= S.n): the smallest
+// t with #{sqdiff <= t} >= kq, bisected on the bits of t (non-negative
+// doubles order as their bit patterns), each count over the pruned ball of
+// radius sqrt(t); the upper end starts at r0^2 and doubles until it holds
+// kq points.
 __device__ inline double knn_distance(const SortedSet &S, const Frame &f,
-                                      const double *c, int kq, int d) {
-  Ranges g;
-  double lim2 = 0.0;
-  for (;; --d) {
-    g = neighbour_ranges(S, f, c, d);
-    if (d == 0)
-      break;
-    const double lim = f.hmin[d] / (1.0 + 1e-9);
-    lim2 = lim * lim;
-    int cnt = 0;
-    scan_ranges(g, [&](int p, bool valid) {
-      const bool hit =
-          valid && sqdiff3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= lim2;
-      cnt += __popcll(__ballot(hit));
-    });
-    if (cnt >= kq)
-      break;
-  }
-  // smallest t with #{d2 <= t} >= kq, bisected on the bits of t
-  uint64_t lo = 0,
-           hi = d == 0 ? 0x7ff0000000000000ull : (uint64_t)__double_as_longlong(lim2);
+                                      const double *c, int kq, double r0,
+                                      ScanNode *stk) {
+  double R = r0 > 0.0 ? r0 : 1e-300;
+  while (count_d2(S, f, c, R * R, stk) < kq)
+    R *= 2.0;
+  uint64_t lo = 0, hi = (uint64_t)__double_as_longlong(R * R);
   while (lo < hi) {
     const uint64_t mid = lo + (hi - lo) / 2;
-    const double t = __longlong_as_double((long long)mid);
-    int cnt = 0;
-    scan_ranges(g, [&](int p, bool valid) {
-      const bool hit =
-          valid && sqdiff3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= t;
-      cnt += __popcll(__ballot(hit));
-    });
-    if (cnt >= kq)
+    if (count_d2(S, f, c, __longlong_as_double((long long)mid), stk) >= kq)
       hi = mid;
     else
       lo = mid + 1;
@@ -430,17 +493,19 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
   if (i >= e.count)
     return;
   const int lane = threadIdx.x & 63;
+  __shared__ ScanNode stacks[kThreads / 64][kStack];
+  ScanNode *stk = stacks[threadIdx.x >> 6];
   const uint64_t p = e.cellP[i];
   const int d = e.cellD[i];
   double c[3], rc;
   cell_geometry(p, d, e.f, e.overlap, c, &rc);
-  int n = ball_count(e.S, e.f, c, rc);
+  int n = ball_count(e.S, e.f, c, rc, stk);
   double r = rc;
   if (n < e.kq) {
-    r = knn_distance(e.S, e.f, c, e.kq, depth_for(e.f, rc));
+    r = knn_distance(e.S, e.f, c, e.kq, rc, stk);
     if (r < rc)  // max(rc, dk) exactly
       r = rc;
-    n = ball_count(e.S, e.f, c, r);
+    n = ball_count(e.S, e.f, c, r, stk);
   }
   if (lane != 0)
     return;
@@ -511,12 +576,13 @@ __global__ __launch_bounds__(kThreads) void member_kernel(MemberArgs m,
   if (k >= m.K)
     return;
   const int lane = threadIdx.x & 63;
+  __shared__ ScanNode stacks[kThreads / 64][kStack];
+  ScanNode *stk = stacks[threadIdx.x >> 6];
   const double c[3] = {m.cx[k], m.cy[k], m.cz[k]};
   const double rad = m.r[k], radMember = rad - kEps;
-  const Ranges g = neighbour_ranges(m.P, m.f, c, depth_for(m.f, rad));
   int64_t w = fill ? m.off[k] : 0;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  scan_ranges(g, [&](int p, bool valid) {
+  ball_scan(m.P, m.f, c, rad, stk, [&](int p, bool valid) {
     double dd = 0.0;
     if (valid)
       dd = dist3(c[0], c[1], c[2], m.P.x[p], m.P.y[p], m.P.z[p]);

@@ -232,9 +232,14 @@ struct WsArgs {
 int64_t ws_slot_blocks(int nbMax);
 int64_t ws_linv_offset(int nbMax);
 int64_t ws_slot_vec(int nbMax);
+// Block rows up to which a slot keeps coordinates and alpha in LDS.
+int ws_lds_block_rows();
 // all = true: every counter (start of an apply); false: the queues and the
 // fallback list only (next chunk of fields).
-void launch_ws_reset(int *counters, bool all, hipStream_t s);
+// keepFallbacks: reset the first-pass queue only (the next queue of the
+// same chunk of fields).
+void launch_ws_reset(int *counters, bool all, hipStream_t s,
+                     bool keepFallbacks = false);
 // One persistent launch of `grid` workgroups over the queue: FP64 storage
 // (fp32 = false; singular pivots counted) or FP32 storage with FP64
 // refinement (non-converging patches appended to the fallback list).

@@ -833,13 +833,20 @@ void RbfPumPlan::patchInfo(int *pv, int *status, hipStream_t s) const {
 }
 
 // Patches above kMaxPatch sources, largest first (ties: lower patch index),
-// and the per-slot workspaces of their persistent launch: a slot holds the
-// lower block triangle of the largest patch plus its Linv blocks (FP64 size;
-// FP32 storage uses the first half) and 4 + 4 F vectors. Slots: two per CU,
-// fewer if the workspaces would exceed the budget (PACMAN_AMD_WS_BUDGET_MB,
-// default 8192).
+// and the per-slot workspaces of their persistent launches: a slot holds the
+// lower block triangle of the largest patch of its queue plus its Linv
+// blocks (FP64 size; FP32 storage uses the first half) and 4 + 4 F vectors.
+// Queue 0: bordered matrices (n + 8 rows) within the LDS variant's limit;
+// queue 1: larger; shape 2 (the fallback pass) covers both. Slots: two per
+// CU, fewer if the workspaces would exceed the budget
+// (PACMAN_AMD_WS_BUDGET_MB, default 8192). Launches are stream-ordered, so
+// the queues share one buffer.
 void RbfPumPlan::plan_workspace(hipStream_t s) {
   wsHost_.clear();
+  for (auto &q : wsQ_) {
+    q.host.clear();
+    q.nbMax = q.slots = 0;
+  }
   for (int64_t k = 0; k < K_; ++k)
     if (patchN_[k] > kMaxPatch)
       wsHost_.push_back((int)k);
@@ -848,10 +855,14 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
   std::stable_sort(wsHost_.begin(), wsHost_.end(), [&](int x, int y) {
     return patchN_[x] > patchN_[y];
   });
-  const int nbMax = (maxN_ + solve_fields_per_launch() + kWsBlock - 1) /
-                    kWsBlock;
-  const int64_t slotBytes =
-      8 * (ws_slot_blocks(nbMax) + ws_slot_vec(nbMax));
+  const int fmax = solve_fields_per_launch();
+  auto rows = [&](int k) { return (patchN_[k] + fmax + kWsBlock - 1) / kWsBlock; };
+  for (int k : wsHost_) {
+    WsQueue &q = wsQ_[rows(k) <= ws_lds_block_rows() ? 0 : 1];
+    q.host.push_back(k);
+    q.nbMax = std::max(q.nbMax, rows(k));
+  }
+  wsQ_[2].nbMax = std::max(wsQ_[0].nbMax, wsQ_[1].nbMax);
   int cu = 0;
   int dev = 0;
   PACMAN_HIP_CHECK(hipGetDevice(&dev));
@@ -860,49 +871,74 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
   double budgetMb = 8192.0;
   if (const char *e = std::getenv("PACMAN_AMD_WS_BUDGET_MB"))
     budgetMb = std::max(1.0, std::atof(e));
-  const int64_t bySize = std::max<int64_t>(
-      1, (int64_t)(budgetMb * 1048576.0) / std::max<int64_t>(slotBytes, 1));
-  wsSlots_ = (int)std::min<int64_t>(
-      {(int64_t)wsHost_.size(), 2 * (int64_t)std::max(cu, 1), bySize});
-  wsNbMax_ = nbMax;
-  wsDev_.alloc(wsHost_.size());
-  PACMAN_HIP_CHECK(hipMemcpyAsync(wsDev_.get(), wsHost_.data(),
-                                  4 * wsHost_.size(), hipMemcpyHostToDevice,
-                                  s));
+  int64_t workMax = 0, vecMax = 0;
+  for (int i = 0; i < 3; ++i) {
+    WsQueue &q = wsQ_[i];
+    const int64_t count = i == 2 ? (int64_t)wsHost_.size()
+                                 : (int64_t)q.host.size();
+    if (count == 0 || q.nbMax == 0)
+      continue;
+    const int64_t slotBytes =
+        8 * (ws_slot_blocks(q.nbMax) + ws_slot_vec(q.nbMax));
+    const int64_t bySize = std::max<int64_t>(
+        1, (int64_t)(budgetMb * 1048576.0) / std::max<int64_t>(slotBytes, 1));
+    q.slots = (int)std::min<int64_t>(
+        {count, 2 * (int64_t)std::max(cu, 1), bySize});
+    workMax = std::max(workMax, q.slots * ws_slot_blocks(q.nbMax));
+    vecMax = std::max(vecMax, q.slots * ws_slot_vec(q.nbMax));
+    if (i < 2 && !q.host.empty()) {
+      q.dev.alloc(q.host.size());
+      PACMAN_HIP_CHECK(hipMemcpyAsync(q.dev.get(), q.host.data(),
+                                      4 * q.host.size(),
+                                      hipMemcpyHostToDevice, s));
+    }
+  }
   wsFallback_.alloc(wsHost_.size());
-  wsWork_.alloc((size_t)wsSlots_ * ws_slot_blocks(nbMax));
-  wsVec_.alloc((size_t)wsSlots_ * ws_slot_vec(nbMax));
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // wsHost_ copied
+  wsWork_.alloc((size_t)workMax);
+  wsVec_.alloc((size_t)vecMax);
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // queues copied
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s) {
-  WsArgs w{};
-  w.a = a;
-  w.a.patches = wsDev_.get();
-  w.a.count = (int)wsHost_.size();
-  w.coef = coef_.get();
-  w.nbMax = wsNbMax_;
-  w.work = wsWork_.get();
-  w.slotBlocks = ws_slot_blocks(wsNbMax_);
-  w.linvOffset = ws_linv_offset(wsNbMax_);
-  w.vec = wsVec_.get();
-  w.slotVec = ws_slot_vec(wsNbMax_);
-  w.counters = counters_.get();
-  w.fallback = wsFallback_.get();
-  w.countPtr = nullptr;
-  w.queueIdx = kWsQueue;
-  w.maxIR = prm_.maxIR;
-  w.phi0 = kPhi0[prm_.kind];
   const bool mixed = prm_.precision == 1;
-  launch_ws_solve(mixed, prm_.kind, w, wsSlots_, s);
+  auto args = [&](const WsQueue &q) {
+    WsArgs w{};
+    w.a = a;
+    w.coef = coef_.get();
+    w.nbMax = q.nbMax;
+    w.work = wsWork_.get();
+    w.slotBlocks = ws_slot_blocks(q.nbMax);
+    w.linvOffset = ws_linv_offset(q.nbMax);
+    w.vec = wsVec_.get();
+    w.slotVec = ws_slot_vec(q.nbMax);
+    w.counters = counters_.get();
+    w.fallback = wsFallback_.get();
+    w.countPtr = nullptr;
+    w.queueIdx = kWsQueue;
+    w.maxIR = prm_.maxIR;
+    w.phi0 = kPhi0[prm_.kind];
+    return w;
+  };
+  for (int i = 0; i < 2; ++i) {
+    const WsQueue &q = wsQ_[i];
+    if (q.host.empty())
+      continue;
+    if (i == 1)  // the queue counter starts again at zero
+      launch_ws_reset(counters_.get(), false, s, /*keepFallbacks=*/true);
+    WsArgs w = args(q);
+    w.a.patches = q.dev.get();
+    w.a.count = (int)q.host.size();
+    launch_ws_solve(mixed, prm_.kind, w, q.slots, s);
+  }
   if (mixed) {
     // FP64 pass over the patches the refinement handed back (the list
     // length is read on the device; an empty list ends every workgroup at
     // once)
+    WsArgs w = args(wsQ_[2]);
     w.a.patches = wsFallback_.get();
     w.countPtr = counters_.get() + kWsFallbacks;
     w.queueIdx = kWsQueue2;
-    launch_ws_solve(false, prm_.kind, w, wsSlots_, s);
+    launch_ws_solve(false, prm_.kind, w, wsQ_[2].slots, s);
   }
 }
 

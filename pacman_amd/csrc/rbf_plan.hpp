@@ -126,10 +126,17 @@ private:
     std::vector<DevBuf<int>> dev;
   };
   std::vector<Bins> bins_;
-  // Workspace path: patches above kMaxPatch sources, largest first.
-  std::vector<int> wsHost_;
-  DevBuf<int> wsDev_, wsFallback_, counters_;
-  int wsNbMax_ = 0, wsSlots_ = 0;
+  // Workspace path: patches above kMaxPatch sources, largest first, in two
+  // queues (bordered matrices that fit the LDS variant, larger ones); the
+  // FP64 fallback pass of mixed precision takes any of them.
+  struct WsQueue {
+    std::vector<int> host;
+    DevBuf<int> dev;
+    int nbMax = 0, slots = 0;
+  };
+  WsQueue wsQ_[3];            // 0: LDS class, 1: larger, 2: fallback shape
+  std::vector<int> wsHost_;   // all workspace patches
+  DevBuf<int> wsFallback_, counters_;
   DevBuf<double> wsWork_, wsVec_;  // wsWork_ also holds FP32 blocks
   void plan_workspace(hipStream_t s);
   void setup_reference(const double *dSrc, const double *dTgt, hipStream_t s,

@@ -52,6 +52,7 @@ constexpr int NT = 512;           // threads per workgroup
 constexpr int NW = NT / 64;       // waves
 constexpr double kPivotTol = 1e-12;
 constexpr double kIrTol = 1e-12;
+constexpr int kLdsRows = 608;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -112,18 +113,27 @@ __device__ double block_max(double v, double *red) {
   return m;
 }
 
-template <typename T, int K>
+// SMALL: the bordered matrix has at most kLdsRows rows, so coordinates and
+// alpha live in LDS; otherwise in the slot's vectors in device memory.
+template <typename T, int K, bool SMALL>
 __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
   using M = Mma<T>;
   using V = typename M::V;
   const SolveArgs &a = w.a;
   __shared__ int sItem, sFlag;
-  __shared__ double eb[TB][2 * TB + 1];  // [A_pp | I] during the elimination
   __shared__ double emul[TB];            // multipliers of the current pivot
   __shared__ T linv[TB][TB + 1];      // Linv of the current panel
   __shared__ double sd[TB], sdinv[TB];
   __shared__ double scoef[kFMax][4];
   __shared__ double red[NW];
+  constexpr int LR = SMALL ? kLdsRows : 1;
+  constexpr int LA = SMALL ? kFMax * kLdsRows : TB * (2 * TB + 1);
+  __shared__ double lxyz[3][LR];
+  __shared__ double lal[LA];
+  // [A_pp | I] during each elimination: shares lal (alpha is written only
+  // after the factorisation)
+  static_assert(TB * (2 * TB + 1) <= LA, "elimination buffer");
+  double(*eb)[2 * TB + 1] = reinterpret_cast<double(*)[2 * TB + 1]>(lal);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int np = w.nbMax * TB;  // vector stride of the slot
@@ -152,6 +162,21 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
     const int NB = (n + F + TB - 1) / TB, P = (n + TB - 1) / TB;
     const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
     const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
+    double *px, *py, *pz, *alp;  // coordinates; alpha with stride ast
+    int ast;
+    if constexpr (SMALL) {
+      px = lxyz[0];
+      py = lxyz[1];
+      pz = lxyz[2];
+      alp = lal;
+      ast = kLdsRows;
+    } else {
+      px = gx;
+      py = gy;
+      pz = gz;
+      alp = al;
+      ast = np;
+    }
     if (tid < kFMax * 4)
       scoef[tid >> 2][tid & 3] =
           (tid >> 2) < F
@@ -171,9 +196,9 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
         y = a.sy[p];
         z = a.sz[p];
       }
-      gx[t] = x;
-      gy[t] = y;
-      gz[t] = z;
+      px[t] = x;
+      py[t] = y;
+      pz[t] = z;
       for (int f = 0; f < F; ++f)
         bv[(int64_t)f * np + t] =
             t < n ? a.vals[(int64_t)f * a.N + p] -
@@ -201,7 +226,7 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
           const int i = I * TB + (rc >> 5), j = J * TB + (rc & 31);
           T v;
           if (i < n && j < n)
-            v = phi_t<K, T>(gx[i], gy[i], gz[i], gx[j], gy[j], gz[j], dinv);
+            v = phi_t<K, T>(px[i], py[i], pz[i], px[j], py[j], pz[j], dinv);
           else if (i >= n && i < n + F && j < n)
             v = (T)bv[(int64_t)(i - n) * np + j];
           else
@@ -327,26 +352,33 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
       return (double)W[blk(i / TB, c / TB) + (i % TB) * TB + (c % TB)];
     };
     // L^T x = s (s destroyed), block by block from the bottom; x = dst.
-    auto back_subst = [&](double *s, double *dst) {
+    auto back_subst = [&](double *s, double *dst, int dstStride) {
       for (int B = P - 1; B >= 0; --B) {
         const int kb = min(TB, n - TB * B);
         const T *LB = Li + (int64_t)B * TB * TB;
         for (int t = tid; t < F * TB; t += NT) {
           const int f = t / TB, cl = t % TB;
+          const double *sb = s + (int64_t)f * np + TB * B;
           double acc = 0.0;
-          for (int c2 = cl; c2 < kb; ++c2)
-            acc = fma((double)LB[c2 * TB + cl], s[(int64_t)f * np + TB * B + c2],
-                      acc);
+          // Linv is lower triangular with zeros past kb: every load is in
+          // range and the masked terms add exact zeros
+#pragma unroll 8
+          for (int c2 = 0; c2 < TB; ++c2)
+            acc = fma((double)LB[c2 * TB + cl], c2 < kb ? sb[c2] : 0.0, acc);
           if (cl < kb)
-            dst[(int64_t)f * np + TB * B + cl] = acc;
+            dst[(int64_t)f * dstStride + TB * B + cl] = acc;
         }
         __syncthreads();
         const int wdt = TB * B;
         for (int t = tid; t < F * wdt; t += NT) {
           const int f = t / wdt, c = t % wdt;
+          // column c of the block row B: L[TB B + r][c], r < kb
+          const T *col = W + blk(B, c / TB) + (c % TB);
+          const double *xb = dst + (int64_t)f * dstStride + TB * B;
           double acc = 0.0;
-          for (int r = 0; r < kb; ++r)
-            acc = fma(Lat(TB * B + r, c), dst[(int64_t)f * np + TB * B + r],
+#pragma unroll 8
+          for (int r = 0; r < TB; ++r)
+            acc = fma(r < kb ? (double)col[r * TB] : 0.0, r < kb ? xb[r] : 0.0,
                       acc);
           s[(int64_t)f * np + c] -= acc;
         }
@@ -360,10 +392,11 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
         const T *LB = Li + (int64_t)B * TB * TB;
         for (int t = tid; t < F * TB; t += NT) {
           const int f = t / TB, cl = t % TB;
+          const double *sb = s + (int64_t)f * np + TB * B;
           double acc = 0.0;
-          for (int c2 = 0; c2 <= cl && c2 < kb; ++c2)
-            acc = fma((double)LB[cl * TB + c2], s[(int64_t)f * np + TB * B + c2],
-                      acc);
+#pragma unroll 8
+          for (int c2 = 0; c2 < TB; ++c2)
+            acc = fma((double)LB[cl * TB + c2], c2 < kb ? sb[c2] : 0.0, acc);
           if (cl < kb)
             dst[(int64_t)f * np + TB * B + cl] = acc;
         }
@@ -371,10 +404,13 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
         const int r0 = TB * (B + 1), rows = n - r0;
         for (int t = tid; t < F * max(rows, 0); t += NT) {
           const int f = t / rows, i = r0 + t % rows;
+          // row i of the block column B: L[i][TB B + c], c < kb
+          const T *row = W + blk(i / TB, B) + (i % TB) * TB;
+          const double *xb = dst + (int64_t)f * np + TB * B;
           double acc = 0.0;
-          for (int c = 0; c < kb; ++c)
-            acc = fma(Lat(i, TB * B + c), dst[(int64_t)f * np + TB * B + c],
-                      acc);
+#pragma unroll 8
+          for (int c = 0; c < TB; ++c)
+            acc = fma(c < kb ? (double)row[c] : 0.0, c < kb ? xb[c] : 0.0, acc);
           s[(int64_t)f * np + i] -= acc;
         }
         __syncthreads();
@@ -387,7 +423,7 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
       sv[(int64_t)f * np + c] = Lat(n + f, c);
     }
     __syncthreads();
-    back_subst(sv, al);
+    back_subst(sv, alp, ast);
 
     int status = 0;
     if constexpr (FP32) {
@@ -403,13 +439,14 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
 #pragma unroll
           for (int f = 0; f < kFMax; ++f)
             acc[f] = 0.0;
-          const double xi = gx[i], yi = gy[i], zi = gz[i];
+          const double xi = px[i], yi = py[i], zi = pz[i];
+#pragma unroll 4
           for (int j = 0; j < n; ++j) {
-            const double ph = phi64<K>(xi, yi, zi, gx[j], gy[j], gz[j], dinv);
+            const double ph = phi64<K>(xi, yi, zi, px[j], py[j], pz[j], dinv);
 #pragma unroll
             for (int f = 0; f < kFMax; ++f)
               if (f < F)
-                acc[f] = fma(ph, al[(int64_t)f * np + j], acc[f]);
+                acc[f] = fma(ph, alp[f * ast + j], acc[f]);
           }
           for (int f = 0; f < F; ++f)
             sv[(int64_t)f * np + i] = bv[(int64_t)f * np + i] - acc[f];
@@ -421,13 +458,13 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
           sv[(int64_t)f * np + c] = zv[(int64_t)f * np + c] / dv[c];
         }
         __syncthreads();
-        back_subst(sv, zv);
+        back_subst(sv, zv, np);
         double dmax = 0.0, amax = 0.0;
         for (int t = tid; t < F * n; t += NT) {
           const int f = t / n, c = t % n;
           const double dl = zv[(int64_t)f * np + c];
-          const double x = al[(int64_t)f * np + c] + dl;
-          al[(int64_t)f * np + c] = x;
+          const double x = alp[f * ast + c] + dl;
+          alp[f * ast + c] = x;
           dmax = fmax(dmax, fabs(dl));
           amax = fmax(amax, fabs(x));
         }
@@ -472,7 +509,7 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
     if (a.alphaOut)
       for (int t = tid; t < F * n; t += NT) {
         const int f = t / n, c = t % n;
-        a.alphaOut[(s0 + c) * F + f] = al[(int64_t)f * np + c];
+        a.alphaOut[(s0 + c) * F + f] = alp[f * ast + c];
       }
     // 6. evaluation at the member targets
     const int64_t p0 = a.pairOff[patch];
@@ -487,12 +524,13 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
         acc[f] = 0.0;
+#pragma unroll 4
       for (int j = 0; j < n; ++j) {
-        const double ph = phi64<K>(yx, yy, yz, gx[j], gy[j], gz[j], dinv);
+        const double ph = phi64<K>(yx, yy, yz, px[j], py[j], pz[j], dinv);
 #pragma unroll
         for (int f = 0; f < kFMax; ++f)
           if (f < F)
-            acc[f] = fma(ph, al[(int64_t)f * np + j], acc[f]);
+            acc[f] = fma(ph, alp[f * ast + j], acc[f]);
       }
       const int slot = a.pairSlot[p0 + q];
       for (int f = 0; f < F; ++f) {
@@ -507,20 +545,22 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
   }
 }
 
-__global__ void ws_reset_kernel(int *counters, int n, int all) {
+// mode 2: every counter; 1: queues and fallback list; 0: first-pass queue
+__global__ void ws_reset_kernel(int *counters, int n, int mode) {
   const int i = threadIdx.x;
-  if (i < n && (all || i == kWsQueue || i == kWsQueue2 || i == kWsFallbacks))
+  if (i < n && (mode == 2 || i == kWsQueue ||
+                (mode == 1 && (i == kWsQueue2 || i == kWsFallbacks))))
     counters[i] = 0;
 }
 
-template <typename T>
+template <typename T, bool SMALL>
 void launch_ws_t(int kind, const WsArgs &w, int grid, hipStream_t s) {
   switch (kind) {
-  case 0: ws_solve_kernel<T, 0><<<grid, NT, 0, s>>>(w); break;
-  case 1: ws_solve_kernel<T, 1><<<grid, NT, 0, s>>>(w); break;
-  case 2: ws_solve_kernel<T, 2><<<grid, NT, 0, s>>>(w); break;
-  case 3: ws_solve_kernel<T, 3><<<grid, NT, 0, s>>>(w); break;
-  default: ws_solve_kernel<T, 4><<<grid, NT, 0, s>>>(w); break;
+  case 0: ws_solve_kernel<T, 0, SMALL><<<grid, NT, 0, s>>>(w); break;
+  case 1: ws_solve_kernel<T, 1, SMALL><<<grid, NT, 0, s>>>(w); break;
+  case 2: ws_solve_kernel<T, 2, SMALL><<<grid, NT, 0, s>>>(w); break;
+  case 3: ws_solve_kernel<T, 3, SMALL><<<grid, NT, 0, s>>>(w); break;
+  default: ws_solve_kernel<T, 4, SMALL><<<grid, NT, 0, s>>>(w); break;
   }
   PACMAN_HIP_CHECK(hipGetLastError());
 }
@@ -533,12 +573,16 @@ int64_t ws_slot_blocks(int nbMax) {
 int64_t ws_linv_offset(int nbMax) {
   return (int64_t)nbMax * (nbMax + 1) / 2 * (TB * TB);
 }
+int ws_lds_block_rows() { return kLdsRows / TB; }
+
 int64_t ws_slot_vec(int nbMax) {
   return (int64_t)(4 + 4 * kFMax) * nbMax * TB;
 }
 
-void launch_ws_reset(int *counters, bool all, hipStream_t s) {
-  ws_reset_kernel<<<1, 64, 0, s>>>(counters, kWsCounters, all ? 1 : 0);
+void launch_ws_reset(int *counters, bool all, hipStream_t s,
+                     bool keepFallbacks) {
+  ws_reset_kernel<<<1, 64, 0, s>>>(counters, kWsCounters,
+                                   all ? 2 : keepFallbacks ? 0 : 1);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
@@ -548,10 +592,19 @@ void launch_ws_solve(bool fp32, int kind, const WsArgs &w, int grid,
     return;
   if (w.a.F < 1 || w.a.F > kFMax)
     throw std::runtime_error("workspace solve: 1 <= fields <= 8 per launch");
-  if (fp32)
-    launch_ws_t<float>(kind, w, grid, s);
-  else
-    launch_ws_t<double>(kind, w, grid, s);
+  // the LDS variant covers slots of up to kLdsRows rows
+  const bool small = w.nbMax * TB <= kLdsRows;
+  if (fp32) {
+    if (small)
+      launch_ws_t<float, true>(kind, w, grid, s);
+    else
+      launch_ws_t<float, false>(kind, w, grid, s);
+  } else {
+    if (small)
+      launch_ws_t<double, true>(kind, w, grid, s);
+    else
+      launch_ws_t<double, false>(kind, w, grid, s);
+  }
 }
 
 } // namespace pacman_amd
