@@ -203,10 +203,19 @@ struct SolveArgs {
   // patch; each one found adds 1 to *singular (when set).
   double phi0;
   int *singular;
+  // Patches with more member-target pairs than kHeavyPairs: heavyOff[k] >= 0
+  // is where their alpha goes (alphaHeavy + heavyOff[k] + e * F + f); the
+  // solve kernels store it and skip the evaluation, which
+  // launch_heavy_eval spreads over many workgroups.
+  const int64_t *heavyOff;
+  double *alphaHeavy;
 };
+constexpr int kHeavyPairs = 4096;
+constexpr int kHeavyChunk = 512;  // pairs per workgroup of the heavy eval
 
 // ---- workspace solve (rbf_solve_ws.hip), patches of any size -------------
 constexpr int kWsBlock = 32;   // block edge of the workspace factorisation
+constexpr int kWsSlotsPerCu = 1;  // resident workgroups of the kernel per CU
 // counters[]: work queue of the first pass, fallback list length, singular
 // patches, refinement iterations, work queue of the FP64 fallback pass,
 // fallbacks of the whole apply
@@ -245,6 +254,10 @@ void launch_ws_reset(int *counters, bool all, hipStream_t s,
 // refinement (non-converging patches appended to the fallback list).
 void launch_ws_solve(bool fp32, int kind, const WsArgs &w, int grid,
                      hipStream_t s);
+// Evaluation of the heavy patches: chunks[c] = (patch, first pair) of
+// kHeavyChunk pairs; alpha from a.alphaHeavy, polynomial from coef.
+void launch_heavy_eval(int kind, const SolveArgs &a, const double *coef,
+                       const int2 *chunks, int nChunks, hipStream_t s);
 // Fields handled by one poly/solve launch.
 int solve_fields_per_launch();
 // Polynomial fit of every patch (one wave per patch): coef[K][F][4], pvOut.

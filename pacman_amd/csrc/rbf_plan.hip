@@ -66,6 +66,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   counters_.alloc(kWsCounters);
   launch_ws_reset(counters_.get(), true, s);
   plan_workspace(s);
+  plan_heavy(s);
   bins_.resize(solve_fields_per_launch());
   bins_for(1, s);
   pv_.alloc(std::max<int64_t>(K_, 1));
@@ -445,6 +446,8 @@ SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
   a.singular = counters_.get() + kWsSingular;
   a.patchDinv = patchDinv_.n ? patchDinv_.get() : nullptr;
   a.patchRinv = patchRinv_.n ? patchRinv_.get() : nullptr;
+  a.heavyOff = nHeavyChunks_ ? heavyOff_.get() : nullptr;
+  a.alphaHeavy = alphaHeavy_.get();
   return a;
 }
 
@@ -701,6 +704,8 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
         launch_ws_reset(counters_.get(), false, s);
       launch_workspace(a, s);
     }
+    launch_heavy_eval(prm_.kind, a, coef_.get(), heavyChunks_.get(),
+                      nHeavyChunks_, s);
     BlendArgs bl = blendArgs(nf);
     bl.out = dOut + (int64_t)f0 * M_;
     if (F <= perLaunch) {
@@ -837,8 +842,9 @@ void RbfPumPlan::patchInfo(int *pv, int *status, hipStream_t s) const {
 // lower block triangle of the largest patch of its queue plus its Linv
 // blocks (FP64 size; FP32 storage uses the first half) and 4 + 4 F vectors.
 // Queue 0: bordered matrices (n + 8 rows) within the LDS variant's limit;
-// queue 1: larger; shape 2 (the fallback pass) covers both. Slots: two per
-// CU, fewer if the workspaces would exceed the budget
+// queue 1: larger; shape 2 (the fallback pass) covers both. Slots: one per
+// CU (the kernel's register budget allows one workgroup per CU), fewer if
+// the workspaces would exceed the budget
 // (PACMAN_AMD_WS_BUDGET_MB, default 8192). Launches are stream-ordered, so
 // the queues share one buffer.
 void RbfPumPlan::plan_workspace(hipStream_t s) {
@@ -883,7 +889,7 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
     const int64_t bySize = std::max<int64_t>(
         1, (int64_t)(budgetMb * 1048576.0) / std::max<int64_t>(slotBytes, 1));
     q.slots = (int)std::min<int64_t>(
-        {count, 2 * (int64_t)std::max(cu, 1), bySize});
+        {count, kWsSlotsPerCu * (int64_t)std::max(cu, 1), bySize});
     workMax = std::max(workMax, q.slots * ws_slot_blocks(q.nbMax));
     vecMax = std::max(vecMax, q.slots * ws_slot_vec(q.nbMax));
     if (i < 2 && !q.host.empty()) {
@@ -897,6 +903,45 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
   wsWork_.alloc((size_t)workMax);
   wsVec_.alloc((size_t)vecMax);
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // queues copied
+}
+
+// Patches holding more than kHeavyPairs (target, patch) pairs (large cells
+// of sparse regions next to dense ones: up to ~1e5 pairs) would keep one
+// workgroup evaluating for milliseconds; their alpha goes to alphaHeavy_
+// (n x 8 doubles each) and chunks of kHeavyChunk pairs are evaluated by
+// separate workgroups.
+void RbfPumPlan::plan_heavy(hipStream_t s) {
+  nHeavyChunks_ = 0;
+  if (K_ == 0)
+    return;
+  std::vector<int64_t> poff(K_ + 1);
+  PACMAN_HIP_CHECK(hipMemcpyAsync(poff.data(), pairs_.off.get(),
+                                  8 * (K_ + 1), hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<int64_t> hoff(K_, -1);
+  std::vector<int2> chunks;
+  int64_t total = 0;
+  for (int64_t k = 0; k < K_; ++k) {
+    const int64_t m = poff[k + 1] - poff[k];
+    if (m <= kHeavyPairs)
+      continue;
+    hoff[k] = total;
+    total += (int64_t)patchN_[k] * solve_fields_per_launch();
+    for (int64_t q = 0; q < m; q += kHeavyChunk)
+      chunks.push_back(make_int2((int)k, (int)q));
+  }
+  if (chunks.empty())
+    return;
+  heavyOff_.alloc(K_);
+  alphaHeavy_.alloc(std::max<int64_t>(total, 1));
+  heavyChunks_.alloc(chunks.size());
+  PACMAN_HIP_CHECK(hipMemcpyAsync(heavyOff_.get(), hoff.data(), 8 * K_,
+                                  hipMemcpyHostToDevice, s));
+  PACMAN_HIP_CHECK(hipMemcpyAsync(heavyChunks_.get(), chunks.data(),
+                                  sizeof(int2) * chunks.size(),
+                                  hipMemcpyHostToDevice, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  nHeavyChunks_ = (int)chunks.size();
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s) {

@@ -143,6 +143,13 @@ private:
                        std::chrono::steady_clock::time_point t0);
   void setup_adaptive(const double *dSrc, const double *dTgt, hipStream_t s);
   DevBuf<double> patchR_, patchRinv_, patchDinv_;  // adaptive plans only
+  // Patches with more than kHeavyPairs target pairs: evaluated in chunks
+  // of kHeavyChunk pairs by their own launch.
+  void plan_heavy(hipStream_t s);
+  DevBuf<int64_t> heavyOff_;
+  DevBuf<double> alphaHeavy_;
+  DevBuf<int2> heavyChunks_;
+  int nHeavyChunks_ = 0;
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, hipStream_t s);
   std::vector<int> patchN_;  // sources per patch (host)

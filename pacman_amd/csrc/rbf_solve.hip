@@ -683,6 +683,93 @@ __global__ void src_reduce_kernel(const int64_t *off, const int *ent,
   }
 }
 
+// Evaluation of one chunk of kHeavyChunk (target, patch) pairs of a heavy
+// patch (src/rbf_pum/solver/solver.hpp:265-273): t = E alpha + V beta at
+// every member target of the chunk, two pairs per thread, the patch's
+// sources staged in LDS 256 at a time (sums in source order).
+template <int K>
+__global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
+                                                         const double *coef,
+                                                         const int2 *chunks) {
+  __shared__ double sxs[256], sys[256], szs[256];
+  __shared__ double sal[kFMax][256];
+  __shared__ double scoef[kFMax][4];
+  const int tid = threadIdx.x;
+  const int patch = chunks[blockIdx.x].x, q0 = chunks[blockIdx.x].y;
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int64_t p0 = a.pairOff[patch];
+  const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + kHeavyChunk);
+  const int F = a.F;
+  const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
+  const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
+  const double *ah = a.alphaHeavy + a.heavyOff[patch];
+  if (tid < kFMax * 4)
+    scoef[tid >> 2][tid & 3] =
+        (tid >> 2) < F ? coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)]
+                       : 0.0;
+  double yx[2], yy[2], yz[2];
+  int slot[2];
+  bool mem[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int q = q0 + tid + 256 * h;
+    mem[h] = false;
+    yx[h] = yy[h] = yz[h] = 0.0;
+    slot[h] = 0;
+    if (q < q1) {
+      const int e = a.pairTgt[p0 + q];
+      mem[h] = ((unsigned)e >> 31) != 0;
+      const int tp = e & 0x7fffffff;
+      yx[h] = a.tx[tp];
+      yy[h] = a.ty[tp];
+      yz[h] = a.tz[tp];
+      slot[h] = a.pairSlot[p0 + q];
+    }
+  }
+  double acc[2][kFMax];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int f = 0; f < kFMax; ++f)
+      acc[h][f] = 0.0;
+  for (int j0 = 0; j0 < n; j0 += 256) {
+    const int jn = min(256, n - j0);
+    __syncthreads();
+    if (tid < jn) {
+      const int p = a.srcIdx[s0 + j0 + tid];
+      sxs[tid] = a.sx[p];
+      sys[tid] = a.sy[p];
+      szs[tid] = a.sz[p];
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f)
+        sal[f][tid] = f < F ? ah[(int64_t)(j0 + tid) * F + f] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < jn; ++j) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double ph = wendland<K, double, true>(
+            dist_fast(yx[h], yy[h], yz[h], sxs[j], sys[j], szs[j]), dinv);
+#pragma unroll
+        for (int f = 0; f < kFMax; ++f)
+          if (f < F)
+            acc[h][f] = fma(ph, sal[f][j], acc[h][f]);
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (!mem[h])
+      continue;
+    for (int f = 0; f < F; ++f) {
+      const double vb = scoef[f][0] + scoef[f][1] * (yx[h] - cx) +
+                        scoef[f][2] * (yy[h] - cy) + scoef[f][3] * (yz[h] - cz);
+      a.contrib[(int64_t)slot[h] * F + f] = acc[h][f] + vb;
+    }
+  }
+}
+
 __global__ void zero_i32_kernel(int *p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n)
@@ -692,6 +779,20 @@ __global__ void zero_i32_kernel(int *p, int64_t n) {
 } // namespace
 
 int solve_fields_per_launch() { return kFMax; }
+
+void launch_heavy_eval(int kind, const SolveArgs &a, const double *coef,
+                       const int2 *chunks, int nChunks, hipStream_t s) {
+  if (nChunks <= 0)
+    return;
+  switch (kind) {
+  case 0: heavy_eval_kernel<0><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  case 1: heavy_eval_kernel<1><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  case 2: heavy_eval_kernel<2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  case 3: heavy_eval_kernel<3><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  default: heavy_eval_kernel<4><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  }
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
 
 void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s) {
   if (K <= 0)

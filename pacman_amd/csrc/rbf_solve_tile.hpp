@@ -706,6 +706,14 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
       for (int t = tid; t < n; t += NT)
         for (int f = 0; f < F; ++f)
           a.alphaOut[(s0 + t) * F + f] = rhs[f][t];
+    if (a.heavyOff && a.heavyOff[patch] >= 0) {
+      // evaluated by the heavy-patch kernel across many workgroups
+      double *ah = a.alphaHeavy + a.heavyOff[patch];
+      for (int t = tid; t < n; t += NT)
+        for (int f = 0; f < F; ++f)
+          ah[t * F + f] = rhs[f][t];
+      return;
+    }
   }
 
   // evaluation at the member targets: thread = (pair, slice) over chunks of

@@ -116,7 +116,7 @@ __device__ double block_max(double v, double *red) {
 // SMALL: the bordered matrix has at most kLdsRows rows, so coordinates and
 // alpha live in LDS; otherwise in the slot's vectors in device memory.
 template <typename T, int K, bool SMALL>
-__global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
+__global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
   using M = Mma<T>;
   using V = typename M::V;
   const SolveArgs &a = w.a;
@@ -286,63 +286,101 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
       __syncthreads();
       if (tid < kk)
         dv[TB * p + tid] = sd[tid];
-      // rows below the pivots: L = A Linv^T D^-1 (16-row slabs per wave)
-      const int firstSlab = kk < TB ? 2 * p + (kk >> 4) : 2 * (p + 1);
-      for (int sl = firstSlab + wave; sl < 2 * NB; sl += NW) {
-        const int I = sl >> 1, rs = (sl & 1) * 16;
+      // rows below the pivots: L = A Linv^T D^-1, one 32-row block per wave
+      // step, every operand load issued before the first product
+      const int firstRow = kk < TB ? p : p + 1;
+      for (int I = firstRow + wave; I < NB; I += NW) {
         T *B = W + blk(I, p);
-        V acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+        T av[2][TB / 4];
 #pragma unroll
-        for (int s = 0; s < TB / 4; ++s) {
-          const int kq = 4 * s + (lane >> 4);
-          const T av = B[(rs + (lane & 15)) * TB + kq];
-          acc0 = M::mma(av, linv[lane & 15][kq], acc0);
-          acc1 = M::mma(av, linv[16 + (lane & 15)][kq], acc1);
-        }
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = rs + M::row(lane, v), c = lane & 15;
-          if (TB * I + r >= TB * p + kk) {
-            B[r * TB + c] = (T)(acc0[v] * sdinv[c]);
-            B[r * TB + 16 + c] = (T)(acc1[v] * sdinv[16 + c]);
+          for (int q = 0; q < TB / 4; ++q)
+            av[h][q] = B[(16 * h + (lane & 15)) * TB + 4 * q + (lane >> 4)];
+        V acc[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int c2 = 0; c2 < 2; ++c2)
+            acc[h][c2] = V{0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < TB / 4; ++q) {
+          const int kq = 4 * q + (lane >> 4);
+          const T l0 = linv[lane & 15][kq], l1 = linv[16 + (lane & 15)][kq];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            acc[h][0] = M::mma(av[h][q], l0, acc[h][0]);
+            acc[h][1] = M::mma(av[h][q], l1, acc[h][1]);
           }
         }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = 16 * h + M::row(lane, v), c = lane & 15;
+            if (TB * I + r >= TB * p + kk) {
+              B[r * TB + c] = (T)(acc[h][0][v] * sdinv[c]);
+              B[r * TB + 16 + c] = (T)(acc[h][1][v] * sdinv[16 + c]);
+            }
+          }
       }
       __syncthreads();
       if (p + 1 < P) {
-        // trailing update of blocks (I, J), p < J < P, J <= I < NB, per
-        // 16 x 16 tile: A_IJ -= (L_Ip D) L_Jp^T
+        // trailing update of blocks (I, J), p < J < P, J <= I < NB, one
+        // block per wave step: A_IJ -= (L_Ip D) L_Jp^T for all four 16 x 16
+        // tiles (the upper one of a diagonal block included: the next
+        // elimination reads whole rows of the diagonal block), every operand
+        // loaded before the first product
         int64_t total = 0;
         for (int J = p + 1; J < P; ++J)
-          total += 4 * (NB - J);
+          total += NB - J;
         for (int64_t it = wave; it < total; it += NW) {
-          int64_t rem = it >> 2;
-          const int t = (int)(it & 3);
+          int64_t rem = it;
           int J = p + 1;
           while (rem >= NB - J) {
             rem -= NB - J;
             ++J;
           }
           const int I = J + (int)rem;
-          // all four tiles, the upper one of a diagonal block included: the
-          // next elimination reads whole rows of the diagonal block
-          const int ti = t >> 1, tj = t & 1;
           T *C = W + blk(I, J);
           const T *LI = W + blk(I, p), *LJ = W + blk(J, p);
-          V c;
+          T a[2][TB / 4], b[2][TB / 4];
 #pragma unroll
-          for (int v = 0; v < 4; ++v)
-            c[v] = C[(16 * ti + M::row(lane, v)) * TB + 16 * tj + (lane & 15)];
+          for (int h = 0; h < 2; ++h)
 #pragma unroll
-          for (int s = 0; s < TB / 4; ++s) {
-            const int kq = 4 * s + (lane >> 4);
-            const T av = LI[(16 * ti + (lane & 15)) * TB + kq] * (T)(-sd[kq]);
-            const T bw = LJ[(16 * tj + (lane & 15)) * TB + kq];
-            c = M::mma(av, bw, c);
+            for (int q = 0; q < TB / 4; ++q) {
+              const int o = (16 * h + (lane & 15)) * TB + 4 * q + (lane >> 4);
+              a[h][q] = LI[o];
+              b[h][q] = LJ[o];
+            }
+          V c[2][2];
+#pragma unroll
+          for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                c[ti][tj][v] = C[(16 * ti + M::row(lane, v)) * TB + 16 * tj +
+                                 (lane & 15)];
+#pragma unroll
+          for (int q = 0; q < TB / 4; ++q) {
+            const T dk = (T)(-sd[4 * q + (lane >> 4)]);
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti) {
+              const T av = a[ti][q] * dk;
+#pragma unroll
+              for (int tj = 0; tj < 2; ++tj)
+                c[ti][tj] = M::mma(av, b[tj][q], c[ti][tj]);
+            }
           }
 #pragma unroll
-          for (int v = 0; v < 4; ++v)
-            C[(16 * ti + M::row(lane, v)) * TB + 16 * tj + (lane & 15)] = c[v];
+          for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                C[(16 * ti + M::row(lane, v)) * TB + 16 * tj + (lane & 15)] =
+                    c[ti][tj][v];
         }
       }
       __syncthreads();
@@ -511,6 +549,18 @@ __global__ __launch_bounds__(NT, 4) void ws_solve_kernel(WsArgs w) {
         const int f = t / n, c = t % n;
         a.alphaOut[(s0 + c) * F + f] = alp[f * ast + c];
       }
+    if (a.heavyOff && a.heavyOff[patch] >= 0) {
+      // evaluated by the heavy-patch kernel across many workgroups
+      double *ah = a.alphaHeavy + a.heavyOff[patch];
+      for (int t = tid; t < F * n; t += NT) {
+        const int f = t / n, c = t % n;
+        ah[c * F + f] = alp[f * ast + c];
+      }
+      if (tid == 0 && a.status)
+        a.status[patch] = status;
+      __syncthreads();
+      continue;
+    }
     // 6. evaluation at the member targets
     const int64_t p0 = a.pairOff[patch];
     const int mall = (int)(a.pairOff[patch + 1] - p0);

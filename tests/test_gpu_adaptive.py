@@ -114,3 +114,24 @@ def test_adaptive_support_scale_and_device_tensors(oracle_mod):
     print(f"support scale 0.7: rel-L2 vs oracle {err:.2e}")
     assert err < 1e-10
     assert plan.support == pytest.approx(0.7 * plan.radius)
+
+
+@pytest.mark.parametrize("rbf,precision", [(0x11, "fp64"), (0x13, "mixed")])
+def test_heavy_patches_chunked_evaluation(rbf, precision, oracle_mod):
+    """Few sources, many targets: patches with more than 4096 (target,
+    patch) pairs are evaluated in chunks across workgroups (their alpha is
+    stored by the solve); the result must equal the oracle's."""
+    S = W.gaussian_mixture(3000, 11)
+    T = W.uniform(200000, 12)
+    V = np.stack([W.franke_3d(*S.T), np.sin(4 * S[:, 1])])
+    o = oracle_mod.AdaptiveSetup(3, rbf, S, T, 40, 20, 600)
+    m = np.diff(o.csr(1)[0])
+    assert m.max() > 4096
+    ref = o.apply(V)
+    plan = pacman.rbf.Plan.adaptive(3, rbf, S, T, 40, 20, 600,
+                                    precision=precision)
+    got = plan.apply(V)
+    err = rel_l2(got, ref)
+    print(f"heavy: {np.sum(m > 4096)} patches above 4096 pairs (max {m.max()})"
+          f", large {plan.num_large_patches}; rel-L2 vs oracle {err:.2e}")
+    assert err < 1e-10
