@@ -176,12 +176,17 @@ int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
  * setup phases 0 grids, 1 patch radius, 2 patch centres, 3 membership,
  * 4 setup total (host clock); apply phases of the last apply (HIP events on
  * its stream) 5 value gather, 6 per-patch solve (polynomial fit included),
- * 7 blend, 8 apply total, 9 polynomial fit of the first field chunk. */
+ * 7 blend, 8 apply total, 9 polynomial fit of the first field chunk,
+ * 10 first-pass workspace launches (patches above 192 sources) of the first
+ * field chunk (0 when the plan has none). */
 int pacman_rbf_plan_phase_times(pacman_rbf_plan plan, double *ms, int n);
 
 /* Work counters (up to n entries; returns the count written): 0 patches K,
  * 1 sum n_k, 2 sum n_k^2, 3 sum n_k^3, 4 sum n_k m_k (member targets),
- * 5 target/patch pairs within r, 6 largest n_k. */
+ * 5 target/patch pairs within r, 6 largest n_k; over the patches above 192
+ * sources: 7 count, 8 sum n_k^3, 9 sum n_k^2, 10 sum n_k m_k of those the
+ * workspace kernel evaluates, 11 sum n_k m_k of those evaluated in chunks
+ * (more than 4096 target/patch pairs). */
 int pacman_rbf_plan_work(pacman_rbf_plan plan, double *work, int n);
 
 const char *pacman_rbf_last_error(void);

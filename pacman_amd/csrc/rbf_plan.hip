@@ -284,6 +284,10 @@ void RbfPumPlan::phaseTimes(double *ms) const {
   ms[8] = t;
   PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[1], ev_[4]));
   ms[9] = t;
+  if (wsTimed_) {
+    PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[5], ev_[6]));
+    ms[10] = t;
+  }
 }
 
 void RbfPumPlan::work(double *w) const {
@@ -297,6 +301,7 @@ void RbfPumPlan::work(double *w) const {
     PACMAN_HIP_CHECK(hipMemcpy(t.data(), pairs_.tgt.get(), 4 * pairs_.P,
                                hipMemcpyDeviceToHost));
   double sn = 0, sn2 = 0, sn3 = 0, snm = 0;
+  double wk = 0, wn3 = 0, wn2 = 0, wnm = 0, hnm = 0;
   for (int64_t k = 0; k < K_; ++k) {
     const double n = (double)(off[k + 1] - off[k]);
     int64_t m = 0;
@@ -306,6 +311,15 @@ void RbfPumPlan::work(double *w) const {
     sn2 += n * n;
     sn3 += n * n * n;
     snm += n * (double)m;
+    if (n > kMaxPatch) {
+      wk += 1;
+      wn3 += n * n * n;
+      wn2 += n * n;
+      if (po[k + 1] - po[k] > kHeavyPairs)
+        hnm += n * (double)m;
+      else
+        wnm += n * (double)m;
+    }
   }
   w[0] = (double)K_;
   w[1] = sn;
@@ -314,6 +328,11 @@ void RbfPumPlan::work(double *w) const {
   w[4] = snm;
   w[5] = (double)pairs_.P;
   w[6] = (double)maxN_;
+  w[7] = wk;
+  w[8] = wn3;
+  w[9] = wn2;
+  w[10] = wnm;
+  w[11] = hnm;
 }
 
 // Buffers and size bins an apply of F fields needs (allocations happen here,
@@ -702,7 +721,7 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     if (!wsHost_.empty()) {
       if (f0 > 0)
         launch_ws_reset(counters_.get(), false, s);
-      launch_workspace(a, s);
+      launch_workspace(a, s, events && f0 == 0);
     }
     launch_heavy_eval(prm_.kind, a, coef_.get(), heavyChunks_.get(),
                       nHeavyChunks_, s);
@@ -944,7 +963,8 @@ void RbfPumPlan::plan_heavy(hipStream_t s) {
   nHeavyChunks_ = (int)chunks.size();
 }
 
-void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s) {
+void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s,
+                                  bool timed) {
   const bool mixed = prm_.precision == 1;
   auto args = [&](const WsQueue &q) {
     WsArgs w{};
@@ -964,6 +984,8 @@ void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s) {
     w.phi0 = kPhi0[prm_.kind];
     return w;
   };
+  if (timed)
+    PACMAN_HIP_CHECK(hipEventRecord(ev_[5], s));
   for (int i = 0; i < 2; ++i) {
     const WsQueue &q = wsQ_[i];
     if (q.host.empty())
@@ -975,6 +997,9 @@ void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s) {
     w.a.count = (int)q.host.size();
     launch_ws_solve(mixed, prm_.kind, w, q.slots, s);
   }
+  if (timed)
+    PACMAN_HIP_CHECK(hipEventRecord(ev_[6], s));
+  wsTimed_ = wsTimed_ || timed;
   if (mixed) {
     // FP64 pass over the patches the refinement handed back (the list
     // length is read on the device; an empty list ends every workgroup at

@@ -96,12 +96,17 @@ public:
   // Phase times in ms. Setup phases are host-timed (each ends with a
   // synchronisation): 0 grids, 1 radius, 2 centres, 3 membership, 4 setup
   // total. Apply phases come from HIP events of the last apply on its
-  // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total.
-  static constexpr int kPhases = 10;
+  // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total,
+  // 9 polynomial fit of the first field chunk, 10 first-pass workspace
+  // launches (patches above kMaxPatch) of the first field chunk.
+  static constexpr int kPhases = 11;
   void phaseTimes(double *ms) const;
   // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
-  // targets), 5 pairs (targets within r of a centre), 6 max n.
-  static constexpr int kWork = 7;
+  // targets), 5 pairs (targets within r of a centre), 6 max n; over the
+  // patches above kMaxPatch: 7 count, 8 sum n^3, 9 sum n^2, 10 sum n m of
+  // those evaluated by the workspace kernel, 11 sum n m of those evaluated
+  // in chunks (more than kHeavyPairs pairs).
+  static constexpr int kWork = 12;
   void work(double *w) const;
   ~RbfPumPlan();
 
@@ -151,7 +156,7 @@ private:
   DevBuf<int2> heavyChunks_;
   int nHeavyChunks_ = 0;
   void require_register_patches(const char *what) const;
-  void launch_workspace(const SolveArgs &a, hipStream_t s);
+  void launch_workspace(const SolveArgs &a, hipStream_t s, bool timed);
   std::vector<int> patchN_;  // sources per patch (host)
   const Bins &bins_for(int F, hipStream_t s);
   void prepare(int F, hipStream_t s);
@@ -196,7 +201,9 @@ private:
   DevBuf<int> tinv_;  // original target index -> cell order (operator apply)
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};
-  hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_[7] = {nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr};
+  bool wsTimed_ = false;  // ev_[5], ev_[6] recorded by the last apply
   bool applied_ = false;
 };
 

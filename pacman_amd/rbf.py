@@ -442,9 +442,10 @@ class Plan:
         return off, idx[:size]
 
     PHASES = ("grids", "radius", "centres", "membership", "setup",
-              "gather", "solve", "blend", "apply", "poly")
+              "gather", "solve", "blend", "apply", "poly", "workspace")
     WORK = ("patches", "sum_n", "sum_n2", "sum_n3", "sum_nm", "pairs",
-            "max_n")
+            "max_n", "large", "large_sum_n3", "large_sum_n2", "large_sum_nm",
+            "heavy_sum_nm")
 
     def phase_times(self) -> dict:
         """Milliseconds per phase (setup: host clock; apply: HIP events)."""
