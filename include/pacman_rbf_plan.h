@@ -56,6 +56,20 @@ int pacman_rbf_plan_apply_graph(pacman_rbf_plan plan,
                                 const double *sourceValues, int nFields,
                                 double *targetValues, void *stream);
 
+/* pacman_rbf_plan_apply on device pointers, asynchronous on `stream`, that
+ * allocates nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The per-call scratch lives in a
+ * static 64 MiB device array per GPU (held until the process ends); a call
+ * that needs more (about 8 (F N + F_c pairs + 4 F_s K) bytes) returns -1
+ * with the size in the error text. Field counts above 1 need one earlier
+ * allocating apply of that count on the plan (it builds the per-count size
+ * bins and the target permutation). The plan must belong to the current
+ * device. Calls that overlap on one device share the scratch and race. */
+int pacman_rbf_plan_apply_noalloc(pacman_rbf_plan plan,
+                                  const double *sourceValues, int nFields,
+                                  double *targetValues, void *stream);
+
 /* Target-coordinate gradient (no reference counterpart). Doubles of solver
  * state that pacman_rbf_plan_apply_state saves for nFields fields (the
  * coefficients of every patch), or -1 on error. */

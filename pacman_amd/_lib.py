@@ -41,6 +41,8 @@ SIGNATURES = {
         _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
     "pacman_rbf_plan_apply_graph": (ctypes.c_int, [
         _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "pacman_rbf_plan_apply_noalloc": (ctypes.c_int, [
+        _vp, _vp, ctypes.c_int, _vp, _vp]),
     "pacman_rbf_plan_state_size": (ctypes.c_longlong, [_vp, ctypes.c_int]),
     "pacman_rbf_plan_apply_state": (ctypes.c_int, [
         _vp, _vp, ctypes.c_int, _vp, _vp, _vp]),

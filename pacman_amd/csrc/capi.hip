@@ -18,6 +18,7 @@
 #include <exception>
 #include <iostream>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "../../include/fortran_interface.h"
@@ -39,9 +40,46 @@ struct pacman_mls_plan_s {
   std::unique_ptr<pacman_amd::MlsPlan> plan;
 };
 
+// Scratch of pacman_rbf_plan_apply_noalloc: the sorted source values, the
+// (target, patch) contributions and the polynomial coefficients of one call.
+// Every device the library runs on holds its own copy from the load of this
+// code object until the process ends. In host code `scratch` is a stand-in,
+// not a device address: host code uses only the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch()).
+// Every element a call reads was written earlier in the same call, so
+// nothing has to start at zero; calls that overlap on one device share the
+// array and race.
+constexpr int64_t kNoAllocScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / 8];
+
 namespace {
 
 thread_local std::string g_lastError;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+double *g_scratchPtr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+double *noalloc_scratch() {
+  int dev = -1;
+  PACMAN_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    throw std::runtime_error("device index " + std::to_string(dev) +
+                             " out of range");
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      throw std::runtime_error(
+          std::string("hipGetSymbolAddress(scratch) failed: ") +
+          hipGetErrorString(e));
+    g_scratchPtr[dev] = static_cast<double *>(p);
+  }
+  return g_scratchPtr[dev];
+}
 
 void set_error(const char *where, const std::string &msg) {
   g_lastError = std::string(where) + ": " + msg;
@@ -299,8 +337,14 @@ MLSInterpolateResult MLS_interpolate(int_t spaceDimension,
 extern "C" {
 
 void pacman_kokkos_initialize_c(void) {
-  if (device_present())
+  if (device_present()) {
     (void)hipFree(nullptr);  // creates the device context
+    try {
+      (void)noalloc_scratch();  // the current device's scratch address
+    } catch (const std::exception &e) {
+      set_error("pacman_kokkos_initialize_c", e.what());
+    }
+  }
 }
 
 void pacman_kokkos_finalize_c(void) {}
@@ -558,6 +602,30 @@ int pacman_rbf_plan_apply_graph(pacman_rbf_plan plan,
     return -1;
   } catch (...) {
     set_error("pacman_rbf_plan_apply_graph", "unknown exception");
+    return -2;
+  }
+}
+
+int pacman_rbf_plan_apply_noalloc(pacman_rbf_plan plan,
+                                  const double *sourceValues, int nFields,
+                                  double *targetValues, void *stream) {
+  try {
+    if (!plan || !plan->plan)
+      throw std::invalid_argument("plan is null");
+    RbfPumPlan &P = *plan->plan;
+    if (nFields < 1)
+      throw std::invalid_argument("nFields must be >= 1");
+    if (sourceValues == nullptr || (targetValues == nullptr && P.numTargets() > 0))
+      throw std::invalid_argument("null values pointer");
+    double *sp = noalloc_scratch();
+    P.applyNoAlloc(sourceValues, nFields, targetValues, (hipStream_t)stream,
+                   sp, kNoAllocScratchBytes / 8);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_apply_noalloc", e.what());
+    return -1;
+  } catch (...) {
+    set_error("pacman_rbf_plan_apply_noalloc", "unknown exception");
     return -2;
   }
 }

@@ -63,6 +63,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   for (int64_t kk = 0; kk < K_; ++kk)
     patchN_[kk] = (int)(off[kk + 1] - off[kk]);
   totalSrc_ = off[K_];
+  PACMAN_HIP_CHECK(hipGetDevice(&device_));
   counters_.alloc(kWsCounters);
   launch_ws_reset(counters_.get(), true, s);
   plan_workspace(s);
@@ -380,6 +381,65 @@ void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
   applied_ = true;
 }
 
+namespace {
+int64_t up256(int64_t doubles) { return (doubles + 31) / 32 * 32; }
+} // namespace
+
+int64_t RbfPumPlan::noAllocScratch(int F) const {
+  if (F < 1)
+    throw std::invalid_argument("apply needs at least one field");
+  const int perSolve = solve_fields_per_launch();
+  const int Fc = std::min(F, opBuilt_ ? kOpFMax : perSolve);
+  const int Fs = std::min(F, perSolve);
+  return up256((int64_t)F * N_) +
+         up256(std::max<int64_t>(pairs_.P, 1) * Fc) +
+         up256(std::max<int64_t>(K_, 1) * Fs * 4);
+}
+
+// The buffers prepare() would grow come from the caller's scratch; the
+// per-field-count state that prepare() builds must already exist.
+void RbfPumPlan::applyNoAlloc(const double *dVals, int F, double *dOut,
+                              hipStream_t s, double *scratch,
+                              int64_t capacity) {
+  const int64_t need = noAllocScratch(F);
+  int dev = -1;
+  PACMAN_HIP_CHECK(hipGetDevice(&dev));
+  if (dev != device_)
+    throw std::invalid_argument(
+        "the plan was built on device " + std::to_string(device_) +
+        " but device " + std::to_string(dev) + " is current");
+  if (need > capacity)
+    throw std::length_error(
+        std::to_string(F) + " fields over this plan need " +
+        std::to_string((need * 8 + 1048575) / 1048576) +
+        " MiB of scratch; the allocation-free apply holds " +
+        std::to_string(capacity * 8 / 1048576) + " MiB");
+  if (M_ == 0)
+    return;
+  if (!opBuilt_) {
+    const int perSolve = solve_fields_per_launch();
+    for (int f0 = 0; f0 < F; f0 += perSolve) {
+      const int nf = std::min(perSolve, F - f0);
+      if (!bins_[nf - 1].built)
+        throw std::logic_error(
+            "the plan has no size bins for " + std::to_string(nf) +
+            " fields per solve; one allocating apply of " +
+            std::to_string(F) + " fields builds them");
+    }
+  }
+  if (F >= 2 && (tinv_.n == 0 || wslot_.n == 0))
+    throw std::logic_error(
+        "the multi-field blend needs the target permutation and slot "
+        "weights; one allocating apply of two or more fields builds them");
+  const int64_t nv = up256((int64_t)F * N_);
+  const int perSolve = solve_fields_per_launch();
+  const int Fc = std::min(F, opBuilt_ ? kOpFMax : perSolve);
+  const int64_t nc = up256(std::max<int64_t>(pairs_.P, 1) * Fc);
+  const ApplyBufs bu{scratch, scratch + nv, scratch + nv + nc};
+  enqueue(dVals, F, dOut, s, true, nullptr, &bu);
+  applied_ = true;
+}
+
 // The apply sequence as a HIP graph, captured on first use for these
 // pointers and field count and replayed afterwards on a private stream
 // ordered after `s` (before and after) through events. No allocation,
@@ -675,7 +735,10 @@ OpStore RbfPumPlan::opStore() const {
 }
 
 void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
-                         hipStream_t s, bool events, double *dState) {
+                         hipStream_t s, bool events, double *dState,
+                         const ApplyBufs *bufs) {
+  const ApplyBufs bu =
+      bufs ? *bufs : ApplyBufs{valsSorted_.get(), contrib_.get(), coef_.get()};
   const bool useOp = opBuilt_ && dState == nullptr;
   // the stored operator takes up to kOpFMax fields per pass over it
   const int perLaunch = useOp ? kOpFMax : solve_fields_per_launch();
@@ -683,63 +746,68 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[0], s));
   launch_ws_reset(counters_.get(), true, s);
-  launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
+  launch_gather_values(dVals, gs_.view.perm, N_, F, bu.vals, s);
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[1], s));
-  std::vector<BlendArgs> blends;
+  // a single chunk blends after all solves (phases stay separable)
+  BlendArgs pending{};
+  bool havePending = false;
   for (int f0 = 0; f0 < F; f0 += perLaunch) {
     const int nf = std::min(perLaunch, F - f0);
-    SolveArgs a = solveArgs(nf, valsSorted_.get() + (int64_t)f0 * N_);
+    SolveArgs a = solveArgs(nf, bu.vals + (int64_t)f0 * N_);
+    a.contrib = bu.contrib;
+    BlendArgs bl = blendArgs(nf);
+    bl.contrib = bu.contrib;
+    bl.out = dOut + (int64_t)f0 * M_;
     if (useOp) {
       if (f0 == 0 && events)
         PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
       launch_op_apply(a, K_, opOff_.get(), opStore(), s);
-      BlendArgs bl = blendArgs(nf);
-      bl.out = dOut + (int64_t)f0 * M_;
-      if (F <= perLaunch)
-        blends.push_back(bl);
-      else
+      if (F <= perLaunch) {
+        pending = bl;
+        havePending = true;
+      } else {
         launch_blend_sum(bl, s);
+      }
       continue;
     }
-    launch_poly(a, K_, coef_.get(), s);
+    launch_poly(a, K_, bu.coef, s);
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[4], s));
     if (dState) {
       a.alphaOut = dState + (int64_t)f0 * total;
       if (K_ > 0)
         PACMAN_HIP_CHECK(hipMemcpyAsync(
-            dState + (int64_t)F * total + (int64_t)f0 * K_ * 4, coef_.get(),
+            dState + (int64_t)F * total + (int64_t)f0 * K_ * 4, bu.coef,
             sizeof(double) * (size_t)K_ * nf * 4, hipMemcpyDeviceToDevice, s));
     }
     const Bins &B = bins_[nf - 1];
     for (size_t b = 0; b < B.R.size(); ++b) {
       a.patches = B.dev[b].get();
       a.count = B.count[b];
-      launch_solve_f64(B.R[b], prm_.kind, a, coef_.get(), s);
+      launch_solve_f64(B.R[b], prm_.kind, a, bu.coef, s);
     }
     if (!wsHost_.empty()) {
       if (f0 > 0)
         launch_ws_reset(counters_.get(), false, s);
-      launch_workspace(a, s, events && f0 == 0);
+      launch_workspace(a, bu.coef, s, events && f0 == 0);
     }
-    launch_heavy_eval(prm_.kind, a, coef_.get(), heavyChunks_.get(),
+    launch_heavy_eval(prm_.kind, a, bu.coef, heavyChunks_.get(),
                       nHeavyChunks_, s);
-    BlendArgs bl = blendArgs(nf);
-    bl.out = dOut + (int64_t)f0 * M_;
     if (F <= perLaunch) {
-      blends.push_back(bl);  // blend after all solves: phases stay separable
+      pending = bl;
+      havePending = true;
     } else {
       launch_blend(bl, s);
     }
   }
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[2], s));
-  for (const BlendArgs &bl : blends) {
+  if (havePending) {
     if (useOp)
-      launch_blend_sum(bl, s);
+      launch_blend_sum(pending, s);
     else
-      launch_blend(bl, s);
+      launch_blend(pending, s);
   }
   if (events)
     PACMAN_HIP_CHECK(hipEventRecord(ev_[3], s));
@@ -963,13 +1031,13 @@ void RbfPumPlan::plan_heavy(hipStream_t s) {
   nHeavyChunks_ = (int)chunks.size();
 }
 
-void RbfPumPlan::launch_workspace(const SolveArgs &a, hipStream_t s,
-                                  bool timed) {
+void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,
+                                  hipStream_t s, bool timed) {
   const bool mixed = prm_.precision == 1;
   auto args = [&](const WsQueue &q) {
     WsArgs w{};
     w.a = a;
-    w.coef = coef_.get();
+    w.coef = coef;
     w.nbMax = q.nbMax;
     w.work = wsWork_.get();
     w.slotBlocks = ws_slot_blocks(q.nbMax);

@@ -40,6 +40,18 @@ public:
   // The same as a HIP graph replay (captured on the first call for these
   // pointers and F; asynchronous on s). For repeated transfers.
   void applyGraph(const double *dVals, int F, double *dOut, hipStream_t s);
+  // apply with the per-call scratch (sorted source values, (target, patch)
+  // contributions, polynomial coefficients) carved from `scratch` (device,
+  // `capacity` doubles, 256-byte aligned) instead of plan buffers: nothing
+  // is allocated. Throws when the scratch is too small, when the plan is on
+  // another device than the current one, or when the plan lacks the
+  // per-field-count bins or (F >= 2) the target permutation and slot weights
+  // that a first apply of that field count builds.
+  void applyNoAlloc(const double *dVals, int F, double *dOut, hipStream_t s,
+                    double *scratch, int64_t capacity);
+  // Doubles of scratch applyNoAlloc needs for F fields.
+  int64_t noAllocScratch(int F) const;
+  int device() const { return device_; }
   // Transpose of apply (the gradient with respect to the source values):
   // dG: F x M (original target order), dGrad: F x N (original source order).
   void applyTranspose(const double *dG, int F, double *dGrad, hipStream_t s);
@@ -156,12 +168,20 @@ private:
   DevBuf<int2> heavyChunks_;
   int nHeavyChunks_ = 0;
   void require_register_patches(const char *what) const;
-  void launch_workspace(const SolveArgs &a, hipStream_t s, bool timed);
+  void launch_workspace(const SolveArgs &a, const double *coef, hipStream_t s,
+                        bool timed);
   std::vector<int> patchN_;  // sources per patch (host)
   const Bins &bins_for(int F, hipStream_t s);
   void prepare(int F, hipStream_t s);
+  // per-call buffers of an apply: F x N sorted source values, pairs x Fc
+  // contributions, K x Fs x 4 polynomial coefficients
+  struct ApplyBufs {
+    double *vals, *contrib, *coef;
+  };
   void enqueue(const double *dVals, int F, double *dOut, hipStream_t s,
-               bool events, double *dState = nullptr);
+               bool events, double *dState = nullptr,
+               const ApplyBufs *bufs = nullptr);
+  int device_ = 0;
   void releaseGraph();
   // Grows a buffer that enqueue() reads or writes; a reallocation bumps the
   // generation that a captured graph was recorded against, so the next graph
