@@ -13,8 +13,9 @@
 //      extra rows, as in the register kernel), A_ij = phi(|x_i - x_j| / delta)
 //      evaluated in T;
 //   3. factor A = L D L^T by 32-column panels: one wave eliminates the
-//      diagonal block in FP64 with the identity beside it (which becomes
-//      Linv = L_pp^-1; pivots d), every wave turns the rows below into
+//      diagonal block in FP64 registers (lane j holds column j of
+//      [A_pp | I]; the identity becomes Linv = L_pp^-1; pivots d), every
+//      wave turns the rows below into
 //      L_Ip = A_Ip Linv^T D^-1 and applies A_IJ -= (L_Ip D) L_Jp^T to the
 //      trailing blocks, both on 16 x 16 x 4 matrix cores in T. This is the
 //      block form of the reference's unpivoted Doolittle LU
@@ -121,19 +122,14 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
   using V = typename M::V;
   const SolveArgs &a = w.a;
   __shared__ int sItem, sFlag;
-  __shared__ double emul[TB];            // multipliers of the current pivot
   __shared__ T linv[TB][TB + 1];      // Linv of the current panel
   __shared__ double sd[TB], sdinv[TB];
   __shared__ double scoef[kFMax][4];
   __shared__ double red[NW];
   constexpr int LR = SMALL ? kLdsRows : 1;
-  constexpr int LA = SMALL ? kFMax * kLdsRows : TB * (2 * TB + 1);
+  constexpr int LA = SMALL ? kFMax * kLdsRows : 1;
   __shared__ double lxyz[3][LR];
   __shared__ double lal[LA];
-  // [A_pp | I] during each elimination: shares lal (alpha is written only
-  // after the factorisation)
-  static_assert(TB * (2 * TB + 1) <= LA, "elimination buffer");
-  double(*eb)[2 * TB + 1] = reinterpret_cast<double(*)[2 * TB + 1]>(lal);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int np = w.nbMax * TB;  // vector stride of the slot
@@ -242,42 +238,47 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
       const int kk = min(TB, n - TB * p);
       const int64_t bpp = blk(p, p);
       if (wave == 0) {
-        // [A_pp | I] in LDS; lane j owns column j of it (0..31: the block,
-        // 32..63: the identity, which the row operations turn into Linv)
+        // [A_pp | I] in registers: lane j owns column j of it (0..31: the
+        // block, 32..63: the identity, which the row operations turn into
+        // Linv), e[r] = row r. Pivot k: the multipliers A_ik / d come from
+        // lane k (readlane), the pivot row is each lane's own e[k]; every
+        // index is a compile-time constant.
+        double e[TB];
+#pragma unroll
         for (int r = 0; r < TB; ++r)
-          eb[r][lane] = lane < TB ? (double)W[bpp + r * TB + lane]
-                                  : (lane - TB == r ? 1.0 : 0.0);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int k = 0; k < kk; ++k) {
-          const double d = eb[k][k];
-          if (lane < TB)
-            emul[lane] = (lane > k && lane < kk) ? eb[lane][k] / d : 0.0;
-          if (lane == 0) {
-            sd[k] = d;
-            sdinv[k] = 1.0 / d;
-            if (!(d > kPivotTol * w.phi0))
-              sFlag = 1;
+          e[r] = lane < TB ? (double)W[bpp + r * TB + lane]
+                           : (lane - TB == r ? 1.0 : 0.0);
+        bool bad = false;
+        static_for<0, TB>([&](auto KC) {
+          constexpr int k = decltype(KC)::value;
+          if (k < kk) {
+            const double d = readlane_f64(e[k], k);
+            const double dinv = 1.0 / d;
+            bad = bad || !(d > kPivotTol * w.phi0);
+            if (lane == 0) {
+              sd[k] = d;
+              sdinv[k] = dinv;
+            }
+            static_for<k + 1, TB>([&](auto IC) {
+              constexpr int i = decltype(IC)::value;
+              if (i < kk) {
+                const double m = readlane_f64(e[i], k) * dinv;
+                e[i] = fma(-m, e[k], e[i]);
+              }
+            });
           }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          const double pk = eb[k][lane];
-          for (int i = k + 1; i < kk; ++i)
-            eb[i][lane] = fma(-emul[i], pk, eb[i][lane]);
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+        });
+        if (lane == 0 && bad)
+          sFlag = 1;
         if (lane < TB && lane >= kk) {
           sd[lane] = 1.0;
           sdinv[lane] = 0.0;
         }
         if (lane >= TB) {
           const int c = lane - TB;
+#pragma unroll
           for (int i = 0; i < TB; ++i) {
-            const T val = (i < kk && c < kk) ? (T)eb[i][lane] : T(0);
+            const T val = (i < kk && c < kk) ? (T)e[i] : T(0);
             linv[i][c] = val;
             Li[(int64_t)p * TB * TB + i * TB + c] = val;
           }
