@@ -101,21 +101,28 @@ __device__ __forceinline__ double phi64(double xi, double yi, double zi,
   return wendland<K, double, true>(dist_fast(xi, yi, zi, xj, yj, zj), dinv);
 }
 
-__device__ double block_max(double v, double *red) {
+// Workgroup maxima of two values (every thread gets both).
+__device__ void block_max2(double &u, double &v, double (*red)[2]) {
+  u = wave_max(u);
   v = wave_max(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  if (lane == 0)
-    red[wave] = v;
+  if (lane == 0) {
+    red[wave][0] = u;
+    red[wave][1] = v;
+  }
   __syncthreads();
-  double m = red[0];
-  for (int w = 1; w < NW; ++w)
-    m = fmax(m, red[w]);
-  return m;
+  u = red[0][0];
+  v = red[0][1];
+  for (int w = 1; w < NW; ++w) {
+    u = fmax(u, red[w][0]);
+    v = fmax(v, red[w][1]);
+  }
 }
 
-// SMALL: the bordered matrix has at most kLdsRows rows, so coordinates and
-// alpha live in LDS; otherwise in the slot's vectors in device memory.
+// SMALL: the bordered matrix has at most kLdsRows rows, so coordinates,
+// alpha, the pivots and the solve vectors live in LDS (about 138 KB for
+// T = float); otherwise in the slot's vectors in device memory.
 template <typename T, int K, bool SMALL>
 __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
   using M = Mma<T>;
@@ -125,11 +132,16 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
   __shared__ T linv[TB][TB + 1];      // Linv of the current panel
   __shared__ double sd[TB], sdinv[TB];
   __shared__ double scoef[kFMax][4];
-  __shared__ double red[NW];
+  __shared__ double red[NW][2];
+  constexpr bool FP32 = sizeof(T) == 4;
   constexpr int LR = SMALL ? kLdsRows : 1;
   constexpr int LA = SMALL ? kFMax * kLdsRows : 1;
+  constexpr int LZ = SMALL && FP32 ? kFMax * kLdsRows : 1;
   __shared__ double lxyz[3][LR];
-  __shared__ double lal[LA];
+  __shared__ double lal[LA];   // alpha
+  __shared__ double lsv[LA];   // solve vector
+  __shared__ double lzv[LZ];   // correction; residual partial sums before
+  __shared__ double ldv[LR];   // pivots
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int np = w.nbMax * TB;  // vector stride of the slot
@@ -142,7 +154,12 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
   double *sv = al + (int64_t)kFMax * np;      // [kFMax][np] work vector
   double *zv = sv + (int64_t)kFMax * np;      // [kFMax][np] work vector
   const int count = w.countPtr ? *w.countPtr : a.count;
-  constexpr bool FP32 = sizeof(T) == 4;
+  // solve vectors: [F][vst] with stride vst, pivots vd
+  double *const vs = SMALL ? lsv : sv;
+  double *const vz = SMALL && FP32 ? lzv : zv;
+  double *const vd = SMALL ? ldv : dv;
+  constexpr int vstC = SMALL ? kLdsRows : 0;
+  const int vst = SMALL ? vstC : np;
 
   for (;;) {
     if (tid == 0)
@@ -286,7 +303,7 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
       }
       __syncthreads();
       if (tid < kk)
-        dv[TB * p + tid] = sd[tid];
+        vd[TB * p + tid] = sd[tid];
       // rows below the pivots: L = A Linv^T D^-1, one 32-row block per wave
       // step, every operand load issued before the first product
       const int firstRow = kk < TB ? p : p + 1;
@@ -391,13 +408,14 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
       return (double)W[blk(i / TB, c / TB) + (i % TB) * TB + (c % TB)];
     };
     // L^T x = s (s destroyed), block by block from the bottom; x = dst.
-    auto back_subst = [&](double *s, double *dst, int dstStride) {
+    // Vectors are [F][stride].
+    auto back_subst = [&](double *s, int sStride, double *dst, int dstStride) {
       for (int B = P - 1; B >= 0; --B) {
         const int kb = min(TB, n - TB * B);
         const T *LB = Li + (int64_t)B * TB * TB;
         for (int t = tid; t < F * TB; t += NT) {
           const int f = t / TB, cl = t % TB;
-          const double *sb = s + (int64_t)f * np + TB * B;
+          const double *sb = s + (int64_t)f * sStride + TB * B;
           double acc = 0.0;
           // Linv is lower triangular with zeros past kb: every load is in
           // range and the masked terms add exact zeros
@@ -419,25 +437,26 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
           for (int r = 0; r < TB; ++r)
             acc = fma(r < kb ? (double)col[r * TB] : 0.0, r < kb ? xb[r] : 0.0,
                       acc);
-          s[(int64_t)f * np + c] -= acc;
+          s[(int64_t)f * sStride + c] -= acc;
         }
         __syncthreads();
       }
     };
-    // L x = s (s destroyed), block by block from the top; x = dst.
-    auto fwd_subst = [&](double *s, double *dst) {
+    // L x = s (s destroyed), block by block from the top; x = dst (both
+    // [F][stride]).
+    auto fwd_subst = [&](double *s, double *dst, int stride) {
       for (int B = 0; B < P; ++B) {
         const int kb = min(TB, n - TB * B);
         const T *LB = Li + (int64_t)B * TB * TB;
         for (int t = tid; t < F * TB; t += NT) {
           const int f = t / TB, cl = t % TB;
-          const double *sb = s + (int64_t)f * np + TB * B;
+          const double *sb = s + (int64_t)f * stride + TB * B;
           double acc = 0.0;
 #pragma unroll 8
           for (int c2 = 0; c2 < TB; ++c2)
             acc = fma((double)LB[cl * TB + c2], c2 < kb ? sb[c2] : 0.0, acc);
           if (cl < kb)
-            dst[(int64_t)f * np + TB * B + cl] = acc;
+            dst[(int64_t)f * stride + TB * B + cl] = acc;
         }
         __syncthreads();
         const int r0 = TB * (B + 1), rows = n - r0;
@@ -445,12 +464,12 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
           const int f = t / rows, i = r0 + t % rows;
           // row i of the block column B: L[i][TB B + c], c < kb
           const T *row = W + blk(i / TB, B) + (i % TB) * TB;
-          const double *xb = dst + (int64_t)f * np + TB * B;
+          const double *xb = dst + (int64_t)f * stride + TB * B;
           double acc = 0.0;
 #pragma unroll 8
           for (int c = 0; c < TB; ++c)
             acc = fma(c < kb ? (double)row[c] : 0.0, c < kb ? xb[c] : 0.0, acc);
-          s[(int64_t)f * np + i] -= acc;
+          s[(int64_t)f * stride + i] -= acc;
         }
         __syncthreads();
       }
@@ -459,10 +478,10 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
     // 4. first solve: y_f = bordered row n + f
     for (int t = tid; t < F * n; t += NT) {
       const int f = t / n, c = t % n;
-      sv[(int64_t)f * np + c] = Lat(n + f, c);
+      vs[(int64_t)f * vst + c] = Lat(n + f, c);
     }
     __syncthreads();
-    back_subst(sv, alp, ast);
+    back_subst(vs, vst, alp, ast);
 
     int status = 0;
     if constexpr (FP32) {
@@ -473,42 +492,63 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
       bool conv = false;
       while (ok && !conv && iters < w.maxIR) {
         ++iters;
-        for (int i = tid; i < n; i += NT) {
+        // residual r = b - A alpha (phi in FP64); in LDS the j range of
+        // every row is split over S parts so that all waves work, and the
+        // parts' sums (in vz, free until the forward solve) are added in
+        // part order: S F n <= max(NT F, 8 n) <= LZ
+        const int S = SMALL ? max(1, NT / n) : 1;
+        const int chunk = (n + S - 1) / S;
+        for (int u = tid; u < n * S; u += NT) {
+          const int i = u % n, part = u / n;
+          const int j0 = part * chunk, j1 = min(n, j0 + chunk);
           double acc[kFMax];
 #pragma unroll
           for (int f = 0; f < kFMax; ++f)
             acc[f] = 0.0;
           const double xi = px[i], yi = py[i], zi = pz[i];
 #pragma unroll 4
-          for (int j = 0; j < n; ++j) {
+          for (int j = j0; j < j1; ++j) {
             const double ph = phi64<K>(xi, yi, zi, px[j], py[j], pz[j], dinv);
 #pragma unroll
             for (int f = 0; f < kFMax; ++f)
               if (f < F)
                 acc[f] = fma(ph, alp[f * ast + j], acc[f]);
           }
-          for (int f = 0; f < F; ++f)
-            sv[(int64_t)f * np + i] = bv[(int64_t)f * np + i] - acc[f];
+          for (int f = 0; f < F; ++f) {
+            if constexpr (SMALL)
+              vz[((int64_t)part * F + f) * n + i] = acc[f];
+            else
+              vs[(int64_t)f * vst + i] = bv[(int64_t)f * np + i] - acc[f];
+          }
         }
         __syncthreads();
-        fwd_subst(sv, zv);
+        if constexpr (SMALL) {
+          for (int t = tid; t < F * n; t += NT) {
+            const int f = t / n, i = t % n;
+            double sum = 0.0;
+            for (int part = 0; part < S; ++part)
+              sum += vz[((int64_t)part * F + f) * n + i];
+            vs[(int64_t)f * vst + i] = bv[(int64_t)f * np + i] - sum;
+          }
+          __syncthreads();
+        }
+        fwd_subst(vs, vz, vst);
         for (int t = tid; t < F * n; t += NT) {
           const int f = t / n, c = t % n;
-          sv[(int64_t)f * np + c] = zv[(int64_t)f * np + c] / dv[c];
+          vs[(int64_t)f * vst + c] = vz[(int64_t)f * vst + c] / vd[c];
         }
         __syncthreads();
-        back_subst(sv, zv, np);
+        back_subst(vs, vst, vz, vst);
         double dmax = 0.0, amax = 0.0;
         for (int t = tid; t < F * n; t += NT) {
           const int f = t / n, c = t % n;
-          const double dl = zv[(int64_t)f * np + c];
+          const double dl = vz[(int64_t)f * vst + c];
           const double x = alp[f * ast + c] + dl;
           alp[f * ast + c] = x;
           dmax = fmax(dmax, fabs(dl));
           amax = fmax(amax, fabs(x));
         }
-        dmax = block_max(dmax, red);
-        amax = block_max(amax, red);
+        block_max2(dmax, amax, red);
         if (!(dmax <= 1e300) || !(amax <= 1e300)) {
           ok = false;
         } else if (dmax == 0.0) {
