@@ -21,7 +21,9 @@
 //     the bits of the squared distances over an exact candidate set);
 //  4. patches holding more than nmax sources are halved again;
 //  5. patches sorted by cell key; members: sources within r_k, targets
-//     within r_k (weights) and r_k - 1e-14 (members).
+//     within r_k (weights) and r_k - 1e-14 (members); the few target balls
+//     of sparse regions expected to hold more than 16384 targets are scanned
+//     by a workgroup each instead of a wave.
 //
 // Ball queries read the sorted codes: the 3^D cells around the centre at
 // the deepest depth whose cells are at least (1 + 1e-9) r wide on every
@@ -332,26 +334,53 @@ __device__ inline double cell_box_dist2(uint64_t p, int d, const Frame &f,
   return s;
 }
 
-constexpr int kRefine = 256;     // longer code ranges are halved first
+// Squared distance from c to the farthest point of the same widened box.
+__device__ inline double cell_box_maxdist2(uint64_t p, int d, const Frame &f,
+                                           const double *c) {
+  double s = 0.0;
+  for (int a = 0; a < f.D; ++a) {
+    const int ba = axis_bits(d, f.D, a);
+    uint64_t ia = 0;
+    for (int t = a; t < d; t += f.D)
+      ia = (ia << 1) | ((p >> (d - 1 - t)) & 1u);
+    const double h = f.ext[a] * ldexp(1.0, -ba);
+    const double pad = 1e-12 * (f.ext[a] + fabs(f.lo[a]));
+    const double blo = f.lo[a] + (double)ia * h - pad;
+    const double bhi = f.lo[a] + (double)(ia + 1) * h + pad;
+    const double e = fmax(fabs(c[a] - blo), fabs(bhi - c[a]));
+    s += e * e;
+  }
+  return s;
+}
+
+constexpr int kRefine = 4096;    // longer code ranges are halved first
 constexpr int kStack = 128;      // pending ranges per wave
 struct ScanNode {
   uint64_t p;
   int d, rs, re;
 };
 
-// fn(position, valid) over a superset of the points of S within r of c (each
-// candidate once; ranges in a fixed order, so the order is deterministic):
-// the 3^D cells around c at the deepest depth whose cells are >= r wide;
-// a range longer than kRefine is halved along the next code bit and the
-// halves whose box lies farther than r from c are dropped.
-template <class Fn>
+// fn(position, valid, inside, x, y, z) over a superset of the points of S
+// within r of c (each candidate once, with its coordinates; ranges in a
+// fixed order, so the order is deterministic): the 3^D cells around c at the
+// deepest depth whose cells are >= r wide; a range longer than kRefine is
+// halved along the next code bit and the halves whose box lies farther than
+// r from c are dropped. A range whose (widened) box lies within
+// r sqrt(1 - 1e-9) of c holds only points within r whatever the rounding of
+// a distance (relative error ~1e-15): such a range of more than 64 points is
+// not halved, whole(rs, re) may take it at once (returning true), else it is
+// scanned with inside = true (INSIDE = false: no such test, for balls that
+// never cover whole ranges). onlyCell >= 0 restricts the scan to that one of
+// the 3^D starting cells (a workgroup shares a large ball out that way).
+template <bool INSIDE = true, class Whole, class Fn>
 __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
                                  const double *c, double r, ScanNode *stk,
-                                 Fn &&fn) {
+                                 Whole &&whole, Fn &&fn, int onlyCell = -1) {
   const int lane = threadIdx.x & 63;
   const int d0 = depth_for(f, r);
   const int nr = f.D == 1 ? 3 : f.D == 2 ? 9 : 27;
   const double r2 = r * r * (1.0 + 1e-9) + 1e-300;
+  const double rin2 = r * r * (1.0 - 1e-9);
   // the neighbour cells, one lane each
   int rs = 0, re = 0;
   uint64_t pref = 0;
@@ -377,7 +406,8 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
     }
   }
   // push the non-empty ones in reverse lane order (popped in lane order)
-  const bool keep = lane < nr && re > rs;
+  const bool keep =
+      lane < nr && re > rs && (onlyCell < 0 || lane == onlyCell);
   const uint64_t km = __ballot(keep);
   const int nk = __popcll(km);
   if (keep) {
@@ -391,7 +421,12 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
   while (top > 0) {
     const ScanNode nd = stk[top - 1];
     --top;
-    if (nd.re - nd.rs > kRefine && nd.d < f.DB && top + 2 <= kStack) {
+    const bool inside = INSIDE && nd.re - nd.rs > 64 &&
+                        cell_box_maxdist2(nd.p, nd.d, f, c) <= rin2;
+    if (inside && whole(nd.rs, nd.re))
+      continue;
+    if (!inside && nd.re - nd.rs > kRefine && nd.d < f.DB &&
+        top + 2 <= kStack) {
       const uint64_t key = (2 * nd.p + 1) << (f.DB - nd.d - 1);
       const int mid = wave_lower_bound(S.code, nd.rs, nd.re, key);
       const ScanNode ch[2] = {{2 * nd.p, nd.d + 1, nd.rs, mid},
@@ -417,17 +452,26 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       continue;
     }
-    for (int b = nd.rs; b < nd.re; b += 64)
-      fn(b + lane, b + lane < nd.re);
+    for (int b = nd.rs; b < nd.re; b += 64) {
+      const int q = b + lane;
+      const int qq = q < nd.re ? q : nd.rs;
+      fn(q, q < nd.re, inside, S.x[qq], S.y[qq], S.z[qq]);
+    }
   }
 }
 
 __device__ inline int ball_count(const SortedSet &S, const Frame &f,
                                  const double *c, double r, ScanNode *stk) {
   int cnt = 0;
-  ball_scan(S, f, c, r, stk, [&](int p, bool valid) {
+  ball_scan(
+      S, f, c, r, stk,
+      [&](int rs, int re) {
+        cnt += re - rs;
+        return true;
+      },
+      [&](int, bool valid, bool inside, double x, double y, double z) {
     const bool hit =
-        valid && dist3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= r;
+        valid && (inside || dist3(c[0], c[1], c[2], x, y, z) <= r);
     cnt += __popcll(__ballot(hit));
   });
   return cnt;
@@ -437,34 +481,109 @@ __device__ inline int ball_count(const SortedSet &S, const Frame &f,
 __device__ inline int count_d2(const SortedSet &S, const Frame &f,
                                const double *c, double t, ScanNode *stk) {
   int cnt = 0;
-  ball_scan(S, f, c, sqrt(t), stk, [&](int p, bool valid) {
+  ball_scan(
+      S, f, c, sqrt(t), stk,
+      [&](int rs, int re) {
+        cnt += re - rs;
+        return true;
+      },
+      [&](int, bool valid, bool inside, double x, double y, double z) {
     const bool hit =
-        valid && sqdiff3(c[0], c[1], c[2], S.x[p], S.y[p], S.z[p]) <= t;
+        valid && (inside || sqdiff3(c[0], c[1], c[2], x, y, z) <= t);
     cnt += __popcll(__ballot(hit));
   });
   return cnt;
 }
 
+constexpr int kKnnCap = 512;  // candidates a wave selects from in LDS
+
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t u = __shfl_xor(v, o);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+
 // Distance to the kq-th nearest point of S from c (kq <= S.n): the smallest
-// t with #{sqdiff <= t} >= kq, bisected on the bits of t (non-negative
-// doubles order as their bit patterns), each count over the pruned ball of
-// radius sqrt(t); the upper end starts at r0^2 and doubles until it holds
-// kq points.
+// t with #{sqdiff <= t} >= kq, i.e. the kq-th smallest squared distance
+// (with multiplicity). The upper end starts at r0^2 and doubles until it
+// holds kq points; while more than kKnnCap candidates lie within it, it is
+// bisected on the bits of t (non-negative doubles order as their bit
+// patterns), each count over the pruned ball of radius sqrt(t). Then the
+// candidates within it are gathered into the wave's LDS buffer (`buf`,
+// kKnnCap doubles) and the kq-th smallest is taken exactly: kq rounds of
+// wave minimum, each removing one instance.
 __device__ inline double knn_distance(const SortedSet &S, const Frame &f,
                                       const double *c, int kq, double r0,
-                                      ScanNode *stk) {
+                                      ScanNode *stk, double *buf) {
   double R = r0 > 0.0 ? r0 : 1e-300;
-  while (count_d2(S, f, c, R * R, stk) < kq)
+  int chi;
+  while ((chi = count_d2(S, f, c, R * R, stk)) < kq)
     R *= 2.0;
   uint64_t lo = 0, hi = (uint64_t)__double_as_longlong(R * R);
-  while (lo < hi) {
+  while (lo < hi && chi > kKnnCap) {
     const uint64_t mid = lo + (hi - lo) / 2;
-    if (count_d2(S, f, c, __longlong_as_double((long long)mid), stk) >= kq)
+    const int cm = count_d2(S, f, c, __longlong_as_double((long long)mid), stk);
+    if (cm >= kq) {
       hi = mid;
-    else
+      chi = cm;
+    } else {
       lo = mid + 1;
+    }
   }
-  return sqrt(__longlong_as_double((long long)hi));
+  if (lo >= hi)
+    return sqrt(__longlong_as_double((long long)hi));
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const double t = __longlong_as_double((long long)hi);
+  int w = 0;
+  ball_scan(
+      S, f, c, sqrt(t), stk, [](int, int) { return false; },
+      [&](int, bool valid, bool inside, double x, double y, double z) {
+    const double q = valid ? sqdiff3(c[0], c[1], c[2], x, y, z) : 0.0;
+    const bool hit = valid && (inside || q <= t);
+    const uint64_t bal = __ballot(hit);
+    const int o = w + __popcll(bal & below);
+    if (hit && o < kKnnCap)
+      buf[o] = q;
+    w += __popcll(bal);
+  });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int nv = min(w, kKnnCap);  // == chi
+  constexpr int kPer = kKnnCap / 64;
+  uint64_t v[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int idx = j * 64 + lane;
+    v[j] = idx < nv ? (uint64_t)__double_as_longlong(buf[idx]) : ~0ull;
+  }
+  // the buffer is read before the next cell's gather writes it
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  uint64_t res = hi;
+  for (int round = 0; round < kq; ++round) {
+    uint64_t lm = v[0];
+#pragma unroll
+    for (int j = 1; j < kPer; ++j)
+      lm = v[j] < lm ? v[j] : lm;
+    const uint64_t m = wave_min_u64(lm);
+    res = m;
+    const uint64_t holders = __ballot(lm == m);
+    if (lane == __ffsll((long long)holders) - 1) {
+      bool done = false;
+#pragma unroll
+      for (int j = 0; j < kPer; ++j)
+        if (!done && v[j] == m) {
+          v[j] = ~0ull;
+          done = true;
+        }
+    }
+  }
+  return sqrt(__longlong_as_double((long long)res));
 }
 
 // One wave per cell: radius, source count; cells within nmax are appended
@@ -494,6 +613,7 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
     return;
   const int lane = threadIdx.x & 63;
   __shared__ ScanNode stacks[kThreads / 64][kStack];
+  __shared__ double knnBuf[kThreads / 64][kKnnCap];
   ScanNode *stk = stacks[threadIdx.x >> 6];
   const uint64_t p = e.cellP[i];
   const int d = e.cellD[i];
@@ -502,7 +622,7 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
   int n = ball_count(e.S, e.f, c, rc, stk);
   double r = rc;
   if (n < e.kq) {
-    r = knn_distance(e.S, e.f, c, e.kq, rc, stk);
+    r = knn_distance(e.S, e.f, c, e.kq, rc, stk, knnBuf[threadIdx.x >> 6]);
     if (r < rc)  // max(rc, dk) exactly
       r = rc;
     n = ball_count(e.S, e.f, c, r, stk);
@@ -565,43 +685,115 @@ struct MemberArgs {
   const double *cx, *cy, *cz, *r;
   int K;
   bool targets;
+  double rHeavy;  // patches with a larger radius: member_heavy_kernel
   int *cnt;
   const int64_t *off;
   int *out, *pairPatch;
 };
 
+// Members of patch k found from the starting cell onlyCell (-1: all), the
+// fill pass writing from position w; returns the position after the last
+// (the count pass: w = 0 and the count). The count pass takes ranges inside
+// the ball at once; the fill pass scans them with every point a hit, so both
+// passes agree by construction.
+template <bool INSIDE>
+__device__ inline int64_t member_scan(const MemberArgs &m, int k, bool fill,
+                                      int64_t w, ScanNode *stk, int onlyCell) {
+  const int lane = threadIdx.x & 63;
+  const double c[3] = {m.cx[k], m.cy[k], m.cz[k]};
+  const double rad = m.r[k], radMember = rad - kEps;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  ball_scan<INSIDE>(
+      m.P, m.f, c, rad, stk,
+      [&](int rs, int re) {
+        if (fill)
+          return false;
+        w += re - rs;
+        return true;
+      },
+      [&](int p, bool valid, bool inside, double x, double y, double z) {
+        double dd = 0.0;
+        if (valid)
+          dd = dist3(c[0], c[1], c[2], x, y, z);
+        const bool hit = valid && (inside || dd <= rad);
+        const uint64_t bal = __ballot(hit);
+        if (fill && hit) {
+          const int64_t o = w + __popcll(bal & below);
+          if (m.targets) {
+            const unsigned member = dd <= radMember ? 1u : 0u;
+            m.out[o] = (int)((unsigned)p | (member << 31));
+            m.pairPatch[o] = k;
+          } else {
+            m.out[o] = p;
+          }
+        }
+        w += __popcll(bal);
+      },
+      onlyCell);
+  return w;
+}
+
+// One wave per patch (the patches above rHeavy are left to
+// member_heavy_kernel). INSIDE: the targets (large balls in sparse regions).
+template <bool INSIDE>
 __global__ __launch_bounds__(kThreads) void member_kernel(MemberArgs m,
                                                           bool fill) {
   const int k = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-  if (k >= m.K)
+  if (k >= m.K || m.r[k] > m.rHeavy)
     return;
-  const int lane = threadIdx.x & 63;
   __shared__ ScanNode stacks[kThreads / 64][kStack];
   ScanNode *stk = stacks[threadIdx.x >> 6];
-  const double c[3] = {m.cx[k], m.cy[k], m.cz[k]};
-  const double rad = m.r[k], radMember = rad - kEps;
-  int64_t w = fill ? m.off[k] : 0;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  ball_scan(m.P, m.f, c, rad, stk, [&](int p, bool valid) {
-    double dd = 0.0;
-    if (valid)
-      dd = dist3(c[0], c[1], c[2], m.P.x[p], m.P.y[p], m.P.z[p]);
-    const bool hit = valid && dd <= rad;
-    const uint64_t bal = __ballot(hit);
-    if (fill && hit) {
-      const int64_t o = w + __popcll(bal & below);
-      if (m.targets) {
-        const unsigned member = dd <= radMember ? 1u : 0u;
-        m.out[o] = (int)((unsigned)p | (member << 31));
-        m.pairPatch[o] = k;
-      } else {
-        m.out[o] = p;
-      }
-    }
-    w += __popcll(bal);
-  });
-  if (!fill && lane == 0)
+  const int64_t w =
+      member_scan<INSIDE>(m, k, fill, fill ? m.off[k] : 0, stk, -1);
+  if (!fill && (threadIdx.x & 63) == 0)
     m.cnt[k] = (int)w;
+}
+
+// Patches with r > rHeavy, listed by heavy_select_kernel (any order: each
+// list entry is handled on its own).
+__global__ void heavy_select_kernel(const double *r, int K, double rHeavy,
+                                    int *list, int *count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K && r[k] > rHeavy)
+    list[atomicAdd(count, 1)] = k;
+}
+
+// Large balls (sparse regions: up to ~2e5 targets): one workgroup per
+// listed patch, its 3^D starting cells dealt over the waves; the count pass
+// keeps a count per starting cell, so the fill pass writes every cell's
+// members where the one-wave scan (cells in order) would.
+constexpr int kHeavyWaves = 16;
+constexpr int kCells = 27;
+__global__ __launch_bounds__(64 * kHeavyWaves) void member_heavy_kernel(
+    MemberArgs m, const int *list, const int *count, int *cellCnt, bool fill) {
+  __shared__ ScanNode stacks[kHeavyWaves][kStack];
+  const int wave = threadIdx.x >> 6;
+  ScanNode *stk = stacks[wave];
+  const int nc = m.f.D == 1 ? 3 : m.f.D == 2 ? 9 : 27;
+  const int hc = *count;
+  for (int h = blockIdx.x; h < hc; h += gridDim.x) {
+    const int k = list[h];
+    int *cc = cellCnt + (int64_t)h * kCells;
+    for (int j = wave; j < nc; j += kHeavyWaves) {
+      int64_t w = 0;
+      if (fill) {
+        w = m.off[k];
+        for (int jj = 0; jj < j; ++jj)
+          w += cc[jj];
+      }
+      w = member_scan<true>(m, k, fill, w, stk, j);
+      if (!fill && (threadIdx.x & 63) == 0)
+        cc[j] = (int)w;
+    }
+    __syncthreads();
+    if (!fill && threadIdx.x == 0) {
+      int64_t t = 0;
+      for (int j = 0; j < nc; ++j)
+        t += cc[j];
+      m.cnt[k] = (int)t;
+    }
+    __syncthreads();
+  }
 }
 
 unsigned blocks(int64_t n, int per) {
@@ -809,13 +1001,14 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   m.K = K;
   DevBuf<int> cnt(K1);
   const unsigned wb = blocks(K, kThreads / 64);
-  // sources
+  // sources (at most nmax per patch: never a large ball)
   m.P = S;
   m.targets = false;
+  m.rHeavy = DBL_MAX;
   m.cnt = cnt.get();
   out.srcOff.alloc(K + 1);
   if (K > 0) {
-    member_kernel<<<wb, kThreads, 0, s>>>(m, false);
+    member_kernel<false><<<wb, kThreads, 0, s>>>(m, false);
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   const int64_t tot = exclusive_scan_i32_to_i64(cnt.get(), out.srcOff.get(), K, s);
@@ -825,16 +1018,36 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   m.off = out.srcOff.get();
   m.out = out.srcIdx.get();
   if (K > 0) {
-    member_kernel<<<wb, kThreads, 0, s>>>(m, true);
+    member_kernel<false><<<wb, kThreads, 0, s>>>(m, true);
     PACMAN_HIP_CHECK(hipGetLastError());
   }
-  // targets
+  // targets; balls expected to hold more than kHeavyPoints targets (from
+  // the mean target density of the frame) go to member_heavy_kernel
   m.P = T;
   m.targets = true;
+  {
+    constexpr double kHeavyPoints = 16384.0;
+    double vol = 1.0;
+    for (int a = 0; a < dim; ++a)
+      vol *= f.ext[a];
+    const double cD = dim == 1 ? 2.0 : dim == 2 ? M_PI : 4.0 * M_PI / 3.0;
+    m.rHeavy = M > 0 ? std::pow(kHeavyPoints * vol / (cD * (double)M),
+                                1.0 / dim)
+                     : DBL_MAX;
+  }
+  DevBuf<int> hList(K1), hCount(1), cellCnt((size_t)K1 * kCells);
+  const unsigned hb = (unsigned)std::min<int64_t>(K1, 1024);
   PairSet &ps = out.pairs;
   ps.off.alloc(K + 1);
   if (K > 0) {
-    member_kernel<<<wb, kThreads, 0, s>>>(m, false);
+    PACMAN_HIP_CHECK(hipMemsetAsync(hCount.get(), 0, sizeof(int), s));
+    heavy_select_kernel<<<blocks(K, kThreads), kThreads, 0, s>>>(
+        out.r.get(), K, m.rHeavy, hList.get(), hCount.get());
+    PACMAN_HIP_CHECK(hipGetLastError());
+    member_kernel<true><<<wb, kThreads, 0, s>>>(m, false);
+    PACMAN_HIP_CHECK(hipGetLastError());
+    member_heavy_kernel<<<hb, 64 * kHeavyWaves, 0, s>>>(
+        m, hList.get(), hCount.get(), cellCnt.get(), false);
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   const int64_t P = exclusive_scan_i32_to_i64(cnt.get(), ps.off.get(), K, s);
@@ -847,7 +1060,10 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   m.out = ps.tgt.get();
   m.pairPatch = pairPatch.get();
   if (K > 0) {
-    member_kernel<<<wb, kThreads, 0, s>>>(m, true);
+    member_kernel<true><<<wb, kThreads, 0, s>>>(m, true);
+    PACMAN_HIP_CHECK(hipGetLastError());
+    member_heavy_kernel<<<hb, 64 * kHeavyWaves, 0, s>>>(
+        m, hList.get(), hCount.get(), cellCnt.get(), true);
     PACMAN_HIP_CHECK(hipGetLastError());
   }
   build_target_slots(ps, pairPatch.get(), M, s);
