@@ -9,7 +9,7 @@ contraction policy:
   must round exactly as written so the patch membership and nearest
   neighbours are bit-identical to the CPU restatement;
 * rbf_solve.hip, rbf_grad.hip and the size-bin units rbf_solve_tile_a.hip
-  ... rbf_solve_tile_d.hip: FMA contraction allowed (dense arithmetic);
+  ... rbf_solve_tile_g.hip: FMA contraction allowed (dense arithmetic);
   rbf_solve.hip also keeps MFMA accumulators in ordinary VGPRs
   (-amdgpu-mfma-vgpr-form: the stored-operator contraction otherwise copies
   them to and from AGPRs every step; 16 fields 7.9 -> 7.4 ms per apply).
@@ -52,6 +52,9 @@ UNITS = {
     "rbf_solve_tile_b.hip": [],
     "rbf_solve_tile_c.hip": [],
     "rbf_solve_tile_d.hip": [],
+    "rbf_solve_tile_e.hip": [],
+    "rbf_solve_tile_f.hip": [],
+    "rbf_solve_tile_g.hip": [],
 }
 
 

@@ -22,8 +22,11 @@
 
 namespace pacman_amd {
 
-// Largest patch (sources) the register-resident solve takes; larger ones go
-// through the workspace solve (rbf_solve_ws.hip).
+// Largest patch (sources) of the 8-wave register-resident kernels: the
+// transpose chain, the stored operator and the target-coordinate gradient
+// take patches up to this size. The forward solve takes patches up to
+// kTileMaxN in its 16-wave bins; larger ones go through the workspace solve
+// (rbf_solve_ws.hip).
 constexpr int kMaxPatch = 192;
 // phi(0) of Wendland C0 .. C8 (rbf_functions.hpp:22-150, not normalised).
 constexpr double kPhi0[5] = {1.0, 1.0, 3.0, 1.0, 15.0};
@@ -265,11 +268,18 @@ void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s);
 // Size bin (16-row tile blocks R) of a patch of n sources solved for F
 // fields in one launch: room for the patch and the F right-hand-side rows.
 int solve_bin_rows(int n, int F);
-// Launches the FP64 per-patch solve for the patches of bin R.
-void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
-                      hipStream_t s);
+// Launches the FP64 per-patch solve for the patches of bin R with a
+// workgroup of `waves` waves (8: bins 1 .. 13; 16: bins 11 .. kTileMaxR).
+void launch_solve_f64(int R, int waves, int kind, const SolveArgs &a,
+                      const double *coef, hipStream_t s);
+// Whether bin R has a variant with this many waves.
+bool solve_tile_variant(int R, int waves);
+// Largest size bin of the register-resident solve, and the largest patch it
+// takes (room for kFMax right-hand-side rows in that bin).
+constexpr int kTileMaxR = 17;
+constexpr int kTileMaxN = 16 * kTileMaxR - 8;
 // The same for one bin (explicitly instantiated in rbf_solve_tile_*.hip).
-template <int R>
+template <int R, int NW>
 void launch_solve_tile(int kind, const SolveArgs &a, const double *coef,
                        hipStream_t s);
 // Transpose of the per-patch chain for the bin with R blocks: right-hand

@@ -28,6 +28,11 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
                        const double *dTgt, int64_t M, hipStream_t s)
     : prm_(p), N_(N), M_(M) {
+  // 8 waves per workgroup (two workgroups per CU) up to bin 10; the larger
+  // bins hold more tiles than 7 tile waves keep in registers at that
+  // occupancy, and run 16 waves (one workgroup per CU) instead.
+  for (int R = 0; R <= kTileMaxR; ++R)
+    tileWaves_[R] = R <= 10 ? 8 : 16;
   const int D = p.dim;
   if (D < 1 || D > 3)
     throw std::runtime_error("The dimension of the points can only be: 1, 2 or 3.");
@@ -234,9 +239,9 @@ const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
   Bins &B = bins_[F - 1];
   if (B.built)
     return B;
-  std::vector<std::vector<int>> bins(solve_bin_rows(kMaxPatch, F) + 1);
+  std::vector<std::vector<int>> bins(solve_bin_rows(tileMaxN_, F) + 1);
   for (int64_t kk = 0; kk < K_; ++kk)
-    if (patchN_[kk] <= kMaxPatch)  // larger ones: workspace path
+    if (patchN_[kk] <= tileMaxN_)  // larger ones: workspace path
       bins[solve_bin_rows(patchN_[kk], F)].push_back((int)kk);
   for (int R = 0; R < (int)bins.size(); ++R) {
     if (bins[R].empty())
@@ -312,7 +317,7 @@ void RbfPumPlan::work(double *w) const {
     sn2 += n * n;
     sn3 += n * n * n;
     snm += n * (double)m;
-    if (n > kMaxPatch) {
+    if (n > tileMaxN_) {
       wk += 1;
       wn3 += n * n * n;
       wn2 += n * n;
@@ -785,7 +790,7 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     for (size_t b = 0; b < B.R.size(); ++b) {
       a.patches = B.dev[b].get();
       a.count = B.count[b];
-      launch_solve_f64(B.R[b], prm_.kind, a, bu.coef, s);
+      launch_solve_f64(B.R[b], tileWaves_[B.R[b]], prm_.kind, a, bu.coef, s);
     }
     if (!wsHost_.empty()) {
       if (f0 > 0)
@@ -924,7 +929,7 @@ void RbfPumPlan::patchInfo(int *pv, int *status, hipStream_t s) const {
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
 }
 
-// Patches above kMaxPatch sources, largest first (ties: lower patch index),
+// Patches above tileMaxN_ sources, largest first (ties: lower patch index),
 // and the per-slot workspaces of their persistent launches: a slot holds the
 // lower block triangle of the largest patch of its queue plus its Linv
 // blocks (FP64 size; FP32 storage uses the first half) and 4 + 4 F vectors.
@@ -941,7 +946,7 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
     q.nbMax = q.slots = 0;
   }
   for (int64_t k = 0; k < K_; ++k)
-    if (patchN_[k] > kMaxPatch)
+    if (patchN_[k] > tileMaxN_)
       wsHost_.push_back((int)k);
   if (wsHost_.empty())
     return;
@@ -1114,12 +1119,16 @@ void RbfPumPlan::patchRadii(double *r, hipStream_t s) const {
 }
 
 void RbfPumPlan::require_register_patches(const char *what) const {
-  if (!wsHost_.empty())
+  if (maxN_ > kMaxPatch) {
+    int64_t larger = 0;
+    for (int n : patchN_)
+      larger += n > kMaxPatch ? 1 : 0;
     throw std::runtime_error(
         std::string(what) + " is available for patches of up to " +
         std::to_string(kMaxPatch) + " source points; this plan has " +
-        std::to_string(wsHost_.size()) + " larger patches (largest " +
+        std::to_string(larger) + " larger patches (largest " +
         std::to_string(maxN_) + ")");
+  }
 }
 
 } // namespace pacman_amd

@@ -86,7 +86,8 @@ public:
   const int64_t *latticeShape() const { return lat_.shape; }
   int64_t numPairs() const { return pairs_.P; }
   int maxPatchSize() const { return maxN_; }
-  // Patches above kMaxPatch sources (solved on the workspace path).
+  // Patches above the register-resident solve's limit (solved on the
+  // workspace path).
   int64_t numLargePatches() const { return (int64_t)wsHost_.size(); }
   void setPrecision(int precision, int maxIR);
   // Counters of the last apply (synchronises s): 0 numerically singular
@@ -110,12 +111,12 @@ public:
   // total. Apply phases come from HIP events of the last apply on its
   // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total,
   // 9 polynomial fit of the first field chunk, 10 first-pass workspace
-  // launches (patches above kMaxPatch) of the first field chunk.
+  // launches (patches above tileMaxN_) of the first field chunk.
   static constexpr int kPhases = 11;
   void phaseTimes(double *ms) const;
   // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
   // targets), 5 pairs (targets within r of a centre), 6 max n; over the
-  // patches above kMaxPatch: 7 count, 8 sum n^3, 9 sum n^2, 10 sum n m of
+  // patches above tileMaxN_: 7 count, 8 sum n^3, 9 sum n^2, 10 sum n m of
   // those evaluated by the workspace kernel, 11 sum n m of those evaluated
   // in chunks (more than kHeavyPairs pairs).
   static constexpr int kWork = 12;
@@ -134,6 +135,10 @@ private:
   DevBuf<int64_t> srcOff_;
   DevBuf<int> srcIdx_;
   PairSet pairs_;
+  // Register-resident solve: patches of up to tileMaxN_ sources, binned by
+  // tile blocks R; bin R runs workgroups of tileWaves_[R] waves.
+  int tileMaxN_ = kTileMaxN;
+  int tileWaves_[kTileMaxR + 1];
   // Patches binned by tile blocks R, one binning per fields-per-launch
   // count F (index F - 1).
   struct Bins {
@@ -143,7 +148,7 @@ private:
     std::vector<DevBuf<int>> dev;
   };
   std::vector<Bins> bins_;
-  // Workspace path: patches above kMaxPatch sources, largest first, in two
+  // Workspace path: patches above tileMaxN_ sources, largest first, in two
   // queues (bordered matrices that fit the LDS variant, larger ones); the
   // FP64 fallback pass of mixed precision takes any of them.
   struct WsQueue {

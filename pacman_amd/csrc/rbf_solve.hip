@@ -805,30 +805,47 @@ void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s) {
 
 int solve_bin_rows(int n, int F) { return (n + F + 15) / 16; }
 
-void launch_solve_f64(int R, int kind, const SolveArgs &a, const double *coef,
-                      hipStream_t s) {
+bool solve_tile_variant(int R, int waves) {
+  return (waves == 8 && R >= 1 && R <= 13) ||
+         (waves == 16 && R >= 11 && R <= kTileMaxR);
+}
+
+void launch_solve_f64(int R, int waves, int kind, const SolveArgs &a,
+                      const double *coef, hipStream_t s) {
   if (a.count <= 0)
     return;
   if (a.F < 1 || a.F > kFMax)
     throw std::runtime_error("solve: 1 <= fields <= 8 per launch");
   if (kind < 0 || kind > 4)
     throw std::runtime_error("solve: unknown Wendland kind");
-  switch (R) {
-  case 1: launch_solve_tile<1>(kind, a, coef, s); break;
-  case 2: launch_solve_tile<2>(kind, a, coef, s); break;
-  case 3: launch_solve_tile<3>(kind, a, coef, s); break;
-  case 4: launch_solve_tile<4>(kind, a, coef, s); break;
-  case 5: launch_solve_tile<5>(kind, a, coef, s); break;
-  case 6: launch_solve_tile<6>(kind, a, coef, s); break;
-  case 7: launch_solve_tile<7>(kind, a, coef, s); break;
-  case 8: launch_solve_tile<8>(kind, a, coef, s); break;
-  case 9: launch_solve_tile<9>(kind, a, coef, s); break;
-  case 10: launch_solve_tile<10>(kind, a, coef, s); break;
-  case 11: launch_solve_tile<11>(kind, a, coef, s); break;
-  case 12: launch_solve_tile<12>(kind, a, coef, s); break;
-  case 13: launch_solve_tile<13>(kind, a, coef, s); break;
-  default:
+  if (!solve_tile_variant(R, waves))
     throw std::runtime_error("solve: unsupported size bin");
+  if (waves == 16) {
+    switch (R) {
+    case 11: launch_solve_tile<11, 16>(kind, a, coef, s); break;
+    case 12: launch_solve_tile<12, 16>(kind, a, coef, s); break;
+    case 13: launch_solve_tile<13, 16>(kind, a, coef, s); break;
+    case 14: launch_solve_tile<14, 16>(kind, a, coef, s); break;
+    case 15: launch_solve_tile<15, 16>(kind, a, coef, s); break;
+    case 16: launch_solve_tile<16, 16>(kind, a, coef, s); break;
+    default: launch_solve_tile<17, 16>(kind, a, coef, s); break;
+    }
+    return;
+  }
+  switch (R) {
+  case 1: launch_solve_tile<1, 8>(kind, a, coef, s); break;
+  case 2: launch_solve_tile<2, 8>(kind, a, coef, s); break;
+  case 3: launch_solve_tile<3, 8>(kind, a, coef, s); break;
+  case 4: launch_solve_tile<4, 8>(kind, a, coef, s); break;
+  case 5: launch_solve_tile<5, 8>(kind, a, coef, s); break;
+  case 6: launch_solve_tile<6, 8>(kind, a, coef, s); break;
+  case 7: launch_solve_tile<7, 8>(kind, a, coef, s); break;
+  case 8: launch_solve_tile<8, 8>(kind, a, coef, s); break;
+  case 9: launch_solve_tile<9, 8>(kind, a, coef, s); break;
+  case 10: launch_solve_tile<10, 8>(kind, a, coef, s); break;
+  case 11: launch_solve_tile<11, 8>(kind, a, coef, s); break;
+  case 12: launch_solve_tile<12, 8>(kind, a, coef, s); break;
+  default: launch_solve_tile<13, 8>(kind, a, coef, s); break;
   }
 }
 

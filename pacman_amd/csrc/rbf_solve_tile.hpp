@@ -1,12 +1,15 @@
 // Blocked per-patch solve on 16 x 16 tiles with FP64 matrix-core updates.
 //
-// One workgroup of 8 waves per patch. The patch matrix A (n x n, n <= 16R)
-// is held in registers as its lower-triangle 16 x 16 tiles (I, J), I >= J,
-// in the accumulator layout of v_mfma_f64_16x16x4f64: lane l, register v
-// holds element (16 I + (l >> 4) + 4 v, 16 J + (l & 15)) of tile (I, J).
-// Waves 1..7 (the tile waves) hold the tiles, dealt in column-major order,
-// cyclically, so every wave keeps work while the trailing matrix shrinks.
-// Wave 0 (the control wave) holds none: it eliminates the diagonal blocks.
+// One workgroup of NW waves per patch (8, or 16 for the larger bins). The
+// patch matrix A (n x n, n <= 16R) is held in registers as its
+// lower-triangle 16 x 16 tiles (I, J), I >= J, in the accumulator layout of
+// v_mfma_f64_16x16x4f64: lane l, register v holds element
+// (16 I + (l >> 4) + 4 v, 16 J + (l & 15)) of tile (I, J). Waves 1..NW-1
+// (the tile waves) hold the tiles, dealt in column-major order, cyclically,
+// so every wave keeps work while the trailing matrix shrinks. Wave 0 (the
+// control wave) holds none: it eliminates the diagonal blocks. With 16 waves
+// (one workgroup per CU, 128 registers per lane) bins of up to R = 17 keep
+// every tile in registers; 8 waves run two workgroups per CU.
 //
 // The right-hand sides ride along as F extra rows n .. n+F-1 holding the
 // residual b - Q beta (the bordered matrix [[A, B^T], [B, I]]): the
@@ -96,22 +99,26 @@ __device__ __forceinline__ void dpp_src_ready(double src) {
   asm volatile("s_nop 1" : : "v"(src));
 }
 
-// Owners of the lower-triangle 16x16 tiles among the 7 tile waves. Tile
+// Owners of the lower-triangle 16x16 tiles among the NWT tile waves. Tile
 // (J+1, J) and the diagonal tile (J+1, J+1) go to the same wave: that wave
 // computes W_{J+1} during panel J and applies it to the diagonal tile at
 // once, so the control wave gets that tile without a further barrier. Pairs
 // are dealt first, then the tile (0, 0), then the other tiles, each to the
 // least-loaded wave; every wave lists its tiles column by column.
+constexpr int kMaxTileWaves = 15;     // tile waves of a 16-wave workgroup
+constexpr int kMaxTilesPerWave = 16;  // R <= 13 at 7 tile waves, 17 at 15
+
 struct TileTable {
-  int I[7][16], J[7][16], cnt[7];
+  int I[kMaxTileWaves][kMaxTilesPerWave], J[kMaxTileWaves][kMaxTilesPerWave],
+      cnt[kMaxTileWaves];
 };
 
-constexpr TileTable make_tile_table(int R) {
+constexpr TileTable make_tile_table(int R, int NWT) {
   TileTable t{};
-  int load[7] = {0, 0, 0, 0, 0, 0, 0};
+  int load[kMaxTileWaves] = {};
   auto pick = [&]() {
     int w = 0;
-    for (int x = 1; x < 7; ++x)
+    for (int x = 1; x < NWT; ++x)
       if (load[x] < load[w])
         w = x;
     return w;
@@ -131,7 +138,7 @@ constexpr TileTable make_tile_table(int R) {
   for (int J = 0; J < R; ++J)
     for (int I = J + 2; I < R; ++I)
       put(pick(), I, J);
-  for (int w = 0; w < 7; ++w)  // column-major order within a wave
+  for (int w = 0; w < NWT; ++w)  // column-major order within a wave
     for (int a = 1; a < t.cnt[w]; ++a)
       for (int b = a; b > 0; --b) {
         const bool less = t.J[w][b] < t.J[w][b - 1] ||
@@ -148,21 +155,23 @@ constexpr TileTable make_tile_table(int R) {
   return t;
 }
 
-constexpr int max_tiles_per_wave(const TileTable &t) {
+constexpr int max_tiles_per_wave(const TileTable &t, int NWT) {
   int m = 1;
-  for (int w = 0; w < 7; ++w)
+  for (int w = 0; w < NWT; ++w)
     m = t.cnt[w] > m ? t.cnt[w] : m;
   return m;
 }
 
-template <int R> struct TileShape {
+template <int R, int NW_ = 8> struct TileShape {
   static constexpr int NMAX = 16 * R;
-  static constexpr int NW = 8;              // waves per workgroup
+  static constexpr int NW = NW_;            // waves per workgroup
   static constexpr int NT = 64 * NW;
   static constexpr int NWT = NW - 1;        // tile waves
   static constexpr int NTILE = R * (R + 1) / 2;
-  static constexpr TileTable TAB = make_tile_table(R);
-  static constexpr int TPW = max_tiles_per_wave(TAB);
+  static constexpr TileTable TAB = make_tile_table(R, NWT);
+  static constexpr int TPW = max_tiles_per_wave(TAB, NWT);
+  static_assert(NWT <= kMaxTileWaves, "tile waves");
+  static_assert(TPW <= kMaxTilesPerWave, "tiles per wave");
   static constexpr int LDW = NMAX + 4;      // panel column stride (doubles)
   static constexpr int PANEL = 2 * 16 * LDW;
   static constexpr int EVAL = NW * kFMax * 64;
@@ -173,12 +182,13 @@ template <int R> struct TileShape {
 
 // ADJ: the transpose chain. The right-hand side is E^T h with h = w g at
 // the member targets (instead of the residual b - Q beta), and the kernel
-// stores gamma = A^-1 E^T h per patch source instead of evaluating.
-template <int R, bool ADJ>
-__global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
-                                                            const double *coef,
-                                                            int kind) {
-  using TS = TileShape<R>;
+// stores gamma = A^-1 E^T h per patch source instead of evaluating (8 waves
+// only: its right-hand-side pass splits 512 threads into two halves).
+template <int R, bool ADJ, int NW>
+__global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
+    SolveArgs a, const double *coef, int kind) {
+  static_assert(!ADJ || NW == 8, "the transpose chain runs 8 waves");
+  using TS = TileShape<R, NW>;
   constexpr int NMAX = TS::NMAX;
   constexpr int NT = TS::NT;
   constexpr int NWT = TS::NWT;
@@ -531,12 +541,12 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
       constexpr int s = decltype(sc)::value;
       tI[s] = R;
       tJ[s] = R;
-      static_for<0, 7>([&](auto wc) {
+      static_for<0, NWT>([&](auto wc) {
         constexpr int w = decltype(wc)::value;
-        constexpr TileTable tab = TileShape<R>::TAB;
+        constexpr TileTable tab = TS::TAB;
         if (tw == w && s < tab.cnt[w]) {
-          tI[s] = tab.I[w][s < 16 ? s : 0];
-          tJ[s] = tab.J[w][s < 16 ? s : 0];
+          tI[s] = tab.I[w][s < kMaxTilesPerWave ? s : 0];
+          tJ[s] = tab.J[w][s < kMaxTilesPerWave ? s : 0];
         }
       });
     });
@@ -814,19 +824,20 @@ __global__ __launch_bounds__(512, 4) void solve_tile_kernel(SolveArgs a,
 
 } // namespace
 
-template <int R>
+template <int R, int NW>
 void launch_solve_tile(int kind, const SolveArgs &a, const double *coef,
                        hipStream_t s) {
-  solve_tile_kernel<R, false><<<dim3((unsigned)a.count),
-                                dim3(TileShape<R>::NT), 0, s>>>(a, coef, kind);
+  solve_tile_kernel<R, false, NW><<<dim3((unsigned)a.count),
+                                    dim3(TileShape<R, NW>::NT), 0, s>>>(
+      a, coef, kind);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
 template <int R>
 void launch_solve_tile_adjoint(int kind, const SolveArgs &a, hipStream_t s) {
-  solve_tile_kernel<R, true><<<dim3((unsigned)a.count),
-                               dim3(TileShape<R>::NT), 0, s>>>(a, nullptr,
-                                                               kind);
+  solve_tile_kernel<R, true, 8><<<dim3((unsigned)a.count),
+                                  dim3(TileShape<R, 8>::NT), 0, s>>>(
+      a, nullptr, kind);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
