@@ -10,12 +10,14 @@ quoted on: 4,096,000 sources from a 12-component Gaussian mixture in
 [0, 1]^3 (seed 4) -> 4,096,000 uniform targets (seed 5), density-aware
 patches (leaf 40 sources, 20 to 600 per patch, overlap 0.15, support = patch
 radius), 8 fields (franke_3d + 7 seeded trig fields), Wendland C2. The timed
-step is the mixed/batched transfer: one plan, FP32 factors with FP64
-refinement for the patches above 192 points, all 8 fields per solve. Beside
-it, on the same GPU: the FP64 plain path (one FP64 setup + one-field apply
-per field, the reference's calling pattern), the FP64 batched transfer, all
-five Wendland bases, and the agreement of the results (rel-L2 vs franke_3d,
-max difference mixed vs FP64).
+step is the batched transfer: one plan, all 8 fields per solve, each size
+class of patches in the variant and precision the size-bin table
+(pacman_amd/autotune.py) measured fastest (precision "auto"). Beside it, on
+the same GPU: the FP64 plain path (one FP64 setup + one-field apply per
+field, the reference's calling pattern), the FP64 batched transfer, the
+mixed transfer (FP32 factors with FP64 refinement forced on the workspace
+patches above 264 points), all five Wendland bases, and the agreement of
+the results (rel-L2 vs franke_3d, max differences vs FP64).
 
 --config c2 / c5: the FP64 plain transfer of one field on the jittered-grid
 workloads (reference patch heuristic, 150 points per patch).
@@ -177,7 +179,7 @@ def main() -> int:
         2 * work["sum_nm"]
     f_exec = (1.0 / 3.0) * work["sum_n3"] + 2 * work["sum_n2"] + \
         2 * work["sum_nm"]
-    kern_ms = ph["solve"] - ph["poly"]
+    kern_ms = ph["tiles"]
     achieved = f_alg / (kern_ms / 1e3) / 1e12
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"solve_tile_{args.config}_pmc.json")
@@ -277,7 +279,8 @@ def median_time(fn, reps=3, warmup=1):
 
 
 def bench_mixture(args) -> int:
-    """Mixed/batched vs FP64 plain on the uneven (Gaussian-mixture) cloud."""
+    """Batched transfer (autotuned precision) vs FP64 plain on the uneven
+    (Gaussian-mixture) cloud."""
     import numpy as np
     import torch
 
@@ -309,28 +312,35 @@ def bench_mixture(args) -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / steps, r
 
-    # the headline step: mixed / batched
-    ms, plan = timed(lambda: transfer(args.rbf, "mixed", dV, out),
+    # the headline step: all fields per solve, precision per the size-bin
+    # table ("auto")
+    ms, plan = timed(lambda: transfer(args.rbf, "auto", dV, out),
                      args.steps, args.warmup)
-    log(f"[bench] mixed/batched: {ms:.2f} ms/step")
-    mixed_out = out.clone()
+    log(f"[bench] batched (auto precision): {ms:.2f} ms/step")
+    head_out = out.clone()
     work = plan.work()
-    cnt = plan.counters()
     large = plan.num_large_patches
     del plan
     # phase times of one more step started on an idle GPU (the host-timed
     # setup phases of back-to-back steps include the previous step's tail)
     torch.cuda.synchronize()
-    plan = transfer(args.rbf, "mixed", dV, out)
+    plan = transfer(args.rbf, "auto", dV, out)
     torch.cuda.synchronize()
     ph = plan.phase_times()
     del plan
-    # FP64 batched (same plan shape, FP64 workspace solve)
+    # FP64 batched (every class in FP64)
     fp64_out = torch.empty_like(out)
     ms_b, plan = timed(lambda: transfer(args.rbf, "fp64", dV, fp64_out),
                        args.steps, 1)
     del plan
     log(f"[bench] fp64 batched: {ms_b:.2f} ms/step")
+    # mixed: FP32 factors + FP64 refinement forced on the workspace path
+    mixed_out = torch.empty_like(out)
+    ms_m, plan = timed(lambda: transfer(args.rbf, "mixed", dV, mixed_out),
+                       args.steps, 1)
+    cnt = plan.counters()
+    del plan
+    log(f"[bench] mixed batched: {ms_m:.2f} ms/step")
     # FP64 plain: one FP64 setup + one-field apply per field
     plain_out = torch.empty_like(out)
 
@@ -347,39 +357,50 @@ def bench_mixture(args) -> int:
         return float(torch.linalg.norm(o[0] - exact) / torch.linalg.norm(exact))
 
     scale = float(fp64_out.abs().max())
-    diff_mixed = float((mixed_out - fp64_out).abs().max()) / scale
-    diff_plain = float((plain_out - fp64_out).abs().max()) / scale
-    rel_mixed = float(torch.linalg.norm(mixed_out - fp64_out) /
-                      torch.linalg.norm(fp64_out))
 
-    # all five bases: one step each, mixed/batched and FP64 batched
+    def maxrel(o):
+        return float((o - fp64_out).abs().max()) / scale
+
+    def rell2(o, ref):
+        return float(torch.linalg.norm(o - ref) / torch.linalg.norm(ref))
+
+    # all five bases: one step each in each precision
     per_basis = {}
     for rbf in (0x10, 0x11, 0x12, 0x13, 0x14):
-        o1 = torch.empty_like(out)
-        o2 = torch.empty_like(out)
-        tm, pl = timed(lambda: transfer(rbf, "mixed", dV, o1), 1, 1)
-        c = pl.counters()
+        o1, o2, o3 = (torch.empty_like(out) for _ in range(3))
+        ta, pl = timed(lambda: transfer(rbf, "auto", dV, o1), 1, 1)
         del pl
         tf, pl = timed(lambda: transfer(rbf, "fp64", dV, o2), 1, 1)
         del pl
+        tm, pl = timed(lambda: transfer(rbf, "mixed", dV, o3), 1, 1)
+        c = pl.counters()
+        del pl
         per_basis[f"0x{rbf:02x}"] = {
-            "mixed_ms": round(tm, 3), "fp64_batched_ms": round(tf, 3),
+            "auto_ms": round(ta, 3), "fp64_batched_ms": round(tf, 3),
+            "mixed_ms": round(tm, 3),
             "rel_l2_vs_franke_3d": rl2(o1),
-            "mixed_vs_fp64_rel_l2": float(torch.linalg.norm(o1 - o2) /
-                                          torch.linalg.norm(o2)),
-            "refinements": c["refinements"], "fallbacks": c["fallbacks"]}
-        log(f"[bench] basis 0x{rbf:02x}: mixed {tm:.1f} ms, fp64 {tf:.1f} ms")
+            "auto_vs_fp64_rel_l2": rell2(o1, o2),
+            "mixed_vs_fp64_rel_l2": rell2(o3, o2),
+            "mixed_refinements": c["refinements"],
+            "mixed_fallbacks": c["fallbacks"]}
+        log(f"[bench] basis 0x{rbf:02x}: auto {ta:.1f} ms, fp64 {tf:.1f} ms, "
+            f"mixed {tm:.1f} ms")
 
-    # dominant kernel: the first-pass workspace launch (ws_solve_kernel,
-    # FP32 factors + FP64 refinement) over the patches above 192 sources;
-    # algorithmic flops of the reference's chain per patch: (2/3) n^3 once
-    # (one factorisation serves all F fields) + F (2 n^2 + 2 n m) for the
-    # patches it evaluates itself (refinement steps are not counted)
-    f_ws = (2.0 / 3.0) * work["large_sum_n3"] + F * (
-        2 * work["large_sum_n2"] + 2 * work["large_sum_nm"])
-    ws_ms = ph["workspace"]
-    achieved = f_ws / (ws_ms / 1e3) / 1e12 if ws_ms > 0 else 0.0
-    peak32 = PEAK_FP32_TFLOPS
+    # dominant kernel: the register-resident solve (all size-bin launches of
+    # the step; the 8 fields share one launch per bin). Algorithmic flops of
+    # the reference's chain per patch: (2/3) n^3 once (one factorisation
+    # serves all F fields) + F (2 n^2 + 2 n m), with m the member targets it
+    # evaluates itself (the patches with more than 4096 target pairs are
+    # evaluated by their own kernel).
+    f_tile = (2.0 / 3.0) * work["tile_sum_n3"] + F * (
+        2 * work["tile_sum_n2"] + 2 * work["tile_sum_nm"])
+    tile_ms = ph["tiles"]
+    achieved = f_tile / (tile_ms / 1e3) / 1e12 if tile_ms > 0 else 0.0
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "solve_tile_mixture_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -424,42 +445,47 @@ def bench_mixture(args) -> int:
                         f"5), density-aware patches (leaf {leaf}, {nmin} to "
                         f"{nmax} sources, overlap 0.15, support = patch "
                         f"radius), Wendland 0x{args.rbf:02x}, {F} fields, "
-                        "mixed precision (FP32 factors + FP64 refinement "
-                        "above 192 points, FP64 below), all fields per solve",
+                        "all fields per solve, precision per size class "
+                        "from the autotuned table (FP64 factors in "
+                        "registers up to 264 points; workspace patches in "
+                        "the precision the table measured faster)",
             "fields": F,
             "mpts_fields_per_s": M * F / (ms / 1e3) / 1e6,
             "fp64_plain_ms": ms_p,
             "fp64_batched_ms": ms_b,
+            "mixed_batched_ms": ms_m,
             "speedup_vs_fp64_plain": ms_p / ms,
             "speedup_vs_fp64_batched": ms_b / ms,
-            "rel_l2_vs_franke_3d": {"mixed": rl2(mixed_out),
+            "speedup_mixed_vs_fp64_plain": ms_p / ms_m,
+            "rel_l2_vs_franke_3d": {"batched": rl2(head_out),
                                     "fp64_batched": rl2(fp64_out),
+                                    "mixed": rl2(mixed_out),
                                     "fp64_plain": rl2(plain_out)},
-            "mixed_vs_fp64_max_rel": diff_mixed,
-            "mixed_vs_fp64_rel_l2": rel_mixed,
-            "plain_vs_batched_max_rel": diff_plain,
+            "batched_vs_fp64_max_rel": maxrel(head_out),
+            "mixed_vs_fp64_max_rel": maxrel(mixed_out),
+            "mixed_vs_fp64_rel_l2": rell2(mixed_out, fp64_out),
+            "plain_vs_batched_max_rel": maxrel(plain_out),
             "patches": int(work["patches"]),
-            "large_patches": large,
+            "workspace_patches": large,
             "max_patch_size": int(work["max_n"]),
             "work": work,
-            "refinements": cnt["refinements"],
-            "fallbacks": cnt["fallbacks"],
+            "mixed_refinements": cnt["refinements"],
+            "mixed_fallbacks": cnt["fallbacks"],
             "phase_ms": {k: round(x, 3) for k, x in ph.items()},
             "per_basis": per_basis,
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "ws_solve_kernel<float> (first pass over the patches "
-                      "above 192 sources: FP32 assembly and L D L^T on "
-                      "matrix cores, FP64 refinement, FP64 evaluation); "
-                      "peak = FP32 matrix",
+            "kernel": "solve_tile_kernel (all size-bin launches of one "
+                      "step: FP64 L D L^T on matrix cores, 8 or 16 waves "
+                      "per bin from the autotuned table)",
             "achieved": achieved,
-            "peak": peak32,
+            "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / peak32,
-            "traffic": None,
-            "kernel_ms": ws_ms,
-            "flops_per_launch": f_ws,
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": traffic,
+            "kernel_ms": tile_ms,
+            "flops_per_launch": f_tile,
         },
         "cpu_baseline": cpu,
     }

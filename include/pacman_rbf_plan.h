@@ -155,12 +155,13 @@ int pacman_rbf_plan_create_adaptive(
  * plan built by pacman_rbf_plan_create). */
 int pacman_rbf_plan_patch_radii(pacman_rbf_plan plan, double *radii);
 
-/* Patches holding more than 192 source points (no reference counterpart:
- * the reference solves every patch the same way). They are solved by a
- * persistent workspace kernel (blocked L D L^T in device memory, any size);
- * the transpose apply, the stored operator and the target-coordinate
- * gradient need every patch within 192 points and return an error
- * otherwise. */
+/* Patches holding more than 264 source points (no reference counterpart:
+ * the reference solves every patch the same way). Patches of up to 264
+ * points are solved in registers (16-wave workgroups above 192 points);
+ * larger ones by a persistent workspace kernel (blocked L D L^T in device
+ * memory, any size). The transpose apply, the stored operator and the
+ * target-coordinate gradient need every patch within 192 points and return
+ * an error otherwise. */
 long long pacman_rbf_plan_num_large_patches(pacman_rbf_plan plan);
 
 /* Arithmetic of the workspace solve (no reference counterpart; the
@@ -168,11 +169,27 @@ long long pacman_rbf_plan_num_large_patches(pacman_rbf_plan plan);
  * precision 1 (mixed): FP32 assembly and factorisation, FP64 iterative
  * refinement with a matrix-free FP64 residual (at most maxRefinements
  * steps) and a per-patch FP64 fallback when it does not converge; results
- * agree with FP64 to refinement accuracy (about 1e-13 relative). Patches of
- * up to 192 points are solved in FP64 either way. Takes effect at the next
+ * agree with FP64 to refinement accuracy (about 1e-13 relative).
+ * precision 2 (auto): per field count, whichever of the two the library's
+ * size-bin table (pacman_amd/autotune.py) measured faster. Patches of up to
+ * 264 points are solved in FP64 either way. Takes effect at the next
  * apply. */
 int pacman_rbf_plan_set_precision(pacman_rbf_plan plan, int precision,
                                   int maxRefinements);
+
+/* Size-bin tuning hook (no reference counterpart; used by
+ * pacman_amd/autotune.py): gathers `values` (device, F x N, 1 <= F <= 8),
+ * fits the polynomials, then times `reps` launches of one class of the
+ * per-patch solve on this plan's patches and writes the median in
+ * milliseconds to *ms: bin R >= 1 of the register-resident solve (tile rows
+ * R = ceil((n + F) / 16)) with `waves` waves per workgroup (8: bins 1..13,
+ * 16: bins 11..17), or with R = 0 the workspace-path launches in FP64
+ * (wsPrecision 0) or FP32 with FP64 refinement and its fallback pass (1).
+ * *ms = -1 when the plan has no patch in that class or the variant does
+ * not exist. Synchronises `stream`. */
+int pacman_rbf_plan_time_solve(pacman_rbf_plan plan, const double *values,
+                               int F, int R, int waves, int wsPrecision,
+                               int reps, double *ms, void *stream);
 
 /* Counters of the last apply (synchronises `stream`): counters3[0] patches
  * found numerically singular (pivot <= 1e-12 phi(0): duplicate or

@@ -62,6 +62,9 @@ SIGNATURES = {
     "pacman_rbf_plan_patch_radii": (ctypes.c_int, [_vp, _dp]),
     "pacman_rbf_plan_set_precision": (ctypes.c_int, [_vp, ctypes.c_int,
                                                      ctypes.c_int]),
+    "pacman_rbf_plan_time_solve": (ctypes.c_int, [
+        _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, _dp, _vp]),
     "pacman_rbf_plan_apply_counters": (ctypes.c_int, [_vp, _ip, _vp]),
     "pacman_rbf_plan_lattice": (ctypes.c_int, [_vp, _llp, _llp]),
     "pacman_rbf_plan_centers": (ctypes.c_int, [_vp, _dp, _llp]),

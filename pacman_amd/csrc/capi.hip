@@ -989,6 +989,23 @@ int pacman_rbf_plan_set_precision(pacman_rbf_plan plan, int precision,
   }
 }
 
+int pacman_rbf_plan_time_solve(pacman_rbf_plan plan, const double *values,
+                               int F, int R, int waves, int wsPrecision,
+                               int reps, double *ms, void *stream) {
+  try {
+    if (!plan || !plan->plan || !values || !ms)
+      throw std::invalid_argument("plan, values or output is null");
+    if (R < 0)
+      throw std::invalid_argument("size bin must be >= 0");
+    *ms = plan->plan->timeSolve(values, F, R, waves, wsPrecision, reps,
+                                (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    set_error("pacman_rbf_plan_time_solve", e.what());
+    return -1;
+  }
+}
+
 int pacman_rbf_plan_apply_counters(pacman_rbf_plan plan, int *counters3,
                                    void *stream) {
   try {

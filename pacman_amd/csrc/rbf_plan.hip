@@ -5,6 +5,7 @@
 // lattice spacing) rounds exactly as the CPU restatement does.
 
 #include "rbf_plan.hpp"
+#include "rbf_tune_table.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -28,11 +29,6 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
                        const double *dTgt, int64_t M, hipStream_t s)
     : prm_(p), N_(N), M_(M) {
-  // 8 waves per workgroup (two workgroups per CU) up to bin 10; the larger
-  // bins hold more tiles than 7 tile waves keep in registers at that
-  // occupancy, and run 16 waves (one workgroup per CU) instead.
-  for (int R = 0; R <= kTileMaxR; ++R)
-    tileWaves_[R] = R <= 10 ? 8 : 16;
   const int D = p.dim;
   if (D < 1 || D > 3)
     throw std::runtime_error("The dimension of the points can only be: 1, 2 or 3.");
@@ -294,6 +290,10 @@ void RbfPumPlan::phaseTimes(double *ms) const {
     PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[5], ev_[6]));
     ms[10] = t;
   }
+  if (!opBuilt_) {
+    PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[4], ev_[7]));
+    ms[11] = t;
+  }
 }
 
 void RbfPumPlan::work(double *w) const {
@@ -308,6 +308,7 @@ void RbfPumPlan::work(double *w) const {
                                hipMemcpyDeviceToHost));
   double sn = 0, sn2 = 0, sn3 = 0, snm = 0;
   double wk = 0, wn3 = 0, wn2 = 0, wnm = 0, hnm = 0;
+  double tn3 = 0, tn2 = 0, tnm = 0;
   for (int64_t k = 0; k < K_; ++k) {
     const double n = (double)(off[k + 1] - off[k]);
     int64_t m = 0;
@@ -317,14 +318,20 @@ void RbfPumPlan::work(double *w) const {
     sn2 += n * n;
     sn3 += n * n * n;
     snm += n * (double)m;
+    const bool heavy = po[k + 1] - po[k] > kHeavyPairs;
+    if (heavy)
+      hnm += n * (double)m;
     if (n > tileMaxN_) {
       wk += 1;
       wn3 += n * n * n;
       wn2 += n * n;
-      if (po[k + 1] - po[k] > kHeavyPairs)
-        hnm += n * (double)m;
-      else
+      if (!heavy)
         wnm += n * (double)m;
+    } else {
+      tn3 += n * n * n;
+      tn2 += n * n;
+      if (!heavy)
+        tnm += n * (double)m;
     }
   }
   w[0] = (double)K_;
@@ -339,6 +346,9 @@ void RbfPumPlan::work(double *w) const {
   w[9] = wn2;
   w[10] = wnm;
   w[11] = hnm;
+  w[12] = tn3;
+  w[13] = tn2;
+  w[14] = tnm;
 }
 
 // Buffers and size bins an apply of F fields needs (allocations happen here,
@@ -790,12 +800,16 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     for (size_t b = 0; b < B.R.size(); ++b) {
       a.patches = B.dev[b].get();
       a.count = B.count[b];
-      launch_solve_f64(B.R[b], tileWaves_[B.R[b]], prm_.kind, a, bu.coef, s);
+      launch_solve_f64(B.R[b], kTune.waves[nf - 1][B.R[b]], prm_.kind, a,
+                       bu.coef, s);
     }
+    if (f0 == 0 && events)
+      PACMAN_HIP_CHECK(hipEventRecord(ev_[7], s));
     if (!wsHost_.empty()) {
       if (f0 > 0)
         launch_ws_reset(counters_.get(), false, s);
-      launch_workspace(a, bu.coef, s, events && f0 == 0);
+      launch_workspace(a, bu.coef, s, events && f0 == 0,
+                       workspace_fp32(nf));
     }
     launch_heavy_eval(prm_.kind, a, bu.coef, heavyChunks_.get(),
                       nHeavyChunks_, s);
@@ -1037,8 +1051,8 @@ void RbfPumPlan::plan_heavy(hipStream_t s) {
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,
-                                  hipStream_t s, bool timed) {
-  const bool mixed = prm_.precision == 1;
+                                  hipStream_t s, bool timed, bool fp32) {
+  const bool mixed = fp32;
   auto args = [&](const WsQueue &q) {
     WsArgs w{};
     w.a = a;
@@ -1085,9 +1099,72 @@ void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,
   }
 }
 
+bool RbfPumPlan::workspace_fp32(int F) const {
+  return prm_.precision == 1 || (prm_.precision == 2 && kTune.wsFp32[F - 1]);
+}
+
+double RbfPumPlan::timeSolve(const double *dVals, int F, int R, int waves,
+                             int wsPrecision, int reps, hipStream_t s) {
+  if (F < 1 || F > solve_fields_per_launch())
+    throw std::invalid_argument("tuning: 1 <= fields <= 8");
+  if (reps < 1 || reps > 1000)
+    throw std::invalid_argument("tuning: 1 <= repetitions <= 1000");
+  if (wsPrecision != 0 && wsPrecision != 1)
+    throw std::invalid_argument("tuning: workspace precision 0 or 1");
+  if (M_ == 0)
+    return -1.0;
+  if (R > 0 && !solve_tile_variant(R, waves))
+    return -1.0;
+  if (R == 0 && wsHost_.empty())
+    return -1.0;
+  prepare(F, s);
+  const Bins &B = bins_for(F, s);
+  int b = -1;
+  for (size_t i = 0; i < B.R.size(); ++i)
+    if (B.R[i] == R)
+      b = (int)i;
+  if (R > 0 && b < 0)
+    return -1.0;
+  SolveArgs a = solveArgs(F, valsSorted_.get());
+  a.contrib = contrib_.get();
+  launch_gather_values(dVals, gs_.view.perm, N_, F, valsSorted_.get(), s);
+  launch_poly(a, K_, coef_.get(), s);
+  hipEvent_t e[2];
+  PACMAN_HIP_CHECK(hipEventCreate(&e[0]));
+  PACMAN_HIP_CHECK(hipEventCreate(&e[1]));
+  std::vector<float> ms;
+  try {
+    for (int r = 0; r < reps; ++r) {
+      launch_ws_reset(counters_.get(), true, s);
+      PACMAN_HIP_CHECK(hipEventRecord(e[0], s));
+      if (R > 0) {
+        a.patches = B.dev[b].get();
+        a.count = B.count[b];
+        launch_solve_f64(R, waves, prm_.kind, a, coef_.get(), s);
+      } else {
+        launch_workspace(a, coef_.get(), s, false, wsPrecision == 1);
+      }
+      PACMAN_HIP_CHECK(hipEventRecord(e[1], s));
+      PACMAN_HIP_CHECK(hipEventSynchronize(e[1]));
+      float t = 0.f;
+      PACMAN_HIP_CHECK(hipEventElapsedTime(&t, e[0], e[1]));
+      ms.push_back(t);
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    throw;
+  }
+  PACMAN_HIP_CHECK(hipEventDestroy(e[0]));
+  PACMAN_HIP_CHECK(hipEventDestroy(e[1]));
+  std::sort(ms.begin(), ms.end());
+  return ms[ms.size() / 2];
+}
+
 void RbfPumPlan::setPrecision(int precision, int maxIR) {
-  if (precision != 0 && precision != 1)
-    throw std::invalid_argument("precision must be 0 (FP64) or 1 (mixed)");
+  if (precision < 0 || precision > 2)
+    throw std::invalid_argument(
+        "precision must be 0 (FP64), 1 (mixed) or 2 (auto)");
   if (maxIR < 0 || maxIR > 100)
     throw std::invalid_argument("refinement iteration limit out of range");
   releaseGraph();

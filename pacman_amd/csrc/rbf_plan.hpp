@@ -21,7 +21,9 @@ struct PlanParams {
   double support = 0.1;    // Wendland support radius; <= 0 means the patch radius
   // 0: FP64 throughout; 1: mixed - patches solved on the workspace path
   // store FP32 factors and are refined to FP64 accuracy (iterative
-  // refinement, FP64 fallback per patch).
+  // refinement, FP64 fallback per patch); 2: auto - per field count, the
+  // precision of the workspace path that the size-bin table measured
+  // faster (rbf_tune_table.hpp).
   int precision = 0;
   int maxIR = 6;           // refinement iteration limit (mixed)
   // Patches from the density-aware builder (rbf_adaptive.hip) instead of the
@@ -90,6 +92,15 @@ public:
   // workspace path).
   int64_t numLargePatches() const { return (int64_t)wsHost_.size(); }
   void setPrecision(int precision, int maxIR);
+  // Size-bin tuning: median time in ms of `reps` launches, after the value
+  // gather and polynomial fit of dVals (F x N, device), of the
+  // register-resident solve of bin R with `waves` waves per workgroup
+  // (R >= 1), or of the workspace-path launches (R = 0) with FP64
+  // (wsPrecision 0) or FP32 factors and FP64 refinement plus the FP64
+  // fallback pass (1). -1 when the plan has no patch in that class or the
+  // variant does not exist.
+  double timeSolve(const double *dVals, int F, int R, int waves,
+                   int wsPrecision, int reps, hipStream_t s);
   // Counters of the last apply (synchronises s): 0 numerically singular
   // patches, 1 refinement iterations (summed over patches), 2 patches that
   // fell back from FP32 refinement to FP64.
@@ -111,15 +122,18 @@ public:
   // total. Apply phases come from HIP events of the last apply on its
   // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total,
   // 9 polynomial fit of the first field chunk, 10 first-pass workspace
-  // launches (patches above tileMaxN_) of the first field chunk.
-  static constexpr int kPhases = 11;
+  // launches (patches above tileMaxN_) of the first field chunk, 11 the
+  // register-resident solve launches of the first field chunk.
+  static constexpr int kPhases = 12;
   void phaseTimes(double *ms) const;
   // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
   // targets), 5 pairs (targets within r of a centre), 6 max n; over the
   // patches above tileMaxN_: 7 count, 8 sum n^3, 9 sum n^2, 10 sum n m of
-  // those evaluated by the workspace kernel, 11 sum n m of those evaluated
-  // in chunks (more than kHeavyPairs pairs).
-  static constexpr int kWork = 12;
+  // those evaluated by the workspace kernel, 11 sum n m of the patches
+  // evaluated in chunks (more than kHeavyPairs pairs); over the patches of
+  // the register-resident solve: 12 sum n^3, 13 sum n^2, 14 sum n m of those
+  // it evaluates itself.
+  static constexpr int kWork = 15;
   void work(double *w) const;
   ~RbfPumPlan();
 
@@ -136,9 +150,8 @@ private:
   DevBuf<int> srcIdx_;
   PairSet pairs_;
   // Register-resident solve: patches of up to tileMaxN_ sources, binned by
-  // tile blocks R; bin R runs workgroups of tileWaves_[R] waves.
+  // tile blocks R (waves per workgroup from the size-bin table).
   int tileMaxN_ = kTileMaxN;
-  int tileWaves_[kTileMaxR + 1];
   // Patches binned by tile blocks R, one binning per fields-per-launch
   // count F (index F - 1).
   struct Bins {
@@ -174,7 +187,8 @@ private:
   int nHeavyChunks_ = 0;
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, const double *coef, hipStream_t s,
-                        bool timed);
+                        bool timed, bool fp32);
+  bool workspace_fp32(int F) const;
   std::vector<int> patchN_;  // sources per patch (host)
   const Bins &bins_for(int F, hipStream_t s);
   void prepare(int F, hipStream_t s);
@@ -226,8 +240,8 @@ private:
   DevBuf<int> tinv_;  // original target index -> cell order (operator apply)
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};
-  hipEvent_t ev_[7] = {nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr};
+  hipEvent_t ev_[8] = {nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr};
   bool wsTimed_ = false;  // ev_[5], ev_[6] recorded by the last apply
   bool applied_ = false;
 };

@@ -212,7 +212,7 @@ class Plan:
         if precision != "fp64" or max_refinements != 6:
             self.set_precision(precision, max_refinements)
 
-    _PRECISIONS = {"fp64": 0, "mixed": 1}
+    _PRECISIONS = {"fp64": 0, "mixed": 1, "auto": 2}
 
     @classmethod
     def adaptive(cls, space_dimension, rbf_function, source_points,
@@ -240,12 +240,14 @@ class Plan:
         return r
 
     def set_precision(self, precision="fp64", max_refinements=6):
-        """Arithmetic of the patches above 192 points (the workspace solve):
-        "fp64", or "mixed" (FP32 assembly and factorisation, FP64 iterative
+        """Arithmetic of the patches above 264 points (the workspace solve):
+        "fp64"; "mixed" (FP32 assembly and factorisation, FP64 iterative
         refinement with at most max_refinements steps, FP64 fallback per
-        patch). Smaller patches are solved in FP64 either way."""
+        patch); or "auto" (per field count, whichever of the two the
+        library's size-bin table measured faster, pacman_amd/autotune.py).
+        Smaller patches are solved in FP64 either way."""
         if precision not in self._PRECISIONS:
-            raise ValueError('precision must be "fp64" or "mixed"')
+            raise ValueError('precision must be "fp64", "mixed" or "auto"')
         rc = self._lib.pacman_rbf_plan_set_precision(
             self._h, self._PRECISIONS[precision], int(max_refinements))
         if rc != 0:
@@ -263,6 +265,29 @@ class Plan:
         if rc != 0:
             raise RuntimeError(_lib.last_error())
         return {"singular": c[0], "refinements": c[1], "fallbacks": c[2]}
+
+    def time_solve(self, values, size_bin, waves=8, ws_precision="fp64",
+                   reps=5, stream=None) -> float:
+        """Size-bin tuning hook: median milliseconds of `reps` launches of
+        one class of the per-patch solve on this plan's patches, for the
+        fields of `values` (device tensor, F x N, F <= 8): register bin
+        `size_bin` >= 1 with `waves` waves per workgroup, or with
+        size_bin = 0 the workspace path in `ws_precision` ("fp64" or
+        "mixed"). -1 when the plan has no such patch or variant."""
+        v, F, _ = _fields_in(values, self.num_sources, "values")
+        pv, dev = _ptr(v)
+        if not dev:
+            raise ValueError("tuning needs device values")
+        if F > 8:
+            raise ValueError("tuning takes at most 8 fields")
+        ms = ctypes.c_double(0.0)
+        rc = self._lib.pacman_rbf_plan_time_solve(
+            self._h, pv, F, int(size_bin), int(waves),
+            {"fp64": 0, "mixed": 1}[ws_precision], int(reps),
+            ctypes.byref(ms), ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError(_lib.last_error())
+        return float(ms.value)
 
     @property
     def num_large_patches(self) -> int:
@@ -442,10 +467,11 @@ class Plan:
         return off, idx[:size]
 
     PHASES = ("grids", "radius", "centres", "membership", "setup",
-              "gather", "solve", "blend", "apply", "poly", "workspace")
+              "gather", "solve", "blend", "apply", "poly", "workspace",
+              "tiles")
     WORK = ("patches", "sum_n", "sum_n2", "sum_n3", "sum_nm", "pairs",
             "max_n", "large", "large_sum_n3", "large_sum_n2", "large_sum_nm",
-            "heavy_sum_nm")
+            "heavy_sum_nm", "tile_sum_n3", "tile_sum_n2", "tile_sum_nm")
 
     def phase_times(self) -> dict:
         """Milliseconds per phase (setup: host clock; apply: HIP events)."""

@@ -4,9 +4,10 @@ The oracle's builder (oracle.AdaptiveSetup) is pinned by a numpy brute-force
 restatement in test_oracle.py. Here the GPU builder must reproduce it
 bit for bit: patch centres, radii and cell keys (`centers()` returns the
 cell key as the candidate index), and the three CSR lists as sets. The
-transfer is compared with the oracle's at rel-L2 1e-10, in FP64 and in mixed
-precision (FP32 workspace factors refined in FP64; patches of up to 192
-points are FP64 either way). Support = the patch radius: the local matrices
+transfer is compared with the oracle's at rel-L2 1e-10, in FP64, in mixed
+precision (FP32 workspace factors refined in FP64; patches of up to 264
+points are FP64 either way) and in the autotuned precision. Support = the
+patch radius: the local matrices
 of these clouds have cond(A) <= ~1e4, so FP64 round-off stays near 1e-13.
 """
 
@@ -135,3 +136,40 @@ def test_heavy_patches_chunked_evaluation(rbf, precision, oracle_mod):
     print(f"heavy: {np.sum(m > 4096)} patches above 4096 pairs (max {m.max()})"
           f", large {plan.num_large_patches}; rel-L2 vs oracle {err:.2e}")
     assert err < 1e-10
+
+
+def test_auto_precision_and_tuning_hook(oracle_mod):
+    """Precision "auto" (the size-bin table's choice per field count) gives
+    the oracle's transfer; the tuning hook times each instantiated variant
+    of a populated bin, reports -1 for the others, and leaves later applies
+    unchanged."""
+    import torch
+    S, T, leaf, nmin, nmax = CLOUDS["3d-mixture"]
+    dev = torch.device("cuda:0")
+    V = np.stack([W.franke_3d(*S.T), np.cos(3 * S[:, 0])])
+    ref = oracle_mod.AdaptiveSetup(3, 0x11, S, T, leaf, nmin, nmax).apply(V)
+    plan = pacman.rbf.Plan.adaptive(3, 0x11, torch.from_numpy(S).to(dev),
+                                    torch.from_numpy(T).to(dev), leaf, nmin,
+                                    nmax, precision="auto")
+    dV = torch.from_numpy(V).to(dev)
+    first = plan.apply(dV).cpu().numpy()
+    err = rel_l2(first, ref)
+    print(f"auto: rel-L2 vs oracle {err:.2e}, "
+          f"{plan.num_large_patches} workspace patches")
+    assert err < 1e-10
+    n = np.diff(plan.csr(0)[0])
+    bins = sorted(set(((n[n <= 264] + 2 + 15) // 16).tolist()))
+    for R in bins:
+        for w in (8, 16):
+            ms = plan.time_solve(dV, R, w, reps=2)
+            exists = (w == 8 and R <= 13) or (w == 16 and 11 <= R <= 17)
+            assert (ms > 0) if exists else (ms == -1.0), (R, w, ms)
+    missing = [R for R in range(1, 18) if R not in bins]
+    if missing:
+        assert plan.time_solve(dV, missing[0], 8 if missing[0] <= 13 else 16,
+                               reps=1) == -1.0
+    for prec in ("fp64", "mixed"):
+        ms = plan.time_solve(dV, 0, ws_precision=prec, reps=2)
+        assert (ms > 0) == (plan.num_large_patches > 0)
+    again = plan.apply(dV).cpu().numpy()
+    np.testing.assert_array_equal(again, first)

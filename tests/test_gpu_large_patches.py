@@ -1,12 +1,13 @@
-"""Patches above 192 source points: the workspace solve against the oracle.
+"""Patches above 192 source points against the oracle.
 
 The clouds are a uniform background plus a dense Gaussian cluster, so the
 reference's single patch radius (FindRadius on the probes, which fall mostly
 in the background) gives patches from a few dozen to several hundred points
-inside the cluster. Those patches go through the persistent workspace kernel
-(blocked L D L^T in device memory), in FP64 or in mixed precision (FP32
-factors, FP64 iterative refinement, FP64 fallback per patch); the smaller
-ones through the register-resident kernel.
+inside the cluster. Patches above 264 points go through the persistent
+workspace kernel (blocked L D L^T in device memory), in FP64 or in mixed
+precision (FP32 factors, FP64 iterative refinement, FP64 fallback per
+patch); the smaller ones through the register-resident kernel (16-wave
+workgroups for the size bins above 10 tile rows).
 
 Tolerances: rel-L2 1e-10 against the oracle with the patch radius as support
 (the local matrices are well conditioned: cond(A) <= ~1e3, so FP64 round-off
