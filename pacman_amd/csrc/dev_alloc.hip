@@ -12,7 +12,16 @@
 // It is parked as "pending"; the first request that would reuse a pending
 // block synchronises the device once, after which every pending block is
 // reusable. A failed hipMalloc releases the whole cache and retries.
-// PACMAN_AMD_NO_MEMORY_CACHE=1 in the environment turns the cache off.
+// PACMAN_AMD_NO_MEMORY_CACHE=1 in the environment turns the cache off; a
+// released block then goes to hipFree, which waits for the device.
+//
+// Callers rely on these two rules: the setup code (scan_device, build_grid,
+// make_centers, patch_sources, patch_targets, build_adaptive) releases its
+// temporaries while the kernels that read them may still run, with no
+// stream synchronisation of its own. A stream-ordered allocator
+// (hipMallocAsync / hipFreeAsync) in place of this cache would keep that
+// safe only on one stream; any other change has to revisit those call
+// sites.
 
 #include <hip/hip_runtime.h>
 

@@ -20,3 +20,19 @@ def oracle_mod():
     import oracle
     oracle.build()
     return oracle
+
+
+@pytest.fixture
+def measured(request):
+    """record(quantity, value, tolerance): prints the measured error beside
+    its bound and, when PACMAN_AMD_MEASURED_LOG names a file, appends
+    "test<TAB>quantity<TAB>value<TAB>tolerance" to it."""
+    path = os.environ.get("PACMAN_AMD_MEASURED_LOG")
+
+    def record(quantity, value, tolerance):
+        line = f"{request.node.nodeid}\t{quantity}\t{value:.3e}\t{tolerance:.1e}"
+        print(f"[measured] {line}", flush=True)
+        if path:
+            with open(path, "a") as fh:
+                fh.write(line + "\n")
+    return record

@@ -67,6 +67,48 @@ def test_device_cache_reuse_gives_identical_results():
     assert np.array_equal(first, fresh)
 
 
+_NO_CACHE_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import pacman_amd as pacman
+from pacman_amd import _lib
+rng = np.random.default_rng(31)
+S = rng.uniform(0, 1, (30000, 3))
+T = rng.uniform(0, 1, (20000, 3))
+f = rng.normal(size=(2, len(S)))
+outs = []
+for _ in range(3):  # plans built and destroyed back to back
+    outs.append(pacman.rbf.Plan(3, 0x11, S, T, 80, 0.15, -1.0).apply(f))
+    assert _lib.load().pacman_device_cache_bytes() == 0
+assert all(np.array_equal(outs[0], o) for o in outs)
+np.save(sys.argv[2], outs[0])
+print("no-cache ok", flush=True)
+"""
+
+
+def test_setup_and_apply_without_device_cache(tmp_path):
+    """With PACMAN_AMD_NO_MEMORY_CACHE=1 set before the GPU is initialised
+    (a fresh process), every temporary goes back to hipFree as soon as it is
+    released, while kernels may still read it: hipFree waits for the device,
+    so the transfer must equal the cached run's."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = str(tmp_path / "nocache.npy")
+    env = dict(os.environ, PACMAN_AMD_NO_MEMORY_CACHE="1")
+    r = subprocess.run([sys.executable, "-c", _NO_CACHE_SCRIPT, root, out],
+                       env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rng = np.random.default_rng(31)
+    S = rng.uniform(0, 1, (30000, 3))
+    T = rng.uniform(0, 1, (20000, 3))
+    f = rng.normal(size=(2, len(S)))
+    cached = pacman.rbf.Plan(3, 0x11, S, T, 80, 0.15, -1.0).apply(f)
+    assert np.array_equal(np.load(out), cached)
+
+
 def test_graph_recaptured_after_buffers_grow():
     """An eager apply with more fields reallocates the plan's scratch buffers;
     the next graph apply with the old field count must capture again (a
