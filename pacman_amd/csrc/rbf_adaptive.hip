@@ -35,6 +35,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <vector>
 
@@ -195,7 +197,8 @@ void radix_sort_pairs(uint64_t *k0, int *v0, uint64_t *k1, int *v1, int64_t n,
   for (int shift = 0; shift < bits; shift += 8) {
     rs_hist_kernel<<<nblk, kThreads, 0, s>>>(ka, n, shift, nblk, hist.get());
     PACMAN_HIP_CHECK(hipGetLastError());
-    exclusive_scan_i32_to_i64(hist.get(), hoff.get(), (int64_t)256 * nblk, s);
+    exclusive_scan_i32_to_i64_async(hist.get(), hoff.get(), (int64_t)256 * nblk,
+                                    s);
     rs_scatter_kernel<<<nblk, kThreads, 0, s>>>(ka, va, n, shift, nblk,
                                                 hoff.get(), kb, vb);
     PACMAN_HIP_CHECK(hipGetLastError());
@@ -227,31 +230,38 @@ struct TNode {
   int d, slo, shi, tlo, thi, pad;
 };
 
-__global__ void tree_level_kernel(const TNode *in, int nIn, TNode *out,
-                                  int *counters, int maxOut, uint64_t *leafP,
-                                  int *leafD, int maxLeaf, SortedSet S,
-                                  SortedSet T, int leaf, int DB) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nIn)
-    return;
-  const TNode nd = in[i];
-  if (nd.shi - nd.slo > leaf && nd.d < DB) {
-    const uint64_t key = (2 * nd.p + 1) << (DB - nd.d - 1);
-    const int ms = lower_bound_u64(S.code, nd.slo, nd.shi, key);
-    const int mt = lower_bound_u64(T.code, nd.tlo, nd.thi, key);
-    const TNode c[2] = {{2 * nd.p, nd.d + 1, nd.slo, ms, nd.tlo, mt, 0},
-                        {2 * nd.p + 1, nd.d + 1, ms, nd.shi, mt, nd.thi, 0}};
-    for (int h = 0; h < 2; ++h)
-      if (c[h].shi > c[h].slo || c[h].thi > c[h].tlo) {
-        const int o = atomicAdd(&counters[0], 1);
-        if (o < maxOut)
-          out[o] = c[h];
+// One depth of the tree: the *inCount nodes of `in` (read on the device, so
+// the depths are launched back to back without a host round trip; a
+// grid-stride loop covers any count), children appended to `out`, leaves
+// with targets to the leaf list. Appends past maxOut / maxLeaf are counted
+// but not stored (the host checks the counts once at the end).
+__global__ void tree_level_kernel(const TNode *in, const int *inCount,
+                                  TNode *out, int *outCount, int maxOut,
+                                  int *leafCount, uint64_t *leafP, int *leafD,
+                                  int maxLeaf, SortedSet S, SortedSet T,
+                                  int leaf, int DB) {
+  const int nIn = min(*inCount, maxOut);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nIn;
+       i += gridDim.x * blockDim.x) {
+    const TNode nd = in[i];
+    if (nd.shi - nd.slo > leaf && nd.d < DB) {
+      const uint64_t key = (2 * nd.p + 1) << (DB - nd.d - 1);
+      const int ms = lower_bound_u64(S.code, nd.slo, nd.shi, key);
+      const int mt = lower_bound_u64(T.code, nd.tlo, nd.thi, key);
+      const TNode c[2] = {{2 * nd.p, nd.d + 1, nd.slo, ms, nd.tlo, mt, 0},
+                          {2 * nd.p + 1, nd.d + 1, ms, nd.shi, mt, nd.thi, 0}};
+      for (int h = 0; h < 2; ++h)
+        if (c[h].shi > c[h].slo || c[h].thi > c[h].tlo) {
+          const int o = atomicAdd(outCount, 1);
+          if (o < maxOut)
+            out[o] = c[h];
+        }
+    } else if (nd.thi > nd.tlo) {
+      const int o = atomicAdd(leafCount, 1);
+      if (o < maxLeaf) {
+        leafP[o] = nd.p;
+        leafD[o] = nd.d;
       }
-  } else if (nd.thi > nd.tlo) {
-    const int o = atomicAdd(&counters[1], 1);
-    if (o < maxLeaf) {
-      leafP[o] = nd.p;
-      leafD[o] = nd.d;
     }
   }
 }
@@ -601,7 +611,7 @@ struct CellEvalArgs {
   int *nextD;
   int maxNext;
   uint64_t *doneKey;
-  int *doneD;
+  int *doneD, *doneN;     // doneN: sources within the final radius
   double *doneC, *doneR;  // doneC: 3 per entry
   int maxDone;
   int *counters;          // [0] next, [1] done
@@ -648,6 +658,7 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
     if (o < e.maxDone) {
       e.doneKey[o] = p << (e.f.DB - d);
       e.doneD[o] = d;
+      e.doneN[o] = n;
       e.doneC[3 * o] = c[0];
       e.doneC[3 * o + 1] = c[1];
       e.doneC[3 * o + 2] = c[2];
@@ -656,16 +667,20 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
   }
 }
 
-// Final patch arrays in key order.
+// Final patch arrays in key order; srcCnt[k]: the patch's source count (the
+// ball count of its final radius from cell_eval_kernel, the same predicate
+// as the membership scan).
 __global__ void patch_arrays_kernel(const int *order, int K, const int *doneD,
-                                    const double *doneC, const double *doneR,
-                                    double supportScale, double *cx,
-                                    double *cy, double *cz, double *r,
-                                    double *rinv, double *dinv, int *depth) {
+                                    const int *doneN, const double *doneC,
+                                    const double *doneR, double supportScale,
+                                    double *cx, double *cy, double *cz,
+                                    double *r, double *rinv, double *dinv,
+                                    int *depth, int *srcCnt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K)
     return;
   const int o = order[k];
+  srcCnt[k] = doneN[o];
   cx[k] = doneC[3 * o];
   cy[k] = doneC[3 * o + 1];
   cz[k] = doneC[3 * o + 2];
@@ -689,6 +704,7 @@ struct MemberArgs {
   int *cnt;
   const int64_t *off;
   int *out, *pairPatch;
+  int *mismatch;  // fill pass: set when a list's fill differs from its count
 };
 
 // Members of patch k found from the starting cell onlyCell (-1: all), the
@@ -703,6 +719,9 @@ __device__ inline int64_t member_scan(const MemberArgs &m, int k, bool fill,
   const double c[3] = {m.cx[k], m.cy[k], m.cz[k]};
   const double rad = m.r[k], radMember = rad - kEps;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // the fill never writes past the patch's list (its count came from
+  // another pass; a disagreement is reported, not written out of range)
+  const int64_t end = fill ? m.off[k + 1] : 0;
   ball_scan<INSIDE>(
       m.P, m.f, c, rad, stk,
       [&](int rs, int re) {
@@ -717,8 +736,8 @@ __device__ inline int64_t member_scan(const MemberArgs &m, int k, bool fill,
           dd = dist3(c[0], c[1], c[2], x, y, z);
         const bool hit = valid && (inside || dd <= rad);
         const uint64_t bal = __ballot(hit);
-        if (fill && hit) {
-          const int64_t o = w + __popcll(bal & below);
+        const int64_t o = w + __popcll(bal & below);
+        if (fill && hit && o < end) {
           if (m.targets) {
             const unsigned member = dd <= radMember ? 1u : 0u;
             m.out[o] = (int)((unsigned)p | (member << 31));
@@ -747,6 +766,8 @@ __global__ __launch_bounds__(kThreads) void member_kernel(MemberArgs m,
       member_scan<INSIDE>(m, k, fill, fill ? m.off[k] : 0, stk, -1);
   if (!fill && (threadIdx.x & 63) == 0)
     m.cnt[k] = (int)w;
+  if (fill && (threadIdx.x & 63) == 0 && w != m.off[k + 1])
+    atomicOr(m.mismatch, 1);
 }
 
 // Patches with r > rHeavy, listed by heavy_select_kernel (any order: each
@@ -781,9 +802,12 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void member_heavy_kernel(
         for (int jj = 0; jj < j; ++jj)
           w += cc[jj];
       }
+      const int64_t w0 = w;
       w = member_scan<true>(m, k, fill, w, stk, j);
       if (!fill && (threadIdx.x & 63) == 0)
         cc[j] = (int)w;
+      if (fill && (threadIdx.x & 63) == 0 && w != w0 + cc[j])
+        atomicOr(m.mismatch, 1);
     }
     __syncthreads();
     if (!fill && threadIdx.x == 0) {
@@ -799,6 +823,36 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void member_heavy_kernel(
 unsigned blocks(int64_t n, int per) {
   return (unsigned)std::max<int64_t>(1, (n + per - 1) / per);
 }
+
+// A second stream per device (default priority), kept for the process: the
+// workgroup scans of the large target balls run beside the one-wave scans
+// of the others. Each build forks and joins it with events of its own.
+hipStream_t side_stream() {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> per;
+  int dev = 0;
+  PACMAN_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  hipStream_t &st = per[dev];
+  if (!st)
+    PACMAN_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  return st;
+}
+
+struct ForkJoin {
+  hipStream_t side;
+  hipEvent_t fork = nullptr, join = nullptr;
+  ForkJoin() : side(side_stream()) {
+    PACMAN_HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    PACMAN_HIP_CHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  }
+  ~ForkJoin() {
+    (void)hipEventDestroy(fork);
+    (void)hipEventDestroy(join);
+  }
+  ForkJoin(const ForkJoin &) = delete;
+  ForkJoin &operator=(const ForkJoin &) = delete;
+};
 
 } // namespace
 
@@ -878,40 +932,50 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   const SortedSet T{tcode.get(), out.tgt.x.get(), out.tgt.y.get(),
                     out.tgt.z.get(), (int)M};
 
-  // 2. the tree, one launch per depth
+  // 2. the tree, one launch per depth. The nodes of a depth below the root
+  // are children of split cells, which are disjoint and hold more than
+  // `leaf` sources each: at most 2 N / (leaf + 1) of them.
   const int64_t maxLeaf = std::max<int64_t>(M, 1);
+  const int maxNodes =
+      (int)std::min<int64_t>(INT_MAX / 2, 2 * (N / (ap.leaf + 1)) + 2);
   DevBuf<uint64_t> leafP(maxLeaf);
-  DevBuf<int> leafD(maxLeaf), ctr(2);
-  std::vector<TNode> root{{0, 0, 0, (int)N, 0, (int)M, 0}};
-  DevBuf<TNode> level(1);
-  PACMAN_HIP_CHECK(hipMemcpyAsync(level.get(), root.data(), sizeof(TNode),
+  DevBuf<int> leafD(maxLeaf);
+  // lv[d]: nodes at depth d (lv[0] = the root), lv[DB + 2]: leaves
+  std::vector<int> lvHost(f.DB + 3, 0);
+  lvHost[0] = 1;
+  DevBuf<int> lv(f.DB + 3), ctr(2);
+  DevBuf<TNode> levels[2] = {DevBuf<TNode>(maxNodes), DevBuf<TNode>(maxNodes)};
+  const TNode root{0, 0, 0, (int)N, 0, (int)M, 0};
+  PACMAN_HIP_CHECK(hipMemcpyAsync(levels[0].get(), &root, sizeof(TNode),
                                   hipMemcpyHostToDevice, s));
-  int nLevel = 1, nLeaves = 0;
-  for (int d = 0; d <= f.DB && nLevel > 0; ++d) {
-    const int maxOut = 2 * nLevel;
-    DevBuf<TNode> next(maxOut);
-    int h[2] = {0, nLeaves};
-    PACMAN_HIP_CHECK(
-        hipMemcpyAsync(ctr.get(), h, 8, hipMemcpyHostToDevice, s));
-    tree_level_kernel<<<blocks(nLevel, kThreads), kThreads, 0, s>>>(
-        level.get(), nLevel, next.get(), ctr.get(), maxOut, leafP.get(),
+  PACMAN_HIP_CHECK(hipMemcpyAsync(lv.get(), lvHost.data(),
+                                  sizeof(int) * lvHost.size(),
+                                  hipMemcpyHostToDevice, s));
+  const unsigned tgrid =
+      std::min<unsigned>(blocks(maxNodes, kThreads), 2048u);
+  for (int d = 0; d <= f.DB; ++d) {
+    tree_level_kernel<<<tgrid, kThreads, 0, s>>>(
+        levels[d & 1].get(), lv.get() + d, levels[(d + 1) & 1].get(),
+        lv.get() + d + 1, maxNodes, lv.get() + f.DB + 2, leafP.get(),
         leafD.get(), (int)maxLeaf, S, T, ap.leaf, f.DB);
     PACMAN_HIP_CHECK(hipGetLastError());
-    PACMAN_HIP_CHECK(
-        hipMemcpyAsync(h, ctr.get(), 8, hipMemcpyDeviceToHost, s));
-    PACMAN_HIP_CHECK(hipStreamSynchronize(s));
-    if (h[1] > maxLeaf)
-      throw std::runtime_error("adaptive patches: leaf list overflow");
-    nLevel = h[0];
-    nLeaves = h[1];
-    level = std::move(next);
   }
+  PACMAN_HIP_CHECK(hipMemcpyAsync(lvHost.data(), lv.get(),
+                                  sizeof(int) * lvHost.size(),
+                                  hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+  for (int d = 0; d <= f.DB + 1; ++d)
+    if (lvHost[d] > maxNodes)
+      throw std::runtime_error("adaptive patches: tree level overflow");
+  const int nLeaves = lvHost[f.DB + 2];
+  if (nLeaves > maxLeaf)
+    throw std::runtime_error("adaptive patches: leaf list overflow");
 
   // 3-4. radii; halve patches above nmax until none is
   const int kq = (int)std::min<int64_t>(ap.nmin, N);
   const int64_t maxDone = std::max<int64_t>(M, 1);
   DevBuf<uint64_t> doneKey(maxDone);
-  DevBuf<int> doneD(maxDone);
+  DevBuf<int> doneD(maxDone), doneN(maxDone);
   DevBuf<double> doneC(3 * maxDone), doneR(maxDone);
   DevBuf<uint64_t> curP = std::move(leafP);
   DevBuf<int> curD = std::move(leafD);
@@ -938,6 +1002,7 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
     e.maxNext = maxNext;
     e.doneKey = doneKey.get();
     e.doneD = doneD.get();
+    e.doneN = doneN.get();
     e.doneC = doneC.get();
     e.doneR = doneR.get();
     e.maxDone = (int)maxDone;
@@ -979,13 +1044,16 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   out.dinv.alloc(K1);
   out.key.alloc(K1);
   out.depth.alloc(K1);
+  DevBuf<int> cnt(K1), mismatch(1);
+  PACMAN_HIP_CHECK(hipMemsetAsync(mismatch.get(), 0, sizeof(int), s));
   // doneKey is now sorted; the other entries are in append order, reached
   // through `order`
   if (K > 0) {
     patch_arrays_kernel<<<blocks(K, kThreads), kThreads, 0, s>>>(
-        order.get(), K, doneD.get(), doneC.get(), doneR.get(),
+        order.get(), K, doneD.get(), doneN.get(), doneC.get(), doneR.get(),
         ap.supportScale, out.cx.get(), out.cy.get(), out.cz.get(),
-        out.r.get(), out.rinv.get(), out.dinv.get(), out.depth.get());
+        out.r.get(), out.rinv.get(), out.dinv.get(), out.depth.get(),
+        cnt.get());
     PACMAN_HIP_CHECK(hipGetLastError());
     PACMAN_HIP_CHECK(hipMemcpyAsync(out.key.get(), doneKey.get(), 8 * K,
                                     hipMemcpyDeviceToDevice, s));
@@ -999,18 +1067,15 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   m.cz = out.cz.get();
   m.r = out.r.get();
   m.K = K;
-  DevBuf<int> cnt(K1);
+  m.mismatch = mismatch.get();
   const unsigned wb = blocks(K, kThreads / 64);
-  // sources (at most nmax per patch: never a large ball)
+  // sources (at most nmax per patch: never a large ball); their counts come
+  // from the radius pass (patch_arrays_kernel)
   m.P = S;
   m.targets = false;
   m.rHeavy = DBL_MAX;
   m.cnt = cnt.get();
   out.srcOff.alloc(K + 1);
-  if (K > 0) {
-    member_kernel<false><<<wb, kThreads, 0, s>>>(m, false);
-    PACMAN_HIP_CHECK(hipGetLastError());
-  }
   const int64_t tot = exclusive_scan_i32_to_i64(cnt.get(), out.srcOff.get(), K, s);
   if (tot > INT_MAX)
     throw std::runtime_error("patch source lists exceed 2^31 entries");
@@ -1036,6 +1101,7 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
                      : DBL_MAX;
   }
   DevBuf<int> hList(K1), hCount(1), cellCnt((size_t)K1 * kCells);
+  ForkJoin fj;
   const unsigned hb = (unsigned)std::min<int64_t>(K1, 1024);
   PairSet &ps = out.pairs;
   ps.off.alloc(K + 1);
@@ -1044,11 +1110,15 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
     heavy_select_kernel<<<blocks(K, kThreads), kThreads, 0, s>>>(
         out.r.get(), K, m.rHeavy, hList.get(), hCount.get());
     PACMAN_HIP_CHECK(hipGetLastError());
-    member_kernel<true><<<wb, kThreads, 0, s>>>(m, false);
-    PACMAN_HIP_CHECK(hipGetLastError());
-    member_heavy_kernel<<<hb, 64 * kHeavyWaves, 0, s>>>(
+    PACMAN_HIP_CHECK(hipEventRecord(fj.fork, s));
+    PACMAN_HIP_CHECK(hipStreamWaitEvent(fj.side, fj.fork, 0));
+    member_heavy_kernel<<<hb, 64 * kHeavyWaves, 0, fj.side>>>(
         m, hList.get(), hCount.get(), cellCnt.get(), false);
     PACMAN_HIP_CHECK(hipGetLastError());
+    member_kernel<true><<<wb, kThreads, 0, s>>>(m, false);
+    PACMAN_HIP_CHECK(hipGetLastError());
+    PACMAN_HIP_CHECK(hipEventRecord(fj.join, fj.side));
+    PACMAN_HIP_CHECK(hipStreamWaitEvent(s, fj.join, 0));
   }
   const int64_t P = exclusive_scan_i32_to_i64(cnt.get(), ps.off.get(), K, s);
   if (P > INT_MAX)
@@ -1060,14 +1130,24 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   m.out = ps.tgt.get();
   m.pairPatch = pairPatch.get();
   if (K > 0) {
-    member_kernel<true><<<wb, kThreads, 0, s>>>(m, true);
-    PACMAN_HIP_CHECK(hipGetLastError());
-    member_heavy_kernel<<<hb, 64 * kHeavyWaves, 0, s>>>(
+    PACMAN_HIP_CHECK(hipEventRecord(fj.fork, s));
+    PACMAN_HIP_CHECK(hipStreamWaitEvent(fj.side, fj.fork, 0));
+    member_heavy_kernel<<<hb, 64 * kHeavyWaves, 0, fj.side>>>(
         m, hList.get(), hCount.get(), cellCnt.get(), true);
     PACMAN_HIP_CHECK(hipGetLastError());
+    member_kernel<true><<<wb, kThreads, 0, s>>>(m, true);
+    PACMAN_HIP_CHECK(hipGetLastError());
+    PACMAN_HIP_CHECK(hipEventRecord(fj.join, fj.side));
+    PACMAN_HIP_CHECK(hipStreamWaitEvent(s, fj.join, 0));
   }
   build_target_slots(ps, pairPatch.get(), M, s);
+  int bad = 0;
+  PACMAN_HIP_CHECK(hipMemcpyAsync(&bad, mismatch.get(), sizeof(int),
+                                  hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // temporaries in use above
+  if (bad)
+    throw std::runtime_error(
+        "adaptive patches: a membership list's fill differs from its count");
 }
 
 } // namespace pacman_amd

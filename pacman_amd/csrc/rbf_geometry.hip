@@ -786,6 +786,11 @@ int64_t exclusive_scan_i32_to_i64(const int *in, int64_t *out, int64_t n,
   return tot;
 }
 
+void exclusive_scan_i32_to_i64_async(const int *in, int64_t *out, int64_t n,
+                                     hipStream_t s) {
+  scan_device<int, int64_t>(in, out, n, s, false, nullptr);
+}
+
 int exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s) {
   int tot = 0;
   scan_device<int, int>(in, out, n, s, true, &tot);
@@ -1048,7 +1053,7 @@ void build_target_slots(PairSet &ps, const int *pairPatch, int64_t M,
         ps.tgt.get(), P, tcnt.get());
     PACMAN_HIP_CHECK(hipGetLastError());
   }
-  exclusive_scan_i32_to_i64(tcnt.get(), ps.toff.get(), M, s);
+  exclusive_scan_i32_to_i64_async(tcnt.get(), ps.toff.get(), M, s);
   if (P > 0) {
     pair_scatter_kernel<<<blocks_for(P, kBlock), kBlock, 0, s>>>(
         ps.tgt.get(), pairPatch, P, ps.toff.get(), tfill.get(),

@@ -94,6 +94,9 @@ void build_grid(const double *pts, int64_t n, int dim, const double *lo,
 // Exclusive scan of n int32 counts into n+1 int64 offsets; returns the total.
 int64_t exclusive_scan_i32_to_i64(const int *in, int64_t *out, int64_t n,
                                   hipStream_t s);
+// The same without reading the total back (no host synchronisation).
+void exclusive_scan_i32_to_i64_async(const int *in, int64_t *out, int64_t n,
+                                     hipStream_t s);
 // Exclusive scan of n int32 flags into n+1 int32 offsets; returns the total.
 int exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s);
 

@@ -52,10 +52,13 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
     setup_reference(dSrc, dTgt, s, tSetup);
   auto tPhase = std::chrono::steady_clock::now();
 
-  // Size bins by register tile.
-  std::vector<int64_t> off(K_ + 1);
+  // Size bins by register tile; per-patch source and pair counts on the
+  // host (one synchronisation for both).
+  std::vector<int64_t> off(K_ + 1), poff(K_ + 1);
   PACMAN_HIP_CHECK(hipMemcpyAsync(off.data(), srcOff_.get(), 8 * (K_ + 1),
                                   hipMemcpyDeviceToHost, s));
+  PACMAN_HIP_CHECK(hipMemcpyAsync(poff.data(), pairs_.off.get(),
+                                  8 * (K_ + 1), hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   maxN_ = 0;
   for (int64_t kk = 0; kk < K_; ++kk)
@@ -68,7 +71,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   counters_.alloc(kWsCounters);
   launch_ws_reset(counters_.get(), true, s);
   plan_workspace(s);
-  plan_heavy(s);
+  plan_heavy(poff, s);
   bins_.resize(solve_fields_per_launch());
   bins_for(1, s);
   pv_.alloc(std::max<int64_t>(K_, 1));
@@ -239,17 +242,20 @@ const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
   for (int64_t kk = 0; kk < K_; ++kk)
     if (patchN_[kk] <= tileMaxN_)  // larger ones: workspace path
       bins[solve_bin_rows(patchN_[kk], F)].push_back((int)kk);
+  // every bin's patch list in one buffer, one copy
   for (int R = 0; R < (int)bins.size(); ++R) {
     if (bins[R].empty())
       continue;
-    B.host.push_back(std::move(bins[R]));
-    const std::vector<int> &h = B.host.back();
-    DevBuf<int> d(h.size());
-    PACMAN_HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), 4 * h.size(),
-                                    hipMemcpyHostToDevice, s));
-    B.dev.push_back(std::move(d));
-    B.count.push_back((int)h.size());
+    B.first.push_back((int64_t)B.host.size());
+    B.host.insert(B.host.end(), bins[R].begin(), bins[R].end());
+    B.count.push_back((int)bins[R].size());
     B.R.push_back(R);
+  }
+  if (!B.host.empty()) {
+    B.dev.alloc(B.host.size());
+    PACMAN_HIP_CHECK(hipMemcpyAsync(B.dev.get(), B.host.data(),
+                                    4 * B.host.size(), hipMemcpyHostToDevice,
+                                    s));
   }
   B.built = true;
   return B;
@@ -609,7 +615,7 @@ void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
       launch_op_transpose(a, K_, opOff_.get(), opStore(), contribS_.get(), s);
     } else {
       for (size_t b = 0; b < B.R.size(); ++b) {
-        a.patches = B.dev[b].get();
+        a.patches = B.dev.get() + B.first[b];
         a.count = B.count[b];
         launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
       }
@@ -716,7 +722,7 @@ void RbfPumPlan::buildOperator(hipStream_t s, bool fp32) {
   for (int64_t r0 = 0; r0 < maxRows; r0 += Fb) {
     a.opRow0 = (int)r0;
     for (size_t b = 0; b < B.R.size(); ++b) {
-      a.patches = B.dev[b].get();
+      a.patches = B.dev.get() + B.first[b];
       a.count = B.count[b];
       launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
     }
@@ -798,7 +804,7 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     }
     const Bins &B = bins_[nf - 1];
     for (size_t b = 0; b < B.R.size(); ++b) {
-      a.patches = B.dev[b].get();
+      a.patches = B.dev.get() + B.first[b];
       a.count = B.count[b];
       launch_solve_f64(B.R[b], kTune.waves[nf - 1][B.R[b]], prm_.kind, a,
                        bu.coef, s);
@@ -1016,14 +1022,10 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
 // workgroup evaluating for milliseconds; their alpha goes to alphaHeavy_
 // (n x 8 doubles each) and chunks of kHeavyChunk pairs are evaluated by
 // separate workgroups.
-void RbfPumPlan::plan_heavy(hipStream_t s) {
+void RbfPumPlan::plan_heavy(const std::vector<int64_t> &poff, hipStream_t s) {
   nHeavyChunks_ = 0;
   if (K_ == 0)
     return;
-  std::vector<int64_t> poff(K_ + 1);
-  PACMAN_HIP_CHECK(hipMemcpyAsync(poff.data(), pairs_.off.get(),
-                                  8 * (K_ + 1), hipMemcpyDeviceToHost, s));
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   std::vector<int64_t> hoff(K_, -1);
   std::vector<int2> chunks;
   int64_t total = 0;
@@ -1138,7 +1140,7 @@ double RbfPumPlan::timeSolve(const double *dVals, int F, int R, int waves,
       launch_ws_reset(counters_.get(), true, s);
       PACMAN_HIP_CHECK(hipEventRecord(e[0], s));
       if (R > 0) {
-        a.patches = B.dev[b].get();
+        a.patches = B.dev.get() + B.first[b];
         a.count = B.count[b];
         launch_solve_f64(R, waves, prm_.kind, a, coef_.get(), s);
       } else {

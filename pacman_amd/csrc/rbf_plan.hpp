@@ -157,8 +157,9 @@ private:
   struct Bins {
     bool built = false;
     std::vector<int> R, count;
-    std::vector<std::vector<int>> host;  // kept alive for the async copies
-    std::vector<DevBuf<int>> dev;
+    std::vector<int64_t> first;  // bin b: patches dev[first[b] ..]
+    std::vector<int> host;       // kept alive for the async copy
+    DevBuf<int> dev;
   };
   std::vector<Bins> bins_;
   // Workspace path: patches above tileMaxN_ sources, largest first, in two
@@ -180,7 +181,7 @@ private:
   DevBuf<double> patchR_, patchRinv_, patchDinv_;  // adaptive plans only
   // Patches with more than kHeavyPairs target pairs: evaluated in chunks
   // of kHeavyChunk pairs by their own launch.
-  void plan_heavy(hipStream_t s);
+  void plan_heavy(const std::vector<int64_t> &poff, hipStream_t s);
   DevBuf<int64_t> heavyOff_;
   DevBuf<double> alphaHeavy_;
   DevBuf<int2> heavyChunks_;
