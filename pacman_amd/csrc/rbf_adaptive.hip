@@ -54,10 +54,17 @@ struct Frame {
   double hmin[3 * kBits + 1];  // smallest cell edge at each depth
 };
 
+// A point set in code order. tab[c] = the first position whose code is at
+// least c << tabShift, for the 2^tabBits coarse cells of the code's leading
+// bits and c = 2^tabBits (= n): a lower bound in the whole set starts inside
+// the coarse cell of its key (an L2-resident table read instead of ~22
+// dependent loads over all n codes).
 struct SortedSet {
   const uint64_t *code;
   const double *x, *y, *z;
   int n;
+  const int *tab;
+  int tabShift, tabBits;
 };
 
 __host__ __device__ inline uint64_t quantize(double x, double lo, double ext) {
@@ -94,6 +101,15 @@ __device__ inline int lower_bound_u64(const uint64_t *a, int lo, int hi,
       hi = mid;
   }
   return lo;
+}
+
+// First position in the whole of S whose code is >= key (as
+// lower_bound_u64(S.code, 0, S.n, key)).
+__device__ inline int code_lower_bound(const SortedSet &S, uint64_t key) {
+  const uint64_t c = key >> S.tabShift;
+  if (c >= ((uint64_t)1 << S.tabBits))
+    return S.n;
+  return lower_bound_u64(S.code, S.tab[c], S.tab[c + 1], key);
 }
 
 // ---- codes and the radix sort ---------------------------------------------
@@ -209,6 +225,16 @@ void radix_sort_pairs(uint64_t *k0, int *v0, uint64_t *k1, int *v1, int64_t n,
     PACMAN_HIP_CHECK(hipMemcpyAsync(k0, ka, 8 * n, hipMemcpyDeviceToDevice, s));
     PACMAN_HIP_CHECK(hipMemcpyAsync(v0, va, 4 * n, hipMemcpyDeviceToDevice, s));
   }
+}
+
+// tab[c] = lower_bound(code, c << shift) for c in [0, 2^bits]; tab[2^bits] = n.
+__global__ void code_table_kernel(const uint64_t *code, int n, int shift,
+                                  int bits, int *tab) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nc = (int64_t)1 << bits;
+  if (c > nc)
+    return;
+  tab[c] = c == nc ? n : lower_bound_u64(code, 0, n, (uint64_t)c << shift);
 }
 
 __global__ void gather_sorted_kernel(const double *pts, int64_t n, int dim,
@@ -370,6 +396,54 @@ struct ScanNode {
   int d, rs, re;
 };
 
+// A chunk of at most 64 consecutive positions of one scanned range.
+struct ScanChunk {
+  int start, count, inside;
+};
+constexpr int kChunks = 64;  // chunks queued per wave before a flush
+constexpr int kChunkGroup = 4;  // chunks whose loads are issued together
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// fn over the queued chunks, in queue order, one 64-lane pass per chunk
+// (lane i: position start + i, valid below count); the coordinates of
+// kChunkGroup chunks are loaded before the first of them is used, so their
+// memory latencies overlap.
+template <class Fn>
+__device__ inline void flush_chunks(const SortedSet &S, const ScanChunk *pend,
+                                    int np, Fn &&fn) {
+  const int lane = threadIdx.x & 63;
+  for (int g = 0; g < np; g += kChunkGroup) {
+    double x[kChunkGroup], y[kChunkGroup], z[kChunkGroup];
+    int q[kChunkGroup];
+    bool v[kChunkGroup], in[kChunkGroup];
+#pragma unroll
+    for (int u = 0; u < kChunkGroup; ++u) {
+      x[u] = y[u] = z[u] = 0.0;
+      q[u] = 0;
+      v[u] = in[u] = false;
+      if (g + u < np) {
+        const ScanChunk ch = pend[g + u];
+        q[u] = ch.start + lane;
+        v[u] = lane < ch.count;
+        in[u] = ch.inside != 0;
+        const int qq = v[u] ? q[u] : ch.start;
+        x[u] = S.x[qq];
+        y[u] = S.y[qq];
+        z[u] = S.z[qq];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kChunkGroup; ++u)
+      if (g + u < np)
+        fn(q[u], v[u], in[u], x[u], y[u], z[u]);
+  }
+}
+
 // fn(position, valid, inside, x, y, z) over a superset of the points of S
 // within r of c (each candidate once, with its coordinates; ranges in a
 // fixed order, so the order is deterministic): the 3^D cells around c at the
@@ -382,10 +456,15 @@ struct ScanNode {
 // scanned with inside = true (INSIDE = false: no such test, for balls that
 // never cover whole ranges). onlyCell >= 0 restricts the scan to that one of
 // the 3^D starting cells (a workgroup shares a large ball out that way).
+// The ranges to scan are cut into chunks of 64 positions and queued in
+// `pend` (kChunks per wave); fn sees them in the order the ranges are
+// reached, so the positions of the hits are the same as scanning each range
+// when it is reached.
 template <bool INSIDE = true, class Whole, class Fn>
 __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
                                  const double *c, double r, ScanNode *stk,
-                                 Whole &&whole, Fn &&fn, int onlyCell = -1) {
+                                 ScanChunk *pend, Whole &&whole, Fn &&fn,
+                                 int onlyCell = -1) {
   const int lane = threadIdx.x & 63;
   const int d0 = depth_for(f, r);
   const int nr = f.D == 1 ? 3 : f.D == 2 ? 9 : 27;
@@ -411,8 +490,8 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
     if (ok) {
       pref = cell_prefix(j, f.D, d0);
       const int sh = f.DB - d0;
-      rs = lower_bound_u64(S.code, 0, S.n, pref << sh);
-      re = lower_bound_u64(S.code, rs, S.n, (pref + 1) << sh);
+      rs = code_lower_bound(S, pref << sh);
+      re = code_lower_bound(S, (pref + 1) << sh);
     }
   }
   // push the non-empty ones in reverse lane order (popped in lane order)
@@ -428,6 +507,7 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int top = nk;
+  int np = 0;
   while (top > 0) {
     const ScanNode nd = stk[top - 1];
     --top;
@@ -462,19 +542,32 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       continue;
     }
-    for (int b = nd.rs; b < nd.re; b += 64) {
-      const int q = b + lane;
-      const int qq = q < nd.re ? q : nd.rs;
-      fn(q, q < nd.re, inside, S.x[qq], S.y[qq], S.z[qq]);
+    for (int b = nd.rs; b < nd.re;) {
+      if (np == kChunks) {
+        flush_chunks(S, pend, np, fn);
+        np = 0;
+        wave_sync_lds();  // every lane has read the queue
+      }
+      const int take = min((nd.re - b + 63) / 64, kChunks - np);
+      if (lane < take) {
+        const int st = b + 64 * lane;
+        pend[np + lane] = ScanChunk{st, min(64, nd.re - st), inside ? 1 : 0};
+      }
+      np += take;
+      b += 64 * take;
     }
+    wave_sync_lds();  // queue entries visible; popped entry read by all
   }
+  flush_chunks(S, pend, np, fn);
+  wave_sync_lds();  // the queue may be reused by the next scan
 }
 
 __device__ inline int ball_count(const SortedSet &S, const Frame &f,
-                                 const double *c, double r, ScanNode *stk) {
+                                 const double *c, double r, ScanNode *stk,
+                                 ScanChunk *pend) {
   int cnt = 0;
   ball_scan(
-      S, f, c, r, stk,
+      S, f, c, r, stk, pend,
       [&](int rs, int re) {
         cnt += re - rs;
         return true;
@@ -489,10 +582,11 @@ __device__ inline int ball_count(const SortedSet &S, const Frame &f,
 
 // #{p : sqdiff(c, p) <= t} (candidates within sqrt(t)).
 __device__ inline int count_d2(const SortedSet &S, const Frame &f,
-                               const double *c, double t, ScanNode *stk) {
+                               const double *c, double t, ScanNode *stk,
+                               ScanChunk *pend) {
   int cnt = 0;
   ball_scan(
-      S, f, c, sqrt(t), stk,
+      S, f, c, sqrt(t), stk, pend,
       [&](int rs, int re) {
         cnt += re - rs;
         return true;
@@ -527,15 +621,17 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
 // wave minimum, each removing one instance.
 __device__ inline double knn_distance(const SortedSet &S, const Frame &f,
                                       const double *c, int kq, double r0,
-                                      ScanNode *stk, double *buf) {
+                                      ScanNode *stk, ScanChunk *pend,
+                                      double *buf) {
   double R = r0 > 0.0 ? r0 : 1e-300;
   int chi;
-  while ((chi = count_d2(S, f, c, R * R, stk)) < kq)
+  while ((chi = count_d2(S, f, c, R * R, stk, pend)) < kq)
     R *= 2.0;
   uint64_t lo = 0, hi = (uint64_t)__double_as_longlong(R * R);
   while (lo < hi && chi > kKnnCap) {
     const uint64_t mid = lo + (hi - lo) / 2;
-    const int cm = count_d2(S, f, c, __longlong_as_double((long long)mid), stk);
+    const int cm =
+        count_d2(S, f, c, __longlong_as_double((long long)mid), stk, pend);
     if (cm >= kq) {
       hi = mid;
       chi = cm;
@@ -550,7 +646,7 @@ __device__ inline double knn_distance(const SortedSet &S, const Frame &f,
   const double t = __longlong_as_double((long long)hi);
   int w = 0;
   ball_scan(
-      S, f, c, sqrt(t), stk, [](int, int) { return false; },
+      S, f, c, sqrt(t), stk, pend, [](int, int) { return false; },
       [&](int, bool valid, bool inside, double x, double y, double z) {
     const double q = valid ? sqdiff3(c[0], c[1], c[2], x, y, z) : 0.0;
     const bool hit = valid && (inside || q <= t);
@@ -623,19 +719,22 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
     return;
   const int lane = threadIdx.x & 63;
   __shared__ ScanNode stacks[kThreads / 64][kStack];
+  __shared__ ScanChunk pends[kThreads / 64][kChunks];
   __shared__ double knnBuf[kThreads / 64][kKnnCap];
   ScanNode *stk = stacks[threadIdx.x >> 6];
+  ScanChunk *pend = pends[threadIdx.x >> 6];
   const uint64_t p = e.cellP[i];
   const int d = e.cellD[i];
   double c[3], rc;
   cell_geometry(p, d, e.f, e.overlap, c, &rc);
-  int n = ball_count(e.S, e.f, c, rc, stk);
+  int n = ball_count(e.S, e.f, c, rc, stk, pend);
   double r = rc;
   if (n < e.kq) {
-    r = knn_distance(e.S, e.f, c, e.kq, rc, stk, knnBuf[threadIdx.x >> 6]);
+    r = knn_distance(e.S, e.f, c, e.kq, rc, stk, pend,
+                     knnBuf[threadIdx.x >> 6]);
     if (r < rc)  // max(rc, dk) exactly
       r = rc;
-    n = ball_count(e.S, e.f, c, r, stk);
+    n = ball_count(e.S, e.f, c, r, stk, pend);
   }
   if (lane != 0)
     return;
@@ -643,8 +742,8 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
     for (int h = 0; h < 2; ++h) {
       const uint64_t cp = 2 * p + (uint64_t)h;
       const int sh = e.f.DB - d - 1;
-      const int a = lower_bound_u64(e.T.code, 0, e.T.n, cp << sh);
-      const int b = lower_bound_u64(e.T.code, a, e.T.n, (cp + 1) << sh);
+      const int a = code_lower_bound(e.T, cp << sh);
+      const int b = code_lower_bound(e.T, (cp + 1) << sh);
       if (b > a) {
         const int o = atomicAdd(&e.counters[0], 1);
         if (o < e.maxNext) {
@@ -714,7 +813,8 @@ struct MemberArgs {
 // passes agree by construction.
 template <bool INSIDE>
 __device__ inline int64_t member_scan(const MemberArgs &m, int k, bool fill,
-                                      int64_t w, ScanNode *stk, int onlyCell) {
+                                      int64_t w, ScanNode *stk,
+                                      ScanChunk *pend, int onlyCell) {
   const int lane = threadIdx.x & 63;
   const double c[3] = {m.cx[k], m.cy[k], m.cz[k]};
   const double rad = m.r[k], radMember = rad - kEps;
@@ -723,7 +823,7 @@ __device__ inline int64_t member_scan(const MemberArgs &m, int k, bool fill,
   // another pass; a disagreement is reported, not written out of range)
   const int64_t end = fill ? m.off[k + 1] : 0;
   ball_scan<INSIDE>(
-      m.P, m.f, c, rad, stk,
+      m.P, m.f, c, rad, stk, pend,
       [&](int rs, int re) {
         if (fill)
           return false;
@@ -761,9 +861,10 @@ __global__ __launch_bounds__(kThreads) void member_kernel(MemberArgs m,
   if (k >= m.K || m.r[k] > m.rHeavy)
     return;
   __shared__ ScanNode stacks[kThreads / 64][kStack];
+  __shared__ ScanChunk pends[kThreads / 64][kChunks];
   ScanNode *stk = stacks[threadIdx.x >> 6];
-  const int64_t w =
-      member_scan<INSIDE>(m, k, fill, fill ? m.off[k] : 0, stk, -1);
+  const int64_t w = member_scan<INSIDE>(m, k, fill, fill ? m.off[k] : 0, stk,
+                                        pends[threadIdx.x >> 6], -1);
   if (!fill && (threadIdx.x & 63) == 0)
     m.cnt[k] = (int)w;
   if (fill && (threadIdx.x & 63) == 0 && w != m.off[k + 1])
@@ -788,6 +889,7 @@ constexpr int kCells = 27;
 __global__ __launch_bounds__(64 * kHeavyWaves) void member_heavy_kernel(
     MemberArgs m, const int *list, const int *count, int *cellCnt, bool fill) {
   __shared__ ScanNode stacks[kHeavyWaves][kStack];
+  __shared__ ScanChunk pends[kHeavyWaves][kChunks];
   const int wave = threadIdx.x >> 6;
   ScanNode *stk = stacks[wave];
   const int nc = m.f.D == 1 ? 3 : m.f.D == 2 ? 9 : 27;
@@ -803,7 +905,7 @@ __global__ __launch_bounds__(64 * kHeavyWaves) void member_heavy_kernel(
           w += cc[jj];
       }
       const int64_t w0 = w;
-      w = member_scan<true>(m, k, fill, w, stk, j);
+      w = member_scan<true>(m, k, fill, w, stk, pends[wave], j);
       if (!fill && (threadIdx.x & 63) == 0)
         cc[j] = (int)w;
       if (fill && (threadIdx.x & 63) == 0 && w != w0 + cc[j])
@@ -927,10 +1029,21 @@ void build_adaptive(const AdaptiveParams &ap, int dim, const double *dSrc,
   DevBuf<uint64_t> scode, tcode;
   sort_set(dSrc, N, scode, out.src);
   sort_set(dTgt, M, tcode, out.tgt);
-  const SortedSet S{scode.get(), out.src.x.get(), out.src.y.get(),
-                    out.src.z.get(), (int)N};
-  const SortedSet T{tcode.get(), out.tgt.x.get(), out.tgt.y.get(),
-                    out.tgt.z.get(), (int)M};
+  const int tabBits = std::min(f.DB, 18);
+  const int64_t nTab = ((int64_t)1 << tabBits) + 1;
+  DevBuf<int> stab(nTab), ttab(nTab);
+  code_table_kernel<<<blocks(nTab, kThreads), kThreads, 0, s>>>(
+      scode.get(), (int)N, f.DB - tabBits, tabBits, stab.get());
+  PACMAN_HIP_CHECK(hipGetLastError());
+  code_table_kernel<<<blocks(nTab, kThreads), kThreads, 0, s>>>(
+      tcode.get(), (int)M, f.DB - tabBits, tabBits, ttab.get());
+  PACMAN_HIP_CHECK(hipGetLastError());
+  const SortedSet S{scode.get(),     out.src.x.get(), out.src.y.get(),
+                    out.src.z.get(), (int)N,          stab.get(),
+                    f.DB - tabBits,  tabBits};
+  const SortedSet T{tcode.get(),     out.tgt.x.get(), out.tgt.y.get(),
+                    out.tgt.z.get(), (int)M,          ttab.get(),
+                    f.DB - tabBits,  tabBits};
 
   // 2. the tree, one launch per depth. The nodes of a depth below the root
   // are children of split cells, which are disjoint and hold more than
