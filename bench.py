@@ -80,6 +80,9 @@ def main() -> int:
                          "from the FP64 solve is reported")
     ap.add_argument("--graph", action="store_true",
                     help="apply mode: replay the apply as a HIP graph")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="uneven cloud: time the headline transfer only "
+                         "(for profiler runs; the comparisons are null)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB "
                          "(calibrates HBM byte counters) and exit")
@@ -328,6 +331,10 @@ def bench_mixture(args) -> int:
     torch.cuda.synchronize()
     ph = plan.phase_times()
     del plan
+    if args.only_headline:
+        print(json.dumps({"ms_per_step": ms, "phase_ms": ph, "work": work}),
+              flush=True)
+        return 0
     # FP64 batched (every class in FP64)
     fp64_out = torch.empty_like(out)
     ms_b, plan = timed(lambda: transfer(args.rbf, "fp64", dV, fp64_out),
@@ -400,7 +407,7 @@ def bench_mixture(args) -> int:
     pmc = os.path.join(ROOT, "profiles", "solve_tile_mixture_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            traffic = json.load(fh).get("hbm_bytes_per_step")
 
     cpu = None
     if not args.no_cpu_baseline:
