@@ -120,6 +120,7 @@ bool device_present() {
 void run_host_transfer(int dim, int kind, const double *src, int64_t ns,
                        const double *vals, const double *tgt, int64_t nt,
                        double *out) {
+  pacman_amd::TraceRange tr("pacman::rbf_interpolate (host arrays)");
   hipStream_t s = nullptr;
   DevBuf<double> dsrc((size_t)ns * dim), dvals(ns), dtgt((size_t)nt * dim),
       dout(std::max<int64_t>(nt, 1));

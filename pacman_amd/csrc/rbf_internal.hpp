@@ -22,6 +22,16 @@
 
 namespace pacman_amd {
 
+// Named host-side range for tracers (rocprofv3 --marker-trace shows the
+// plan's setup, apply and gradient calls and the setup phases); a no-op
+// when no tracer is attached.
+struct TraceRange {
+  explicit TraceRange(const char *name);
+  ~TraceRange();
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+};
+
 // Largest patch (sources) of the 8-wave register-resident kernels: the
 // transpose chain, the stored operator and the target-coordinate gradient
 // take patches up to this size. The forward solve takes patches up to

@@ -7,6 +7,8 @@
 #include "rbf_plan.hpp"
 #include "rbf_tune_table.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -15,6 +17,9 @@
 #include <cstdlib>
 
 namespace pacman_amd {
+
+TraceRange::TraceRange(const char *name) { roctxRangePushA(name); }
+TraceRange::~TraceRange() { roctxRangePop(); }
 
 namespace {
 
@@ -29,6 +34,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
                        const double *dTgt, int64_t M, hipStream_t s)
     : prm_(p), N_(N), M_(M) {
+  TraceRange tr("pacman::plan setup");
   const int D = p.dim;
   if (D < 1 || D > 3)
     throw std::runtime_error("The dimension of the points can only be: 1, 2 or 3.");
@@ -88,6 +94,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
 // phase 2; radius and lattice phases are zero.
 void RbfPumPlan::setup_adaptive(const double *dSrc, const double *dTgt,
                                 hipStream_t s) {
+  TraceRange tr("pacman::setup density-aware patches");
   const auto t0 = std::chrono::steady_clock::now();
   AdaptivePatches ad;
   build_adaptive(prm_.ap, prm_.dim, dSrc, N_, dTgt, M_, ad, s);
@@ -131,6 +138,7 @@ void RbfPumPlan::setup_adaptive(const double *dSrc, const double *dTgt,
 void RbfPumPlan::setup_reference(const double *dSrc, const double *dTgt,
                                  hipStream_t s,
                                  std::chrono::steady_clock::time_point t0) {
+  TraceRange tr("pacman::setup reference patches");
   const PlanParams &p = prm_;
   const int D = p.dim;
   const int64_t N = N_, M = M_;
@@ -395,6 +403,7 @@ void RbfPumPlan::ensure_wslot(hipStream_t s) {
 
 void RbfPumPlan::apply(const double *dVals, int F, double *dOut,
                        hipStream_t s) {
+  TraceRange tr("pacman::apply");
   if (M_ == 0)
     return;
   prepare(F, s);
@@ -422,6 +431,7 @@ int64_t RbfPumPlan::noAllocScratch(int F) const {
 void RbfPumPlan::applyNoAlloc(const double *dVals, int F, double *dOut,
                               hipStream_t s, double *scratch,
                               int64_t capacity) {
+  TraceRange tr("pacman::apply (no allocation)");
   const int64_t need = noAllocScratch(F);
   int dev = -1;
   PACMAN_HIP_CHECK(hipGetDevice(&dev));
@@ -467,6 +477,7 @@ void RbfPumPlan::applyNoAlloc(const double *dVals, int F, double *dOut,
 // host synchronisation or memset happens inside the captured sequence.
 void RbfPumPlan::applyGraph(const double *dVals, int F, double *dOut,
                             hipStream_t s) {
+  TraceRange tr("pacman::apply (graph)");
   if (M_ == 0)
     return;
   prepare(F, s);
@@ -579,6 +590,7 @@ BlendArgs RbfPumPlan::blendArgs(int F) const {
 // source sums its patch entries in ascending patch order.
 void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
                                 hipStream_t s) {
+  TraceRange tr("pacman::apply transpose");
   require_register_patches("the transpose apply");
   if (F < 1)
     throw std::invalid_argument("transpose apply needs at least one field");
@@ -647,6 +659,7 @@ void RbfPumPlan::applyState(const double *dVals, int F, double *dOut,
 // one blend with the derivative of the Shepard weights.
 void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
                                  double *dCoordGrad, hipStream_t s) {
+  TraceRange tr("pacman::target coordinate gradient");
   require_register_patches("the target-coordinate gradient");
   if (prm_.adaptive)
     throw std::runtime_error("the target-coordinate gradient is available "
@@ -683,6 +696,7 @@ void RbfPumPlan::targetCoordGrad(const double *dState, int F, const double *dG,
 // (halves its memory and the bytes each apply reads; applied with FP64
 // accumulation).
 void RbfPumPlan::buildOperator(hipStream_t s, bool fp32) {
+  TraceRange tr("pacman::build operator");
   require_register_patches("the stored operator");
   releaseGraph();
   op_.release();
