@@ -732,6 +732,29 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
   // slices' sums are added in slice order. S depends only on n and the pair
   // count, so the result does not depend on the number of fields.
   {
+    // With more than four fields, alpha with the fields fastest (alT[j][f],
+    // zero past F) in place of rhs: one 16-byte LDS read gives two fields of
+    // a source, and all kFMax products are formed (the same sums for the
+    // first F). With up to four, rhs as it is.
+    const bool wide = F > 4;
+    double *alT = &rhs[0][0];
+    if (wide) {
+      double v[(kFMax * NMAX + NT - 1) / NT];
+#pragma unroll
+      for (int u = 0; u < (kFMax * NMAX + NT - 1) / NT; ++u) {
+        const int idx = tid + u * NT;  // (j, f) of alT
+        const int j = idx / kFMax, f = idx % kFMax;
+        v[u] = (idx < kFMax * NMAX && f < F && j < n) ? rhs[f][j] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (kFMax * NMAX + NT - 1) / NT; ++u) {
+        const int idx = tid + u * NT;
+        if (idx < kFMax * NMAX)
+          alT[idx] = v[u];
+      }
+      __syncthreads();
+    }
     double *epart = sbuf;  // [S][F][c], S * c <= NT
     const int64_t p0 = a.pairOff[patch], p1 = a.pairOff[patch + 1];
     const int mall = (int)(p1 - p0);
@@ -771,21 +794,37 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
           for (; l + 3 < l1; l += 4) {
             const double f0 = phi(l), f1 = phi(l + 1), f2 = phi(l + 2),
                          f3 = phi(l + 3);
+            if (wide) {
 #pragma unroll
-            for (int f = 0; f < kFMax; ++f)
-              if (f < F) {
-                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
-                acc1[f] = fma(f1, rhs[f][l + 1], acc1[f]);
-                acc2[f] = fma(f2, rhs[f][l + 2], acc2[f]);
-                acc3[f] = fma(f3, rhs[f][l + 3], acc3[f]);
+              for (int f = 0; f < kFMax; ++f) {
+                acc0[f] = fma(f0, alT[l * kFMax + f], acc0[f]);
+                acc1[f] = fma(f1, alT[(l + 1) * kFMax + f], acc1[f]);
+                acc2[f] = fma(f2, alT[(l + 2) * kFMax + f], acc2[f]);
+                acc3[f] = fma(f3, alT[(l + 3) * kFMax + f], acc3[f]);
               }
+            } else {
+#pragma unroll
+              for (int f = 0; f < kFMax; ++f)
+                if (f < F) {
+                  acc0[f] = fma(f0, rhs[f][l], acc0[f]);
+                  acc1[f] = fma(f1, rhs[f][l + 1], acc1[f]);
+                  acc2[f] = fma(f2, rhs[f][l + 2], acc2[f]);
+                  acc3[f] = fma(f3, rhs[f][l + 3], acc3[f]);
+                }
+            }
           }
           for (; l < l1; ++l) {
             const double f0 = phi(l);
+            if (wide) {
 #pragma unroll
-            for (int f = 0; f < kFMax; ++f)
-              if (f < F)
-                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
+              for (int f = 0; f < kFMax; ++f)
+                acc0[f] = fma(f0, alT[l * kFMax + f], acc0[f]);
+            } else {
+#pragma unroll
+              for (int f = 0; f < kFMax; ++f)
+                if (f < F)
+                  acc0[f] = fma(f0, rhs[f][l], acc0[f]);
+            }
           }
         });
       }
