@@ -692,7 +692,9 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
                                                          const double *coef,
                                                          const int2 *chunks) {
   __shared__ double sxs[256], sys[256], szs[256];
-  __shared__ double sal[kFMax][256];
+  // alpha of the staged sources with the fields fastest (zero past F): one
+  // 16-byte LDS read gives two fields
+  __shared__ double sal[256][kFMax];
   __shared__ double scoef[kFMax][4];
   const int tid = threadIdx.x;
   const int patch = chunks[blockIdx.x].x, q0 = chunks[blockIdx.x].y;
@@ -743,7 +745,7 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
       szs[tid] = a.sz[p];
 #pragma unroll
       for (int f = 0; f < kFMax; ++f)
-        sal[f][tid] = f < F ? ah[(int64_t)(j0 + tid) * F + f] : 0.0;
+        sal[tid][f] = f < F ? ah[(int64_t)(j0 + tid) * F + f] : 0.0;
     }
     __syncthreads();
     for (int j = 0; j < jn; ++j) {
@@ -751,10 +753,16 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
       for (int h = 0; h < 2; ++h) {
         const double ph = wendland<K, double, true>(
             dist_fast(yx[h], yy[h], yz[h], sxs[j], sys[j], szs[j]), dinv);
+        if (F > 4) {  // every product (zeros past F): paired reads
 #pragma unroll
-        for (int f = 0; f < kFMax; ++f)
-          if (f < F)
-            acc[h][f] = fma(ph, sal[f][j], acc[h][f]);
+          for (int f = 0; f < kFMax; ++f)
+            acc[h][f] = fma(ph, sal[j][f], acc[h][f]);
+        } else {
+#pragma unroll
+          for (int f = 0; f < kFMax; ++f)
+            if (f < F)
+              acc[h][f] = fma(ph, sal[j][f], acc[h][f]);
+        }
       }
     }
   }
