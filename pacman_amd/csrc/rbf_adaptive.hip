@@ -80,15 +80,65 @@ __host__ __device__ inline int axis_bits(int d, int D, int a) {
   return (d + D - 1 - a) / D;
 }
 
+// Bit k of a 16-bit axis coordinate moves to bit D k (and back): the code
+// holds axis a's bit k at position D k + (D - 1 - a), axis 0 first from the
+// most significant bit.
+__host__ __device__ inline uint64_t spread_bits(uint64_t x, int D) {
+  x &= 0xFFFFull;
+  if (D == 3) {
+    x = (x | (x << 16)) & 0x0000FF0000FFull;
+    x = (x | (x << 8)) & 0x00F00F00F00Full;
+    x = (x | (x << 4)) & 0x0C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x249249249249ull;
+  } else if (D == 2) {
+    x = (x | (x << 8)) & 0x00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0Full;
+    x = (x | (x << 2)) & 0x33333333ull;
+    x = (x | (x << 1)) & 0x55555555ull;
+  }
+  return x;
+}
+
+__host__ __device__ inline uint64_t compact_bits(uint64_t x, int D) {
+  if (D == 3) {
+    x &= 0x249249249249ull;
+    x = (x ^ (x >> 2)) & 0x0C30C30C30C3ull;
+    x = (x ^ (x >> 4)) & 0x00F00F00F00Full;
+    x = (x ^ (x >> 8)) & 0x0000FF0000FFull;
+    x = (x ^ (x >> 16)) & 0xFFFFull;
+  } else if (D == 2) {
+    x &= 0x55555555ull;
+    x = (x ^ (x >> 1)) & 0x33333333ull;
+    x = (x ^ (x >> 2)) & 0x0F0F0F0Full;
+    x = (x ^ (x >> 4)) & 0x00FF00FFull;
+    x = (x ^ (x >> 8)) & 0xFFFFull;
+  } else {
+    x &= 0xFFFFull;
+  }
+  return x;
+}
+
+// Prefix (the leading d code bits) of the cell at depth d whose axis-a
+// index is j[a] (axis_bits(d, D, a) bits): j[a] as the leading bits of each
+// axis coordinate, interleaved, cut to d bits.
 __host__ __device__ inline uint64_t cell_prefix(const uint64_t *j, int D,
                                                 int d) {
-  uint64_t p = 0;
-  for (int t = 0; t < d; ++t) {
-    const int a = t % D;
-    const int bit = axis_bits(d, D, a) - 1 - t / D;
-    p = (p << 1) | ((j[a] >> bit) & 1u);
+  uint64_t c = 0;
+  for (int a = 0; a < D; ++a) {
+    const int ba = axis_bits(d, D, a);
+    c |= spread_bits(j[a] << (kBits - ba), D) << (D - 1 - a);
   }
-  return p;
+  return c >> (kBits * D - d);
+}
+
+// Axis-a index of the cell with prefix p at depth d (cell_prefix inverted).
+__host__ __device__ inline uint64_t cell_axis_index(uint64_t p, int d, int D,
+                                                    int a) {
+  const int ba = axis_bits(d, D, a);
+  if (ba == 0)
+    return 0;
+  const uint64_t c = p << (kBits * D - d);
+  return compact_bits(c >> (D - 1 - a), D) >> (kBits - ba);
 }
 
 __device__ inline int lower_bound_u64(const uint64_t *a, int lo, int hi,
@@ -300,9 +350,7 @@ __device__ inline void cell_geometry(uint64_t p, int d, const Frame &f,
   cen[0] = cen[1] = cen[2] = 0.0;
   for (int a = 0; a < f.D; ++a) {
     const int ba = axis_bits(d, f.D, a);
-    uint64_t ia = 0;
-    for (int t = a; t < d; t += f.D)
-      ia = (ia << 1) | ((p >> (d - 1 - t)) & 1u);
+    const uint64_t ia = cell_axis_index(p, d, f.D, a);
     const double h = f.ext[a] * ldexp(1.0, -ba);
     cen[a] = f.lo[a] + ((double)ia + 0.5) * h;
     s = s + h * h;
@@ -357,9 +405,7 @@ __device__ inline double cell_box_dist2(uint64_t p, int d, const Frame &f,
   double s = 0.0;
   for (int a = 0; a < f.D; ++a) {
     const int ba = axis_bits(d, f.D, a);
-    uint64_t ia = 0;
-    for (int t = a; t < d; t += f.D)
-      ia = (ia << 1) | ((p >> (d - 1 - t)) & 1u);
+    const uint64_t ia = cell_axis_index(p, d, f.D, a);
     const double h = f.ext[a] * ldexp(1.0, -ba);
     const double pad = 1e-12 * (f.ext[a] + fabs(f.lo[a]));
     const double blo = f.lo[a] + (double)ia * h - pad;
@@ -376,9 +422,7 @@ __device__ inline double cell_box_maxdist2(uint64_t p, int d, const Frame &f,
   double s = 0.0;
   for (int a = 0; a < f.D; ++a) {
     const int ba = axis_bits(d, f.D, a);
-    uint64_t ia = 0;
-    for (int t = a; t < d; t += f.D)
-      ia = (ia << 1) | ((p >> (d - 1 - t)) & 1u);
+    const uint64_t ia = cell_axis_index(p, d, f.D, a);
     const double h = f.ext[a] * ldexp(1.0, -ba);
     const double pad = 1e-12 * (f.ext[a] + fabs(f.lo[a]));
     const double blo = f.lo[a] + (double)ia * h - pad;
