@@ -84,6 +84,13 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   status_.alloc(std::max<int64_t>(K_, 1));
   for (auto &e : ev_)
     PACMAN_HIP_CHECK(hipEventCreate(&e));
+  PACMAN_HIP_CHECK(hipEventCreateWithFlags(&bfork_, hipEventDisableTiming));
+  for (int k = 0; k + 1 < kBinStreams; ++k) {
+    PACMAN_HIP_CHECK(
+        hipStreamCreateWithFlags(&bstream_[k], hipStreamNonBlocking));
+    PACMAN_HIP_CHECK(
+        hipEventCreateWithFlags(&bjoin_[k], hipEventDisableTiming));
+  }
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   setupMs_[3] += ms_since(tPhase);
   setupMs_[4] = ms_since(tSetup);
@@ -269,6 +276,30 @@ const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
   return B;
 }
 
+// The size bins go round-robin over s and kBinStreams - 1 more streams
+// (their patches and slots are disjoint), so one bin's last workgroups run
+// beside the next bins' first; s waits for all of them at the end.
+template <class Launch>
+void RbfPumPlan::launch_bins(const Bins &B, SolveArgs a, hipStream_t s,
+                             Launch &&launch) {
+  const int ns = (int)std::min<size_t>(kBinStreams, B.R.size());
+  if (ns > 1) {
+    PACMAN_HIP_CHECK(hipEventRecord(bfork_, s));
+    for (int k = 1; k < ns; ++k)
+      PACMAN_HIP_CHECK(hipStreamWaitEvent(bstream_[k - 1], bfork_, 0));
+  }
+  for (size_t b = 0; b < B.R.size(); ++b) {
+    a.patches = B.dev.get() + B.first[b];
+    a.count = B.count[b];
+    const int k = (int)(b % (size_t)ns);
+    launch(B.R[b], a, k == 0 ? s : bstream_[k - 1]);
+  }
+  for (int k = 1; k < ns; ++k) {
+    PACMAN_HIP_CHECK(hipEventRecord(bjoin_[k - 1], bstream_[k - 1]));
+    PACMAN_HIP_CHECK(hipStreamWaitEvent(s, bjoin_[k - 1], 0));
+  }
+}
+
 RbfPumPlan::~RbfPumPlan() {
   releaseGraph();
   if (gstream_)
@@ -279,6 +310,14 @@ RbfPumPlan::~RbfPumPlan() {
   for (auto &e : ev_)
     if (e)
       (void)hipEventDestroy(e);
+  if (bfork_)
+    (void)hipEventDestroy(bfork_);
+  for (int k = 0; k + 1 < kBinStreams; ++k) {
+    if (bjoin_[k])
+      (void)hipEventDestroy(bjoin_[k]);
+    if (bstream_[k])
+      (void)hipStreamDestroy(bstream_[k]);
+  }
 }
 
 void RbfPumPlan::phaseTimes(double *ms) const {
@@ -626,11 +665,9 @@ void RbfPumPlan::applyTranspose(const double *dG, int F, double *dGrad,
       // the stored rows already hold the weights: a column sum per source
       launch_op_transpose(a, K_, opOff_.get(), opStore(), contribS_.get(), s);
     } else {
-      for (size_t b = 0; b < B.R.size(); ++b) {
-        a.patches = B.dev.get() + B.first[b];
-        a.count = B.count[b];
-        launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
-      }
+      launch_bins(B, a, s, [&](int R, const SolveArgs &ab, hipStream_t sb) {
+        launch_solve_f64_adjoint(R, prm_.kind, ab, sb);
+      });
       launch_adjoint_poly(a, K_, gamma_.get(), contribS_.get(), s);
     }
     launch_source_reduce(srcTOff_.get(), srcTEnt_.get(), N_, nf,
@@ -735,11 +772,9 @@ void RbfPumPlan::buildOperator(hipStream_t s, bool fp32) {
   a.gammaOut = gamma_.get();
   for (int64_t r0 = 0; r0 < maxRows; r0 += Fb) {
     a.opRow0 = (int)r0;
-    for (size_t b = 0; b < B.R.size(); ++b) {
-      a.patches = B.dev.get() + B.first[b];
-      a.count = B.count[b];
-      launch_solve_f64_adjoint(B.R[b], prm_.kind, a, s);
-    }
+    launch_bins(B, a, s, [&](int R, const SolveArgs &ab, hipStream_t sb) {
+      launch_solve_f64_adjoint(R, prm_.kind, ab, sb);
+    });
     launch_adjoint_poly(a, K_, gamma_.get(), contribS_.get(), s);
     launch_op_store(a, K_, contribS_.get(), opOff_.get(), opStore(), s);
   }
@@ -816,13 +851,10 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
             dState + (int64_t)F * total + (int64_t)f0 * K_ * 4, bu.coef,
             sizeof(double) * (size_t)K_ * nf * 4, hipMemcpyDeviceToDevice, s));
     }
-    const Bins &B = bins_[nf - 1];
-    for (size_t b = 0; b < B.R.size(); ++b) {
-      a.patches = B.dev.get() + B.first[b];
-      a.count = B.count[b];
-      launch_solve_f64(B.R[b], kTune.waves[nf - 1][B.R[b]], prm_.kind, a,
-                       bu.coef, s);
-    }
+    launch_bins(bins_[nf - 1], a, s, [&](int R, const SolveArgs &ab,
+                                         hipStream_t sb) {
+      launch_solve_f64(R, kTune.waves[nf - 1][R], prm_.kind, ab, bu.coef, sb);
+    });
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[7], s));
     if (!wsHost_.empty()) {

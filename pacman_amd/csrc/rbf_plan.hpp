@@ -192,6 +192,9 @@ private:
   bool workspace_fp32(int F) const;
   std::vector<int> patchN_;  // sources per patch (host)
   const Bins &bins_for(int F, hipStream_t s);
+  // launch(R, args, stream) once per size bin, args.patches/count set
+  template <class Launch>
+  void launch_bins(const Bins &B, SolveArgs a, hipStream_t s, Launch &&launch);
   void prepare(int F, hipStream_t s);
   // per-call buffers of an apply: F x N sorted source values, pairs x Fc
   // contributions, K x Fs x 4 polynomial coefficients
@@ -244,6 +247,11 @@ private:
   hipEvent_t ev_[8] = {nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, nullptr, nullptr};
   bool wsTimed_ = false;  // ev_[5], ev_[6] recorded by the last apply
+  // more streams of the size-bin launches, forked from and joined to the
+  // apply's stream
+  static constexpr int kBinStreams = 3;
+  hipStream_t bstream_[kBinStreams - 1] = {};
+  hipEvent_t bfork_ = nullptr, bjoin_[kBinStreams - 1] = {};
   bool applied_ = false;
 };
 
