@@ -775,6 +775,9 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
         qz = a.tz[tp];
       }
       const bool mem = act && ((unsigned)e >> 31) != 0;
+      // four partial sums per field: chain k takes the sources l0 + 4i + k
+      // of the full groups of four, chain 0 then the remaining ones, and
+      // the slice's sum is (chain 0 + chain 1) + (chain 2 + chain 3)
       double acc0[kFMax], acc1[kFMax], acc2[kFMax], acc3[kFMax];
 #pragma unroll
       for (int f = 0; f < kFMax; ++f) {
@@ -790,50 +793,64 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
             return wendland<K, double, true>(
                 dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
           };
+          if (wide) {
+            // one chain after the other: 8 fields x at most three chains
+            // live, where four interleaved chains exceed the registers
+            const int nq = (l1 - l0) >> 2;
+            auto chain = [&](int l, int end, int step, double *acc)
+                __attribute__((always_inline)) {
+              for (; l < end; l += step) {
+                const double w = phi(l);
+#pragma unroll
+                for (int f = 0; f < kFMax; ++f)
+                  acc[f] = fma(w, alT[l * kFMax + f], acc[f]);
+              }
+            };
+            chain(l0, l0 + 4 * nq, 4, acc0);
+            chain(l0 + 4 * nq, l1, 1, acc0);
+            chain(l0 + 1, l0 + 4 * nq, 4, acc1);
+#pragma unroll
+            for (int f = 0; f < kFMax; ++f) {
+              acc0[f] = acc0[f] + acc1[f];
+              acc1[f] = 0.0;
+            }
+            chain(l0 + 2, l0 + 4 * nq, 4, acc1);
+            chain(l0 + 3, l0 + 4 * nq, 4, acc2);
+#pragma unroll
+            for (int f = 0; f < kFMax; ++f)
+              acc0[f] = acc0[f] + (acc1[f] + acc2[f]);
+            return;
+          }
           int l = l0;
           for (; l + 3 < l1; l += 4) {
             const double f0 = phi(l), f1 = phi(l + 1), f2 = phi(l + 2),
                          f3 = phi(l + 3);
-            if (wide) {
 #pragma unroll
-              for (int f = 0; f < kFMax; ++f) {
-                acc0[f] = fma(f0, alT[l * kFMax + f], acc0[f]);
-                acc1[f] = fma(f1, alT[(l + 1) * kFMax + f], acc1[f]);
-                acc2[f] = fma(f2, alT[(l + 2) * kFMax + f], acc2[f]);
-                acc3[f] = fma(f3, alT[(l + 3) * kFMax + f], acc3[f]);
+            for (int f = 0; f < kFMax; ++f)
+              if (f < F) {
+                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
+                acc1[f] = fma(f1, rhs[f][l + 1], acc1[f]);
+                acc2[f] = fma(f2, rhs[f][l + 2], acc2[f]);
+                acc3[f] = fma(f3, rhs[f][l + 3], acc3[f]);
               }
-            } else {
-#pragma unroll
-              for (int f = 0; f < kFMax; ++f)
-                if (f < F) {
-                  acc0[f] = fma(f0, rhs[f][l], acc0[f]);
-                  acc1[f] = fma(f1, rhs[f][l + 1], acc1[f]);
-                  acc2[f] = fma(f2, rhs[f][l + 2], acc2[f]);
-                  acc3[f] = fma(f3, rhs[f][l + 3], acc3[f]);
-                }
-            }
           }
           for (; l < l1; ++l) {
             const double f0 = phi(l);
-            if (wide) {
 #pragma unroll
-              for (int f = 0; f < kFMax; ++f)
-                acc0[f] = fma(f0, alT[l * kFMax + f], acc0[f]);
-            } else {
-#pragma unroll
-              for (int f = 0; f < kFMax; ++f)
-                if (f < F)
-                  acc0[f] = fma(f0, rhs[f][l], acc0[f]);
-            }
+            for (int f = 0; f < kFMax; ++f)
+              if (f < F)
+                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
           }
+#pragma unroll
+          for (int f = 0; f < kFMax; ++f)
+            acc0[f] = (acc0[f] + acc1[f]) + (acc2[f] + acc3[f]);
         });
       }
       if (act) {
 #pragma unroll
         for (int f = 0; f < kFMax; ++f)
           if (f < F)
-            epart[(sl * F + f) * c + pi] =
-                (acc0[f] + acc1[f]) + (acc2[f] + acc3[f]);
+            epart[(sl * F + f) * c + pi] = acc0[f];
       }
       __syncthreads();
       if (tid < c) {
