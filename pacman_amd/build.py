@@ -45,6 +45,7 @@ UNITS = {
     "fe_nearest.hip": ["-ffp-contract=off"],
     "mls.hip": ["-ffp-contract=off"],
     "dev_alloc.hip": [],
+    "host_copy.hip": [],
     "rbf_solve.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "rbf_grad.hip": [],
     "rbf_solve_ws.hip": [],

@@ -20,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 
 #include "../../include/fortran_interface.h"
 #include "../../include/interface.hpp"
@@ -124,26 +125,41 @@ void run_host_transfer(int dim, int kind, const double *src, int64_t ns,
   hipStream_t s = nullptr;
   DevBuf<double> dsrc((size_t)ns * dim), dvals(ns), dtgt((size_t)nt * dim),
       dout(std::max<int64_t>(nt, 1));
-  PACMAN_HIP_CHECK(hipMemcpyAsync(dsrc.get(), src, 8 * (size_t)ns * dim,
-                                  hipMemcpyHostToDevice, s));
-  PACMAN_HIP_CHECK(
-      hipMemcpyAsync(dvals.get(), vals, 8 * (size_t)ns, hipMemcpyHostToDevice, s));
-  if (nt > 0)
-    PACMAN_HIP_CHECK(hipMemcpyAsync(dtgt.get(), tgt, 8 * (size_t)nt * dim,
-                                    hipMemcpyHostToDevice, s));
+  pacman_amd::copy_to_device(dsrc.get(), src, 8 * (size_t)ns * dim, s);
+  pacman_amd::copy_to_device(dtgt.get(), tgt, 8 * (size_t)nt * dim, s);
+  // the values are first read by the apply: their copy runs beside the setup
+  int dev = 0;
+  PACMAN_HIP_CHECK(hipGetDevice(&dev));
+  std::exception_ptr valsErr;
+  std::thread valsCopy([&] {
+    try {
+      PACMAN_HIP_CHECK(hipSetDevice(dev));
+      pacman_amd::copy_to_device(dvals.get(), vals, 8 * (size_t)ns, s,
+                                 /*afterStream=*/false);
+    } catch (...) {
+      valsErr = std::current_exception();
+    }
+  });
   PlanParams p;
   p.dim = dim;
   p.kind = kind;
-  RbfPumPlan plan(p, dsrc.get(), ns, dtgt.get(), nt, s);
-  plan.apply(dvals.get(), 1, dout.get(), s);
-  if (nt > 0)
-    PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
-                                    hipMemcpyDeviceToHost, s));
+  std::unique_ptr<RbfPumPlan> plan;
+  try {
+    plan = std::make_unique<RbfPumPlan>(p, dsrc.get(), ns, dtgt.get(), nt, s);
+  } catch (...) {
+    valsCopy.join();
+    throw;
+  }
+  valsCopy.join();
+  if (valsErr)
+    std::rethrow_exception(valsErr);
+  plan->apply(dvals.get(), 1, dout.get(), s);
+  pacman_amd::copy_to_host(out, dout.get(), 8 * (size_t)nt, s);
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   // The reference's unpivoted LU returns inf / NaN silently on duplicate
   // sources (serial_solve.hpp:76-84); here such a patch is an error.
   int c[3];
-  plan.applyCounters(c, s);
+  plan->applyCounters(c, s);
   if (c[0] > 0)
     throw std::runtime_error(
         std::to_string(c[0]) +
@@ -161,19 +177,14 @@ void run_host_nearest(int dim, const double *src, int64_t ns,
       dout(std::max<int64_t>(nt, 1));
   DevBuf<int> dst(std::max<int64_t>(nt, 1));
   if (ns > 0) {
-    PACMAN_HIP_CHECK(hipMemcpyAsync(dsrc.get(), src, 8 * (size_t)ns * dim,
-                                    hipMemcpyHostToDevice, s));
-    PACMAN_HIP_CHECK(hipMemcpyAsync(dvals.get(), vals, 8 * (size_t)ns,
-                                    hipMemcpyHostToDevice, s));
+    pacman_amd::copy_to_device(dsrc.get(), src, 8 * (size_t)ns * dim, s);
+    pacman_amd::copy_to_device(dvals.get(), vals, 8 * (size_t)ns, s);
   }
-  PACMAN_HIP_CHECK(hipMemcpyAsync(dtgt.get(), tgt, 8 * (size_t)nt * dim,
-                                  hipMemcpyHostToDevice, s));
+  pacman_amd::copy_to_device(dtgt.get(), tgt, 8 * (size_t)nt * dim, s);
   pacman_amd::fe_nearest_transfer(dim, dsrc.get(), ns, dvals.get(), dtgt.get(),
                                   nt, dout.get(), dst.get(), s);
-  PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
-                                  hipMemcpyDeviceToHost, s));
-  PACMAN_HIP_CHECK(hipMemcpyAsync(status, dst.get(), 4 * (size_t)nt,
-                                  hipMemcpyDeviceToHost, s));
+  pacman_amd::copy_to_host(out, dout.get(), 8 * (size_t)nt, s);
+  pacman_amd::copy_to_host(status, dst.get(), 4 * (size_t)nt, s);
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -186,17 +197,13 @@ void run_host_mls(int dim, const double *src, int64_t ns, const double *vals,
       dtgt((size_t)std::max<int64_t>(nt * dim, 1)),
       dout((size_t)std::max<int64_t>(nt, 1));
   if (ns > 0) {
-    PACMAN_HIP_CHECK(hipMemcpyAsync(dsrc.get(), src, 8 * (size_t)ns * dim,
-                                    hipMemcpyHostToDevice, s));
-    PACMAN_HIP_CHECK(hipMemcpyAsync(dvals.get(), vals, 8 * (size_t)ns,
-                                    hipMemcpyHostToDevice, s));
+    pacman_amd::copy_to_device(dsrc.get(), src, 8 * (size_t)ns * dim, s);
+    pacman_amd::copy_to_device(dvals.get(), vals, 8 * (size_t)ns, s);
   }
-  PACMAN_HIP_CHECK(hipMemcpyAsync(dtgt.get(), tgt, 8 * (size_t)nt * dim,
-                                  hipMemcpyHostToDevice, s));
+  pacman_amd::copy_to_device(dtgt.get(), tgt, 8 * (size_t)nt * dim, s);
   pacman_amd::MlsPlan plan(dim, dsrc.get(), ns, dtgt.get(), nt, s);
   plan.apply(dvals.get(), 1, dout.get(), s);
-  PACMAN_HIP_CHECK(hipMemcpyAsync(out, dout.get(), 8 * (size_t)nt,
-                                  hipMemcpyDeviceToHost, s));
+  pacman_amd::copy_to_host(out, dout.get(), 8 * (size_t)nt, s);
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -207,14 +214,14 @@ void run_host_mls(int dim, const double *src, int64_t ns, const double *vals,
 // ---------------------------------------------------------------------------
 namespace PACMAN {
 
-RbfInterpolateResult rbf_interpolate(int_t spaceDimension,
-                                     unsigned char execSpace,
-                                     unsigned char rbfFunction,
-                                     coordinates_t *sourcePoints,
-                                     int_t nSourcePoints, fp_t *sourceValues,
-                                     coordinates_t *targetPoints,
-                                     int_t nTargetPoints) {
-  // src/rbf_interface.cpp:109-114
+// Argument checks of the RBF drop-in (src/rbf_interface.cpp:109-114);
+// returns the Wendland kind.
+static int check_rbf_args(int_t spaceDimension, unsigned char execSpace,
+                          unsigned char rbfFunction,
+                          const coordinates_t *sourcePoints,
+                          int_t nSourcePoints, const fp_t *sourceValues,
+                          const coordinates_t *targetPoints,
+                          int_t nTargetPoints) {
   if (sourcePoints == nullptr && nSourcePoints > 0)
     throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
   if (targetPoints == nullptr && nTargetPoints > 0)
@@ -237,6 +244,19 @@ RbfInterpolateResult rbf_interpolate(int_t spaceDimension,
                                 std::to_string(rbfFunction));
   if (nSourcePoints < 0 || nTargetPoints < 0)
     throw std::invalid_argument("negative number of points");
+  return kind;
+}
+
+RbfInterpolateResult rbf_interpolate(int_t spaceDimension,
+                                     unsigned char execSpace,
+                                     unsigned char rbfFunction,
+                                     coordinates_t *sourcePoints,
+                                     int_t nSourcePoints, fp_t *sourceValues,
+                                     coordinates_t *targetPoints,
+                                     int_t nTargetPoints) {
+  const int kind =
+      check_rbf_args(spaceDimension, execSpace, rbfFunction, sourcePoints,
+                     nSourcePoints, sourceValues, targetPoints, nTargetPoints);
   RbfInterpolateResult result;
   result.targetValues.assign((size_t)nTargetPoints, 0.0);
   if (nTargetPoints == 0)
@@ -247,22 +267,13 @@ RbfInterpolateResult rbf_interpolate(int_t spaceDimension,
   return result;
 }
 
-FeInterpolateResult fe_interpolate(int_t spaceDimension,
-                                   unsigned char execSpace, method_t method,
-                                   coordinates_t *sourcePoints,
-                                   int_t nSourcePoints, fp_t *sourceValues,
-                                   int_t *connVal, int_t connValSize,
-                                   offset_t *connOff, int_t connOffSize,
-                                   cell_t *cellTypes,
-                                   coordinates_t *targetPoints,
-                                   int_t nTargetPoints, bool fortranIndexing) {
-  (void)connVal;
-  (void)connValSize;
-  (void)connOff;
-  (void)connOffSize;
-  (void)cellTypes;
-  (void)fortranIndexing;
-  // src/interface.cpp:101-107
+// Argument checks of the FE drop-in (src/interface.cpp:101-107).
+static void check_fe_args(int_t spaceDimension, unsigned char execSpace,
+                          method_t method,
+                          const coordinates_t *sourcePoints,
+                          int_t nSourcePoints, const fp_t *sourceValues,
+                          const coordinates_t *targetPoints,
+                          int_t nTargetPoints) {
   if (sourcePoints == nullptr && nSourcePoints > 0)
     throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
   if (targetPoints == nullptr && nTargetPoints > 0)
@@ -285,6 +296,25 @@ FeInterpolateResult fe_interpolate(int_t spaceDimension,
                                 "NEAREST_NEAREST)");
   if (nSourcePoints < 0 || nTargetPoints < 0)
     throw std::invalid_argument("negative number of points");
+}
+
+FeInterpolateResult fe_interpolate(int_t spaceDimension,
+                                   unsigned char execSpace, method_t method,
+                                   coordinates_t *sourcePoints,
+                                   int_t nSourcePoints, fp_t *sourceValues,
+                                   int_t *connVal, int_t connValSize,
+                                   offset_t *connOff, int_t connOffSize,
+                                   cell_t *cellTypes,
+                                   coordinates_t *targetPoints,
+                                   int_t nTargetPoints, bool fortranIndexing) {
+  (void)connVal;
+  (void)connValSize;
+  (void)connOff;
+  (void)connOffSize;
+  (void)cellTypes;
+  (void)fortranIndexing;
+  check_fe_args(spaceDimension, execSpace, method, sourcePoints, nSourcePoints,
+                sourceValues, targetPoints, nTargetPoints);
   FeInterpolateResult result;
   result.targetValues.assign((size_t)nTargetPoints, 0.0);
   result.targetStatus.assign((size_t)nTargetPoints, 0);
@@ -296,13 +326,12 @@ FeInterpolateResult fe_interpolate(int_t spaceDimension,
   return result;
 }
 
-MLSInterpolateResult MLS_interpolate(int_t spaceDimension,
-                                     unsigned char execSpace,
-                                     coordinates_t *sourcePoints,
-                                     int_t nSourcePoints, fp_t *sourceValues,
-                                     coordinates_t *targetPoints,
-                                     int_t nTargetPoints) {
-  // src/mls_interface.cpp:83-105: null pointers, dimension, exec space
+// Argument checks of the MLS drop-in (src/mls_interface.cpp:83-105).
+static void check_mls_args(int_t spaceDimension, unsigned char execSpace,
+                           const coordinates_t *sourcePoints,
+                           int_t nSourcePoints, const fp_t *sourceValues,
+                           const coordinates_t *targetPoints,
+                           int_t nTargetPoints) {
   if (sourcePoints == nullptr && nSourcePoints > 0)
     throw std::invalid_argument("sourcePoints is null but nSourcePoints > 0");
   if (targetPoints == nullptr && nTargetPoints > 0)
@@ -321,6 +350,16 @@ MLSInterpolateResult MLS_interpolate(int_t spaceDimension,
                                 " is not enabled in this build (available: HIP)");
   if (nSourcePoints < 0 || nTargetPoints < 0)
     throw std::invalid_argument("negative number of points");
+}
+
+MLSInterpolateResult MLS_interpolate(int_t spaceDimension,
+                                     unsigned char execSpace,
+                                     coordinates_t *sourcePoints,
+                                     int_t nSourcePoints, fp_t *sourceValues,
+                                     coordinates_t *targetPoints,
+                                     int_t nTargetPoints) {
+  check_mls_args(spaceDimension, execSpace, sourcePoints, nSourcePoints,
+                 sourceValues, targetPoints, nTargetPoints);
   MLSInterpolateResult result;
   result.targetValues.assign((size_t)nTargetPoints, 0.0);
   if (nTargetPoints == 0)
@@ -362,13 +401,15 @@ int pacman_rbf_interpolate_c(int spaceDimension, int execSpace, int rbfFunction,
   try {
     if (targetValues == nullptr && nTargetPoints > 0)
       throw std::invalid_argument("targetValues is null but nTargetPoints > 0");
-    PACMAN::RbfInterpolateResult r = PACMAN::rbf_interpolate(
+    // straight into the caller's array (no intermediate result vector)
+    const int kind = PACMAN::check_rbf_args(
         spaceDimension, (unsigned char)execSpace, (unsigned char)rbfFunction,
-        const_cast<double *>(sourcePoints), nSourcePoints,
-        const_cast<double *>(sourceValues), const_cast<double *>(targetPoints),
+        sourcePoints, nSourcePoints, sourceValues, targetPoints,
         nTargetPoints);
-    for (int i = 0; i < nTargetPoints; ++i)
-      targetValues[i] = r.targetValues[(size_t)i];
+    if (nTargetPoints > 0)
+      run_host_transfer(spaceDimension, kind, sourcePoints, nSourcePoints,
+                        sourceValues, targetPoints, nTargetPoints,
+                        targetValues);
     return 0;
   } catch (const std::exception &e) {
     set_error("pacman_rbf_interpolate_c", e.what());
@@ -393,17 +434,20 @@ int pacman_fe_interpolate_c(int spaceDimension, int execSpace, int method,
         nTargetPoints > 0)
       throw std::invalid_argument(
           "targetValues / targetStatus is null but nTargetPoints > 0");
-    PACMAN::FeInterpolateResult r = PACMAN::fe_interpolate(
-        spaceDimension, (unsigned char)execSpace, method,
-        const_cast<double *>(sourcePoints), nSourcePoints,
-        const_cast<double *>(sourceValues), const_cast<int *>(connVal),
-        connValSize, const_cast<int *>(connOff), connOffSize,
-        const_cast<int *>(cellTypes), const_cast<double *>(targetPoints),
-        nTargetPoints);
-    for (int i = 0; i < nTargetPoints; ++i) {
-      targetValues[i] = r.targetValues[(size_t)i];
-      targetStatus[i] = r.targetStatus[(size_t)i];
-    }
+    // the mesh connectivity is not read by NEAREST_NEAREST
+    (void)connVal;
+    (void)connValSize;
+    (void)connOff;
+    (void)connOffSize;
+    (void)cellTypes;
+    PACMAN::check_fe_args(spaceDimension, (unsigned char)execSpace,
+                          (PACMAN::method_t)method, sourcePoints,
+                          nSourcePoints, sourceValues, targetPoints,
+                          nTargetPoints);
+    if (nTargetPoints > 0)
+      run_host_nearest(spaceDimension, sourcePoints, nSourcePoints,
+                       sourceValues, targetPoints, nTargetPoints,
+                       targetValues, targetStatus);
     return 0;
   } catch (const std::exception &e) {
     set_error("pacman_fe_interpolate_c", e.what());
@@ -490,11 +534,9 @@ int pacman_rbf_plan_create(pacman_rbf_plan *plan, int spaceDimension,
                         spaceDimension),
           dt((size_t)std::max<long long>(nTargetPoints, 0) * spaceDimension);
       if (nSourcePoints > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(ds.get(), sourcePoints,
-                                        8 * ds.n, hipMemcpyHostToDevice, s));
+        pacman_amd::copy_to_device(ds.get(), sourcePoints, 8 * ds.n, s);
       if (nTargetPoints > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(dt.get(), targetPoints,
-                                        8 * dt.n, hipMemcpyHostToDevice, s));
+        pacman_amd::copy_to_device(dt.get(), targetPoints, 8 * dt.n, s);
       h->plan = std::make_unique<RbfPumPlan>(p, ds.get(), nSourcePoints,
                                              dt.get(), nTargetPoints, s);
     }
@@ -526,13 +568,11 @@ int pacman_rbf_plan_apply(pacman_rbf_plan plan, const double *sourceValues,
     } else {
       DevBuf<double> dv((size_t)nFields * P.numSources()),
           dout((size_t)nFields * std::max<int64_t>(P.numTargets(), 1));
-      PACMAN_HIP_CHECK(hipMemcpyAsync(dv.get(), sourceValues, 8 * dv.n,
-                                      hipMemcpyHostToDevice, s));
+      pacman_amd::copy_to_device(dv.get(), sourceValues, 8 * dv.n, s);
       P.apply(dv.get(), nFields, dout.get(), s);
       if (P.numTargets() > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(
-            targetValues, dout.get(),
-            8 * (size_t)nFields * P.numTargets(), hipMemcpyDeviceToHost, s));
+        pacman_amd::copy_to_host(targetValues, dout.get(),
+                                 8 * (size_t)nFields * P.numTargets(), s);
       PACMAN_HIP_CHECK(hipStreamSynchronize(s));
     }
     return 0;
@@ -565,14 +605,12 @@ int pacman_rbf_plan_apply_transpose(pacman_rbf_plan plan,
       DevBuf<double> dg((size_t)nFields * std::max<int64_t>(P.numTargets(), 1)),
           dout((size_t)nFields * std::max<int64_t>(P.numSources(), 1));
       if (P.numTargets() > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(dg.get(), targetGrad,
-                                        8 * (size_t)nFields * P.numTargets(),
-                                        hipMemcpyHostToDevice, s));
+        pacman_amd::copy_to_device(dg.get(), targetGrad,
+                                   8 * (size_t)nFields * P.numTargets(), s);
       P.applyTranspose(dg.get(), nFields, dout.get(), s);
       if (P.numSources() > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(sourceGrad, dout.get(),
-                                        8 * (size_t)nFields * P.numSources(),
-                                        hipMemcpyDeviceToHost, s));
+        pacman_amd::copy_to_host(sourceGrad, dout.get(),
+                                 8 * (size_t)nFields * P.numSources(), s);
       PACMAN_HIP_CHECK(hipStreamSynchronize(s));
     }
     return 0;
@@ -697,13 +735,12 @@ int pacman_mls_interpolate_c(int spaceDimension, int execSpace,
   try {
     if (targetValues == nullptr && nTargetPoints > 0)
       throw std::invalid_argument("targetValues is null but nTargetPoints > 0");
-    PACMAN::MLSInterpolateResult r = PACMAN::MLS_interpolate(
-        spaceDimension, (unsigned char)execSpace,
-        const_cast<double *>(sourcePoints), nSourcePoints,
-        const_cast<double *>(sourceValues), const_cast<double *>(targetPoints),
-        nTargetPoints);
-    for (int i = 0; i < nTargetPoints; ++i)
-      targetValues[i] = r.targetValues[(size_t)i];
+    PACMAN::check_mls_args(spaceDimension, (unsigned char)execSpace,
+                           sourcePoints, nSourcePoints, sourceValues,
+                           targetPoints, nTargetPoints);
+    if (nTargetPoints > 0)
+      run_host_mls(spaceDimension, sourcePoints, nSourcePoints, sourceValues,
+                   targetPoints, nTargetPoints, targetValues);
     return 0;
   } catch (const std::exception &e) {
     set_error("pacman_mls_interpolate_c", e.what());
@@ -743,13 +780,13 @@ int pacman_mls_plan_create(pacman_mls_plan *plan, int spaceDimension,
                         spaceDimension),
           dt((size_t)std::max<long long>(nTargetPoints, 1) * spaceDimension);
       if (nSourcePoints > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(
-            ds.get(), sourcePoints, 8 * (size_t)nSourcePoints * spaceDimension,
-            hipMemcpyHostToDevice, s));
+        pacman_amd::copy_to_device(
+            ds.get(), sourcePoints,
+            8 * (size_t)nSourcePoints * spaceDimension, s);
       if (nTargetPoints > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(
-            dt.get(), targetPoints, 8 * (size_t)nTargetPoints * spaceDimension,
-            hipMemcpyHostToDevice, s));
+        pacman_amd::copy_to_device(
+            dt.get(), targetPoints,
+            8 * (size_t)nTargetPoints * spaceDimension, s);
       h->plan = std::make_unique<pacman_amd::MlsPlan>(
           spaceDimension, ds.get(), nSourcePoints, dt.get(), nTargetPoints, s);
     }
@@ -782,13 +819,11 @@ int pacman_mls_plan_apply(pacman_mls_plan plan, const double *sourceValues,
     } else {
       DevBuf<double> dv((size_t)nFields * P.numSources()),
           dout((size_t)nFields * std::max<int64_t>(P.numTargets(), 1));
-      PACMAN_HIP_CHECK(hipMemcpyAsync(dv.get(), sourceValues, 8 * dv.n,
-                                      hipMemcpyHostToDevice, s));
+      pacman_amd::copy_to_device(dv.get(), sourceValues, 8 * dv.n, s);
       P.apply(dv.get(), nFields, dout.get(), s);
       if (P.numTargets() > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(
-            targetValues, dout.get(), 8 * (size_t)nFields * P.numTargets(),
-            hipMemcpyDeviceToHost, s));
+        pacman_amd::copy_to_host(targetValues, dout.get(),
+                                 8 * (size_t)nFields * P.numTargets(), s);
       PACMAN_HIP_CHECK(hipStreamSynchronize(s));
     }
     return 0;
@@ -945,11 +980,9 @@ int pacman_rbf_plan_create_adaptive(
                         spaceDimension),
           dt((size_t)std::max<long long>(nTargetPoints, 0) * spaceDimension);
       if (nSourcePoints > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(ds.get(), sourcePoints, 8 * ds.n,
-                                        hipMemcpyHostToDevice, s));
+        pacman_amd::copy_to_device(ds.get(), sourcePoints, 8 * ds.n, s);
       if (nTargetPoints > 0)
-        PACMAN_HIP_CHECK(hipMemcpyAsync(dt.get(), targetPoints, 8 * dt.n,
-                                        hipMemcpyHostToDevice, s));
+        pacman_amd::copy_to_device(dt.get(), targetPoints, 8 * dt.n, s);
       h->plan = std::make_unique<RbfPumPlan>(p, ds.get(), nSourcePoints,
                                              dt.get(), nTargetPoints, s);
     }

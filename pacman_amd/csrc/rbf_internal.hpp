@@ -32,6 +32,16 @@ struct TraceRange {
   TraceRange &operator=(const TraceRange &) = delete;
 };
 
+// Pageable host array -> device memory through pinned staging buffers
+// (host_copy.hip). With afterStream the copy follows the work queued on s;
+// without, it is ordered after nothing. Returns once the data is on the
+// device.
+void copy_to_device(void *dst, const void *src, size_t bytes, hipStream_t s,
+                    bool afterStream = true);
+// Device memory -> pageable host array after the work queued on s; returns
+// once the data is in the host array.
+void copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 // Largest patch (sources) of the 8-wave register-resident kernels: the
 // transpose chain, the stored operator and the target-coordinate gradient
 // take patches up to this size. The forward solve takes patches up to
