@@ -214,7 +214,9 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lr = lane >> 4, lc = lane & 15;
+  // lane row / column in a 16 x 16 tile (not const: the tile waves launder
+  // them once per panel phase, below)
+  int lr = lane >> 4, lc = lane & 15;
   const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
   const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
 
@@ -575,6 +577,12 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
     for (int p = 0; p < P; ++p) {
       const int buf = p & 1;
       const int kk = min(16, n - 16 * p);
+      // Opaque copies of the lane indices for this phase: the LDS addresses
+      // of every tile (16 I + lc, lc * LDW + 16 I, ...) are then formed
+      // where they are used instead of being hoisted out of the panel loop
+      // and kept live across it, one or two registers per tile, which
+      // spilled the larger bins (R = 11 at 8 waves: 104 B per lane).
+      asm volatile("" : "+v"(lc), "+v"(lr));
       __syncthreads();  // A
       // panel tiles: W = (A_Ip Linv^T) D^-1, kept and published
       static_for<0, TPW>([&](auto sc) {
@@ -624,6 +632,7 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
         }
       });
       __syncthreads();  // B: panel published, next diagonal tile handed over
+      asm volatile("" : "+v"(lc), "+v"(lr));
       // trailing update; publish the next panel's tiles
       static_for<0, TPW>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
