@@ -185,7 +185,9 @@ def main() -> int:
     kern_ms = ph["tiles"]
     achieved = f_alg / (kern_ms / 1e3) / 1e12
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"solve_tile_{args.config}_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", f"solve_tile_{args.config}_pmc.json"
+                       if args.fields == 1 else
+                       f"solve_tile_{args.config}_f{args.fields}_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_apply")
@@ -248,6 +250,7 @@ def main() -> int:
             "mpts_fields_per_s": M * args.fields / (ms / 1e3) / 1e6,
             "patches": int(work["patches"]),
             "max_patch_size": int(work["max_n"]),
+            "pairs": int(work["pairs"]),
             "phase_ms": {k: round(x, 3) for k, x in ph.items()},
             "rel_l2_vs_franke_3d": rel_l2,
             "host_buffers_ms": round(host_ms, 3),
