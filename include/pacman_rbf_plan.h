@@ -208,16 +208,20 @@ int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
  * 4 setup total (host clock); apply phases of the last apply (HIP events on
  * its stream) 5 value gather, 6 per-patch solve (polynomial fit included),
  * 7 blend, 8 apply total, 9 polynomial fit of the first field chunk,
- * 10 first-pass workspace launches (patches above 192 sources) of the first
- * field chunk (0 when the plan has none). */
+ * 10 first-pass workspace launches (patches above 264 sources) of the first
+ * field chunk (0 when the plan has none), 11 the register-resident solve
+ * launches of the first field chunk, 12 the chunked evaluation launches of
+ * the first field chunk (the patches whose solve stored alpha). */
 int pacman_rbf_plan_phase_times(pacman_rbf_plan plan, double *ms, int n);
 
 /* Work counters (up to n entries; returns the count written): 0 patches K,
  * 1 sum n_k, 2 sum n_k^2, 3 sum n_k^3, 4 sum n_k m_k (member targets),
- * 5 target/patch pairs within r, 6 largest n_k; over the patches above 192
+ * 5 target/patch pairs within r, 6 largest n_k; over the patches above 264
  * sources: 7 count, 8 sum n_k^3, 9 sum n_k^2, 10 sum n_k m_k of those the
- * workspace kernel evaluates, 11 sum n_k m_k of those evaluated in chunks
- * (more than 4096 target/patch pairs). */
+ * workspace kernel evaluates; 11 sum n_k m_k of those evaluated in chunks
+ * (every patch of up to 264 sources and every one with more than 4096
+ * target/patch pairs); over the patches of up to 264 sources: 12 sum
+ * n_k^3, 13 sum n_k^2, 14 sum n_k m_k of those their solve evaluates. */
 int pacman_rbf_plan_work(pacman_rbf_plan plan, double *work, int n);
 
 const char *pacman_rbf_last_error(void);

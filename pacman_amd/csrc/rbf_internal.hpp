@@ -229,15 +229,20 @@ struct SolveArgs {
   // patch; each one found adds 1 to *singular (when set).
   double phi0;
   int *singular;
-  // Patches with more member-target pairs than kHeavyPairs: heavyOff[k] >= 0
-  // is where their alpha goes (alphaHeavy + heavyOff[k] + e * F + f); the
-  // solve kernels store it and skip the evaluation, which
-  // launch_heavy_eval spreads over many workgroups.
+  // Patches evaluated apart from their solve (every register-resident one,
+  // and those with more member-target pairs than kHeavyPairs): heavyOff[k]
+  // >= 0 is where their alpha goes (alphaHeavy + heavyOff[k] + e * F + f);
+  // the solve kernels store it and skip the evaluation, which
+  // launch_chunk_eval spreads over many workgroups.
   const int64_t *heavyOff;
   double *alphaHeavy;
+  // x y z 0 of the source in each member slot (4 per slot, slot order);
+  // read by the one-wave-per-chunk evaluation
+  const double *slotXyz;
 };
 constexpr int kHeavyPairs = 4096;
-constexpr int kHeavyChunk = 512;  // pairs per workgroup of the heavy eval
+constexpr int kHeavyChunk = 512;  // pairs per workgroup, heavy patches
+constexpr int kLightChunk = 64;   // pairs per wave, the others
 
 // ---- workspace solve (rbf_solve_ws.hip), patches of any size -------------
 constexpr int kWsBlock = 32;   // block edge of the workspace factorisation
@@ -280,10 +285,21 @@ void launch_ws_reset(int *counters, bool all, hipStream_t s,
 // refinement (non-converging patches appended to the fallback list).
 void launch_ws_solve(bool fp32, int kind, const WsArgs &w, int grid,
                      hipStream_t s);
-// Evaluation of the heavy patches: chunks[c] = (patch, first pair) of
-// kHeavyChunk pairs; alpha from a.alphaHeavy, polynomial from coef.
-void launch_heavy_eval(int kind, const SolveArgs &a, const double *coef,
-                       const int2 *chunks, int nChunks, hipStream_t s);
+// Evaluation of the patches whose alpha the solve stored (alpha of source
+// t at a.alphaHeavy + a.heavyOff[k] + 8 t + f, zero past F; polynomial from
+// coef). Heavy patches: chunks[c] = (patch, first pair) of kHeavyChunk
+// pairs, one 256-thread workgroup each (pairsPerThread = 2).
+void launch_chunk_eval(int kind, const SolveArgs &a, const double *coef,
+                       const int2 *chunks, int nChunks, int pairsPerThread,
+                       hipStream_t s);
+// The others: chunks of kLightChunk pairs, one wave each (needs
+// a.slotXyz).
+void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,
+                      const int2 *chunks, int nChunks, hipStream_t s);
+// slotXyz of the member slots [0, total): 4 doubles each.
+void launch_slot_xyz(const int *srcIdx, int64_t total, const double *sx,
+                     const double *sy, const double *sz, double *xyz,
+                     hipStream_t s);
 // Fields handled by one poly/solve launch.
 int solve_fields_per_launch();
 // Polynomial fit of every patch (one wave per patch): coef[K][F][4], pvOut.

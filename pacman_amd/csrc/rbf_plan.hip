@@ -77,7 +77,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   counters_.alloc(kWsCounters);
   launch_ws_reset(counters_.get(), true, s);
   plan_workspace(s);
-  plan_heavy(poff, s);
+  plan_eval(poff, s);
   bins_.resize(solve_fields_per_launch());
   bins_for(1, s);
   pv_.alloc(std::max<int64_t>(K_, 1));
@@ -346,6 +346,8 @@ void RbfPumPlan::phaseTimes(double *ms) const {
   if (!opBuilt_) {
     PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[4], ev_[7]));
     ms[11] = t;
+    PACMAN_HIP_CHECK(hipEventElapsedTime(&t, ev_[8], ev_[9]));
+    ms[12] = t;
   }
 }
 
@@ -371,7 +373,7 @@ void RbfPumPlan::work(double *w) const {
     sn2 += n * n;
     sn3 += n * n * n;
     snm += n * (double)m;
-    const bool heavy = po[k + 1] - po[k] > kHeavyPairs;
+    const bool heavy = chunkEval_.size() == (size_t)K_ && chunkEval_[k];
     if (heavy)
       hnm += n * (double)m;
     if (n > tileMaxN_) {
@@ -596,8 +598,9 @@ SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
   a.singular = counters_.get() + kWsSingular;
   a.patchDinv = patchDinv_.n ? patchDinv_.get() : nullptr;
   a.patchRinv = patchRinv_.n ? patchRinv_.get() : nullptr;
-  a.heavyOff = nHeavyChunks_ ? heavyOff_.get() : nullptr;
+  a.heavyOff = nHeavyChunks_ + nLightChunks_ ? heavyOff_.get() : nullptr;
   a.alphaHeavy = alphaHeavy_.get();
+  a.slotXyz = slotXyz_.n ? slotXyz_.get() : nullptr;
   return a;
 }
 
@@ -863,8 +866,14 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
       launch_workspace(a, bu.coef, s, events && f0 == 0,
                        workspace_fp32(nf));
     }
-    launch_heavy_eval(prm_.kind, a, bu.coef, heavyChunks_.get(),
-                      nHeavyChunks_, s);
+    if (f0 == 0 && events)
+      PACMAN_HIP_CHECK(hipEventRecord(ev_[8], s));
+    launch_chunk_eval(prm_.kind, a, bu.coef, heavyChunks_.get(),
+                      nHeavyChunks_, 2, s);
+    launch_wave_eval(prm_.kind, a, bu.coef, lightChunks_.get(),
+                     nLightChunks_, s);
+    if (f0 == 0 && events)
+      PACMAN_HIP_CHECK(hipEventRecord(ev_[9], s));
     if (F <= perLaunch) {
       pending = bl;
       havePending = true;
@@ -1063,39 +1072,62 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // queues copied
 }
 
-// Patches holding more than kHeavyPairs (target, patch) pairs (large cells
-// of sparse regions next to dense ones: up to ~1e5 pairs) would keep one
-// workgroup evaluating for milliseconds; their alpha goes to alphaHeavy_
-// (n x 8 doubles each) and chunks of kHeavyChunk pairs are evaluated by
-// separate workgroups.
-void RbfPumPlan::plan_heavy(const std::vector<int64_t> &poff, hipStream_t s) {
-  nHeavyChunks_ = 0;
+// The solve kernels store alpha (alphaHeavy_, n x 8 doubles per patch) and
+// skip the evaluation for:
+//  * every patch of the register-resident solve: inside that kernel the
+//    evaluation ran at the solve's occupancy (two workgroups per CU, 8 or
+//    16 waves each), where its dependent pair, target and slot loads left
+//    it latency-bound (about a quarter of the solve's time); a launch of
+//    its own keeps seven 256-thread workgroups per CU in flight;
+//  * every patch with more than kHeavyPairs (target, patch) pairs (large
+//    cells of sparse regions next to dense ones: up to ~1e5 pairs), which
+//    would keep one workgroup evaluating for milliseconds.
+// Chunks of kLightChunk pairs (one wave each, sources read as wave-uniform
+// slot records) and kHeavyChunk pairs (a 256-thread workgroup each, two
+// pairs per thread) are then evaluated apart.
+void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
+  nHeavyChunks_ = nLightChunks_ = 0;
+  chunkEval_.assign(K_, 0);
   if (K_ == 0)
     return;
   std::vector<int64_t> hoff(K_, -1);
-  std::vector<int2> chunks;
+  std::vector<int2> heavy, light;
   int64_t total = 0;
   for (int64_t k = 0; k < K_; ++k) {
     const int64_t m = poff[k + 1] - poff[k];
-    if (m <= kHeavyPairs)
-      continue;
+    const bool isHeavy = m > kHeavyPairs;
+    if (!isHeavy && patchN_[k] > tileMaxN_)
+      continue;  // evaluated by the workspace kernel
+    chunkEval_[k] = 1;
     hoff[k] = total;
     total += (int64_t)patchN_[k] * solve_fields_per_launch();
-    for (int64_t q = 0; q < m; q += kHeavyChunk)
-      chunks.push_back(make_int2((int)k, (int)q));
+    const int step = isHeavy ? kHeavyChunk : kLightChunk;
+    for (int64_t q = 0; q < m; q += step)
+      (isHeavy ? heavy : light).push_back(make_int2((int)k, (int)q));
   }
-  if (chunks.empty())
+  if (heavy.empty() && light.empty())
     return;
   heavyOff_.alloc(K_);
   alphaHeavy_.alloc(std::max<int64_t>(total, 1));
-  heavyChunks_.alloc(chunks.size());
+  heavyChunks_.alloc(std::max<size_t>(heavy.size(), 1));
+  lightChunks_.alloc(std::max<size_t>(light.size(), 1));
   PACMAN_HIP_CHECK(hipMemcpyAsync(heavyOff_.get(), hoff.data(), 8 * K_,
                                   hipMemcpyHostToDevice, s));
-  PACMAN_HIP_CHECK(hipMemcpyAsync(heavyChunks_.get(), chunks.data(),
-                                  sizeof(int2) * chunks.size(),
-                                  hipMemcpyHostToDevice, s));
+  if (!heavy.empty())
+    PACMAN_HIP_CHECK(hipMemcpyAsync(heavyChunks_.get(), heavy.data(),
+                                    sizeof(int2) * heavy.size(),
+                                    hipMemcpyHostToDevice, s));
+  if (!light.empty()) {
+    PACMAN_HIP_CHECK(hipMemcpyAsync(lightChunks_.get(), light.data(),
+                                    sizeof(int2) * light.size(),
+                                    hipMemcpyHostToDevice, s));
+    slotXyz_.alloc((size_t)4 * std::max<int64_t>(totalSrc_, 1));
+    launch_slot_xyz(srcIdx_.get(), totalSrc_, gs_.view.x, gs_.view.y,
+                    gs_.view.z, slotXyz_.get(), s);
+  }
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
-  nHeavyChunks_ = (int)chunks.size();
+  nHeavyChunks_ = (int)heavy.size();
+  nLightChunks_ = (int)light.size();
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,

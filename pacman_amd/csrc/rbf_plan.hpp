@@ -123,16 +123,18 @@ public:
   // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total,
   // 9 polynomial fit of the first field chunk, 10 first-pass workspace
   // launches (patches above tileMaxN_) of the first field chunk, 11 the
-  // register-resident solve launches of the first field chunk.
-  static constexpr int kPhases = 12;
+  // register-resident solve launches of the first field chunk, 12 the
+  // chunked evaluation launches of the first field chunk.
+  static constexpr int kPhases = 13;
   void phaseTimes(double *ms) const;
   // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
   // targets), 5 pairs (targets within r of a centre), 6 max n; over the
   // patches above tileMaxN_: 7 count, 8 sum n^3, 9 sum n^2, 10 sum n m of
   // those evaluated by the workspace kernel, 11 sum n m of the patches
-  // evaluated in chunks (more than kHeavyPairs pairs); over the patches of
-  // the register-resident solve: 12 sum n^3, 13 sum n^2, 14 sum n m of those
-  // it evaluates itself.
+  // evaluated in chunks (every register-resident patch and every one with
+  // more than kHeavyPairs pairs); over the patches of the register-resident
+  // solve: 12 sum n^3, 13 sum n^2, 14 sum n m of those it evaluates itself
+  // (none: they are all evaluated in chunks).
   static constexpr int kWork = 15;
   void work(double *w) const;
   ~RbfPumPlan();
@@ -179,13 +181,16 @@ private:
                        std::chrono::steady_clock::time_point t0);
   void setup_adaptive(const double *dSrc, const double *dTgt, hipStream_t s);
   DevBuf<double> patchR_, patchRinv_, patchDinv_;  // adaptive plans only
-  // Patches with more than kHeavyPairs target pairs: evaluated in chunks
-  // of kHeavyChunk pairs by their own launch.
-  void plan_heavy(const std::vector<int64_t> &poff, hipStream_t s);
+  // Patches evaluated apart from their solve, in chunks of pairs by their
+  // own launches: every register-resident patch (kLightChunk pairs per
+  // chunk) and every patch with more than kHeavyPairs pairs (kHeavyChunk).
+  void plan_eval(const std::vector<int64_t> &poff, hipStream_t s);
   DevBuf<int64_t> heavyOff_;
   DevBuf<double> alphaHeavy_;
-  DevBuf<int2> heavyChunks_;
-  int nHeavyChunks_ = 0;
+  DevBuf<int2> heavyChunks_, lightChunks_;
+  DevBuf<double> slotXyz_;  // 4 per member slot, for the wave evaluation
+  int nHeavyChunks_ = 0, nLightChunks_ = 0;
+  std::vector<char> chunkEval_;  // per patch: evaluated in chunks
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, const double *coef, hipStream_t s,
                         bool timed, bool fp32);
@@ -244,8 +249,7 @@ private:
   DevBuf<int> tinv_;  // original target index -> cell order (operator apply)
   DevBuf<double> valsSorted_, contrib_, coef_;
   double setupMs_[5] = {0, 0, 0, 0, 0};
-  hipEvent_t ev_[8] = {nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_[10] = {};
   bool wsTimed_ = false;  // ev_[5], ev_[6] recorded by the last apply
   // more streams of the size-bin launches, forked from and joined to the
   // apply's stream

@@ -683,12 +683,15 @@ __global__ void src_reduce_kernel(const int64_t *off, const int *ent,
   }
 }
 
-// Evaluation of one chunk of kHeavyChunk (target, patch) pairs of a heavy
-// patch (src/rbf_pum/solver/solver.hpp:265-273): t = E alpha + V beta at
-// every member target of the chunk, two pairs per thread, the patch's
-// sources staged in LDS 256 at a time (sums in source order).
-template <int K>
-__global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
+// Evaluation of one chunk of 256 * PPT (target, patch) pairs of a patch
+// whose alpha the solve stored (src/rbf_pum/solver/solver.hpp:265-273):
+// t = E alpha + V beta at every member target of the chunk, PPT pairs per
+// thread, the patch's sources staged in LDS 256 at a time (sums in source
+// order, the same for every field count). Launched with PPT = 2 for the
+// patches with more than kHeavyPairs pairs (kHeavyChunk per chunk); a wave
+// whose lanes hold no pair only helps stage.
+template <int K, int PPT>
+__global__ __launch_bounds__(256) void chunk_eval_kernel(SolveArgs a,
                                                          const double *coef,
                                                          const int2 *chunks) {
   __shared__ double sxs[256], sys[256], szs[256];
@@ -701,7 +704,7 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
   const int64_t s0 = a.srcOff[patch];
   const int n = (int)(a.srcOff[patch + 1] - s0);
   const int64_t p0 = a.pairOff[patch];
-  const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + kHeavyChunk);
+  const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + 256 * PPT);
   const int F = a.F;
   const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
   const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
@@ -710,11 +713,11 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
     scoef[tid >> 2][tid & 3] =
         (tid >> 2) < F ? coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)]
                        : 0.0;
-  double yx[2], yy[2], yz[2];
-  int slot[2];
-  bool mem[2];
+  double yx[PPT], yy[PPT], yz[PPT];
+  int slot[PPT];
+  bool mem[PPT];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < PPT; ++h) {
     const int q = q0 + tid + 256 * h;
     mem[h] = false;
     yx[h] = yy[h] = yz[h] = 0.0;
@@ -729,9 +732,11 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
       slot[h] = a.pairSlot[p0 + q];
     }
   }
-  double acc[2][kFMax];
+  // pairs of this wave: q0 + (tid & ~63) + 256 h (the same for every lane)
+  const bool waveActive = q0 + (tid & ~63) < q1;
+  double acc[PPT][kFMax];
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+  for (int h = 0; h < PPT; ++h)
 #pragma unroll
     for (int f = 0; f < kFMax; ++f)
       acc[h][f] = 0.0;
@@ -744,13 +749,15 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
       sys[tid] = a.sy[p];
       szs[tid] = a.sz[p];
 #pragma unroll
-      for (int f = 0; f < kFMax; ++f)
-        sal[tid][f] = f < F ? ah[(int64_t)(j0 + tid) * F + f] : 0.0;
+      for (int f = 0; f < kFMax; ++f)  // zero past F in memory
+        sal[tid][f] = ah[(int64_t)(j0 + tid) * kFMax + f];
     }
     __syncthreads();
+    if (!waveActive)
+      continue;
     for (int j = 0; j < jn; ++j) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < PPT; ++h) {
         const double ph = wendland<K, double, true>(
             dist_fast(yx[h], yy[h], yz[h], sxs[j], sys[j], szs[j]), dinv);
         if (F > 4) {  // every product (zeros past F): paired reads
@@ -767,7 +774,7 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
     }
   }
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < PPT; ++h) {
     if (!mem[h])
       continue;
     for (int f = 0; f < F; ++f) {
@@ -776,6 +783,96 @@ __global__ __launch_bounds__(256) void heavy_eval_kernel(SolveArgs a,
       a.contrib[(int64_t)slot[h] * F + f] = acc[h][f] + vb;
     }
   }
+}
+
+// Evaluation of one chunk of up to kLightChunk (= 64) (target, patch) pairs
+// of a register-resident patch by one wave (src/rbf_pum/solver/solver.hpp:
+// 265-273): lane = pair; every lane walks the patch's sources in order, so
+// each source's coordinates (slotXyz, x y z 0 per member slot) and its
+// alpha record (8 doubles, zero past F) are wave-uniform reads (scalar
+// loads). Sums in source order, as chunk_eval_kernel. No LDS and no
+// barriers: waves of many patches stay in flight and hide one another's
+// load latencies.
+template <int K>
+__global__ __launch_bounds__(256) void wave_eval_kernel(
+    SolveArgs a, const double *__restrict__ coef,
+    const int2 *__restrict__ chunks, int nChunks) {
+  const int w = __builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (w >= nChunks)
+    return;
+  const int lane = threadIdx.x & 63;
+  const int2 ch = chunks[w];
+  const int patch = ch.x, q0 = ch.y;
+  const int64_t s0 = a.srcOff[patch];
+  const int n = (int)(a.srcOff[patch + 1] - s0);
+  const int64_t p0 = a.pairOff[patch];
+  const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + kLightChunk);
+  const int F = a.F;
+  const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
+  const double *__restrict__ al = a.alphaHeavy + a.heavyOff[patch];
+  const double *__restrict__ xyz = a.slotXyz + 4 * s0;
+  const int q = q0 + lane;
+  bool mem = false;
+  double yx = 0.0, yy = 0.0, yz = 0.0;
+  int slot = 0;
+  if (q < q1) {
+    const int e = a.pairTgt[p0 + q];
+    mem = ((unsigned)e >> 31) != 0;
+    const int tp = e & 0x7fffffff;
+    yx = a.tx[tp];
+    yy = a.ty[tp];
+    yz = a.tz[tp];
+    slot = a.pairSlot[p0 + q];
+  }
+  double acc[kFMax];
+#pragma unroll
+  for (int f = 0; f < kFMax; ++f)
+    acc[f] = 0.0;
+  if (F > 4) {  // every product (zeros past F)
+    for (int j = 0; j < n; ++j) {
+      const double ph = wendland<K, double, true>(
+          dist_fast(yx, yy, yz, xyz[4 * j], xyz[4 * j + 1], xyz[4 * j + 2]),
+          dinv);
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f)
+        acc[f] = fma(ph, al[j * kFMax + f], acc[f]);
+    }
+  } else {
+    for (int j = 0; j < n; ++j) {
+      const double ph = wendland<K, double, true>(
+          dist_fast(yx, yy, yz, xyz[4 * j], xyz[4 * j + 1], xyz[4 * j + 2]),
+          dinv);
+#pragma unroll
+      for (int f = 0; f < 4; ++f)
+        if (f < F)
+          acc[f] = fma(ph, al[j * kFMax + f], acc[f]);
+    }
+  }
+  if (!mem)
+    return;
+  const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
+  for (int f = 0; f < F; ++f) {
+    const double *c = coef + ((int64_t)patch * F + f) * 4;
+    const double vb =
+        c[0] + c[1] * (yx - cx) + c[2] * (yy - cy) + c[3] * (yz - cz);
+    a.contrib[(int64_t)slot * F + f] = acc[f] + vb;
+  }
+}
+
+// slotXyz[4 i + d] = coordinate d of the source in member slot i (the
+// fourth entry 0): the sources of a patch in slot order, contiguous.
+__global__ void slot_xyz_kernel(const int *srcIdx, int64_t total,
+                                const double *sx, const double *sy,
+                                const double *sz, double *xyz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total)
+    return;
+  const int p = srcIdx[i];
+  xyz[4 * i] = sx[p];
+  xyz[4 * i + 1] = sy[p];
+  xyz[4 * i + 2] = sz ? sz[p] : 0.0;
+  xyz[4 * i + 3] = 0.0;
 }
 
 __global__ void zero_i32_kernel(int *p, int64_t n) {
@@ -788,16 +885,46 @@ __global__ void zero_i32_kernel(int *p, int64_t n) {
 
 int solve_fields_per_launch() { return kFMax; }
 
-void launch_heavy_eval(int kind, const SolveArgs &a, const double *coef,
-                       const int2 *chunks, int nChunks, hipStream_t s) {
+void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,
+                      const int2 *chunks, int nChunks, hipStream_t s) {
   if (nChunks <= 0)
     return;
+  if (a.slotXyz == nullptr)
+    throw std::runtime_error("wave evaluation: slot coordinates missing");
+  const unsigned g = (unsigned)((nChunks + 3) / 4);
   switch (kind) {
-  case 0: heavy_eval_kernel<0><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  case 1: heavy_eval_kernel<1><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  case 2: heavy_eval_kernel<2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  case 3: heavy_eval_kernel<3><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  default: heavy_eval_kernel<4><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  case 0: wave_eval_kernel<0><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+  case 1: wave_eval_kernel<1><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+  case 2: wave_eval_kernel<2><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+  case 3: wave_eval_kernel<3><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+  default: wave_eval_kernel<4><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+  }
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void launch_slot_xyz(const int *srcIdx, int64_t total, const double *sx,
+                     const double *sy, const double *sz, double *xyz,
+                     hipStream_t s) {
+  if (total <= 0)
+    return;
+  slot_xyz_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+      srcIdx, total, sx, sy, sz, xyz);
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
+void launch_chunk_eval(int kind, const SolveArgs &a, const double *coef,
+                       const int2 *chunks, int nChunks, int pairsPerThread,
+                       hipStream_t s) {
+  if (nChunks <= 0)
+    return;
+  if (pairsPerThread != 2)
+    throw std::runtime_error("chunk evaluation: 2 pairs per thread");
+  switch (kind) {
+  case 0: chunk_eval_kernel<0, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  case 1: chunk_eval_kernel<1, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  case 2: chunk_eval_kernel<2, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  case 3: chunk_eval_kernel<3, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
+  default: chunk_eval_kernel<4, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
   }
   PACMAN_HIP_CHECK(hipGetLastError());
 }

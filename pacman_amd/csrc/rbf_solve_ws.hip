@@ -591,11 +591,12 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
         a.alphaOut[(s0 + c) * F + f] = alp[f * ast + c];
       }
     if (a.heavyOff && a.heavyOff[patch] >= 0) {
-      // evaluated by the heavy-patch kernel across many workgroups
+      // evaluated in chunks across many workgroups; alpha of source c at
+      // ah[c * kFMax + f], zero past F
       double *ah = a.alphaHeavy + a.heavyOff[patch];
-      for (int t = tid; t < F * n; t += NT) {
+      for (int t = tid; t < kFMax * n; t += NT) {
         const int f = t / n, c = t % n;
-        ah[c * F + f] = alp[f * ast + c];
+        ah[c * kFMax + f] = f < F ? alp[f * ast + c] : 0.0;
       }
       if (tid == 0 && a.status)
         a.status[patch] = status;

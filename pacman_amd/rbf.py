@@ -468,7 +468,7 @@ class Plan:
 
     PHASES = ("grids", "radius", "centres", "membership", "setup",
               "gather", "solve", "blend", "apply", "poly", "workspace",
-              "tiles")
+              "tiles", "eval")
     WORK = ("patches", "sum_n", "sum_n2", "sum_n3", "sum_nm", "pairs",
             "max_n", "large", "large_sum_n3", "large_sum_n2", "large_sum_nm",
             "heavy_sum_nm", "tile_sum_n3", "tile_sum_n2", "tile_sum_nm")
