@@ -233,7 +233,7 @@ struct SolveArgs {
   // and those with more member-target pairs than kHeavyPairs): heavyOff[k]
   // >= 0 is where their alpha goes (alphaHeavy + heavyOff[k] + e * F + f);
   // the solve kernels store it and skip the evaluation, which
-  // launch_chunk_eval spreads over many workgroups.
+  // launch_wave_eval spreads over many waves.
   const int64_t *heavyOff;
   double *alphaHeavy;
   // x y z 0 of the source in each member slot (4 per slot, slot order);
@@ -241,8 +241,6 @@ struct SolveArgs {
   const double *slotXyz;
 };
 constexpr int kHeavyPairs = 4096;
-constexpr int kHeavyChunk = 512;  // pairs per workgroup, heavy patches
-constexpr int kLightChunk = 64;   // pairs per wave, the others
 
 // ---- workspace solve (rbf_solve_ws.hip), patches of any size -------------
 constexpr int kWsBlock = 32;   // block edge of the workspace factorisation
@@ -287,15 +285,11 @@ void launch_ws_solve(bool fp32, int kind, const WsArgs &w, int grid,
                      hipStream_t s);
 // Evaluation of the patches whose alpha the solve stored (alpha of source
 // t at a.alphaHeavy + a.heavyOff[k] + 8 t + f, zero past F; polynomial from
-// coef). Heavy patches: chunks[c] = (patch, first pair) of kHeavyChunk
-// pairs, one 256-thread workgroup each (pairsPerThread = 2).
-void launch_chunk_eval(int kind, const SolveArgs &a, const double *coef,
-                       const int2 *chunks, int nChunks, int pairsPerThread,
-                       hipStream_t s);
-// The others: chunks of kLightChunk pairs, one wave each (needs
-// a.slotXyz).
+// coef; source coordinates from a.slotXyz): chunks[c] = (patch, first pair)
+// of up to 64 pairsPerLane (1 to 3) pairs, one wave each.
 void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,
-                      const int2 *chunks, int nChunks, hipStream_t s);
+                      const int2 *chunks, int nChunks, int pairsPerLane,
+                      hipStream_t s);
 // slotXyz of the member slots [0, total): 4 doubles each.
 void launch_slot_xyz(const int *srcIdx, int64_t total, const double *sx,
                      const double *sy, const double *sz, double *xyz,

@@ -598,7 +598,7 @@ SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
   a.singular = counters_.get() + kWsSingular;
   a.patchDinv = patchDinv_.n ? patchDinv_.get() : nullptr;
   a.patchRinv = patchRinv_.n ? patchRinv_.get() : nullptr;
-  a.heavyOff = nHeavyChunks_ + nLightChunks_ ? heavyOff_.get() : nullptr;
+  a.heavyOff = heavyOff_.n ? heavyOff_.get() : nullptr;
   a.alphaHeavy = alphaHeavy_.get();
   a.slotXyz = slotXyz_.n ? slotXyz_.get() : nullptr;
   return a;
@@ -868,10 +868,9 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     }
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[8], s));
-    launch_chunk_eval(prm_.kind, a, bu.coef, heavyChunks_.get(),
-                      nHeavyChunks_, 2, s);
-    launch_wave_eval(prm_.kind, a, bu.coef, lightChunks_.get(),
-                     nLightChunks_, s);
+    for (int i = 2; i >= 0; --i)
+      launch_wave_eval(prm_.kind, a, bu.coef, evalChunks_[i].get(),
+                       nEvalChunks_[i], i + 1, s);
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[9], s));
     if (F <= perLaunch) {
@@ -1077,57 +1076,64 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
 //  * every patch of the register-resident solve: inside that kernel the
 //    evaluation ran at the solve's occupancy (two workgroups per CU, 8 or
 //    16 waves each), where its dependent pair, target and slot loads left
-//    it latency-bound (about a quarter of the solve's time); a launch of
-//    its own keeps seven 256-thread workgroups per CU in flight;
+//    it latency-bound (about a quarter of the solve's time);
 //  * every patch with more than kHeavyPairs (target, patch) pairs (large
 //    cells of sparse regions next to dense ones: up to ~1e5 pairs), which
 //    would keep one workgroup evaluating for milliseconds.
-// Chunks of kLightChunk pairs (one wave each, sources read as wave-uniform
-// slot records) and kHeavyChunk pairs (a 256-thread workgroup each, two
-// pairs per thread) are then evaluated apart.
+// They are evaluated apart, one wave per chunk of u units of 64 pairs
+// (u pairs per lane): a patch of one unit in one chunk, the others in
+// chunks of two units and, for an odd count, one last chunk of three.
 void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
-  nHeavyChunks_ = nLightChunks_ = 0;
+  for (int i = 0; i < 3; ++i)
+    nEvalChunks_[i] = 0;
   chunkEval_.assign(K_, 0);
   if (K_ == 0)
     return;
   std::vector<int64_t> hoff(K_, -1);
-  std::vector<int2> heavy, light;
+  std::vector<int2> lists[3];
   int64_t total = 0;
   for (int64_t k = 0; k < K_; ++k) {
     const int64_t m = poff[k + 1] - poff[k];
-    const bool isHeavy = m > kHeavyPairs;
-    if (!isHeavy && patchN_[k] > tileMaxN_)
+    if (m <= kHeavyPairs && patchN_[k] > tileMaxN_)
       continue;  // evaluated by the workspace kernel
     chunkEval_[k] = 1;
     hoff[k] = total;
     total += (int64_t)patchN_[k] * solve_fields_per_launch();
-    const int step = isHeavy ? kHeavyChunk : kLightChunk;
-    for (int64_t q = 0; q < m; q += step)
-      (isHeavy ? heavy : light).push_back(make_int2((int)k, (int)q));
+    const int64_t u = (m + 63) / 64;
+    if (u <= 1) {
+      if (m > 0)
+        lists[0].push_back(make_int2((int)k, 0));
+      continue;
+    }
+    int64_t q = 0;
+    for (int64_t left = u; left > 0;) {
+      const int take = left == 3 ? 3 : 2;
+      lists[take - 1].push_back(make_int2((int)k, (int)q));
+      q += 64 * take;
+      left -= take;
+    }
   }
-  if (heavy.empty() && light.empty())
+  if (total == 0)
     return;
   heavyOff_.alloc(K_);
-  alphaHeavy_.alloc(std::max<int64_t>(total, 1));
-  heavyChunks_.alloc(std::max<size_t>(heavy.size(), 1));
-  lightChunks_.alloc(std::max<size_t>(light.size(), 1));
+  // one spare record: the wave evaluation loads the record past a patch's
+  // last before it knows it is done
+  alphaHeavy_.alloc(total + solve_fields_per_launch());
   PACMAN_HIP_CHECK(hipMemcpyAsync(heavyOff_.get(), hoff.data(), 8 * K_,
                                   hipMemcpyHostToDevice, s));
-  if (!heavy.empty())
-    PACMAN_HIP_CHECK(hipMemcpyAsync(heavyChunks_.get(), heavy.data(),
-                                    sizeof(int2) * heavy.size(),
-                                    hipMemcpyHostToDevice, s));
-  if (!light.empty()) {
-    PACMAN_HIP_CHECK(hipMemcpyAsync(lightChunks_.get(), light.data(),
-                                    sizeof(int2) * light.size(),
-                                    hipMemcpyHostToDevice, s));
-    slotXyz_.alloc((size_t)4 * std::max<int64_t>(totalSrc_, 1));
-    launch_slot_xyz(srcIdx_.get(), totalSrc_, gs_.view.x, gs_.view.y,
-                    gs_.view.z, slotXyz_.get(), s);
+  for (int i = 0; i < 3; ++i) {
+    evalChunks_[i].alloc(std::max<size_t>(lists[i].size(), 1));
+    if (!lists[i].empty())
+      PACMAN_HIP_CHECK(hipMemcpyAsync(evalChunks_[i].get(), lists[i].data(),
+                                      sizeof(int2) * lists[i].size(),
+                                      hipMemcpyHostToDevice, s));
   }
+  slotXyz_.alloc((size_t)4 * (totalSrc_ + 1));  // one spare record
+  launch_slot_xyz(srcIdx_.get(), totalSrc_, gs_.view.x, gs_.view.y,
+                  gs_.view.z, slotXyz_.get(), s);
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
-  nHeavyChunks_ = (int)heavy.size();
-  nLightChunks_ = (int)light.size();
+  for (int i = 0; i < 3; ++i)
+    nEvalChunks_[i] = (int)lists[i].size();
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,

@@ -181,15 +181,16 @@ private:
                        std::chrono::steady_clock::time_point t0);
   void setup_adaptive(const double *dSrc, const double *dTgt, hipStream_t s);
   DevBuf<double> patchR_, patchRinv_, patchDinv_;  // adaptive plans only
-  // Patches evaluated apart from their solve, in chunks of pairs by their
-  // own launches: every register-resident patch (kLightChunk pairs per
-  // chunk) and every patch with more than kHeavyPairs pairs (kHeavyChunk).
+  // Patches evaluated apart from their solve, in chunks of pairs (one wave
+  // each) by their own launches: every register-resident patch and every
+  // patch with more than kHeavyPairs pairs.
   void plan_eval(const std::vector<int64_t> &poff, hipStream_t s);
   DevBuf<int64_t> heavyOff_;
   DevBuf<double> alphaHeavy_;
-  DevBuf<int2> heavyChunks_, lightChunks_;
+  // chunks of the wave evaluation with 1, 2 and 3 pairs per lane
+  DevBuf<int2> evalChunks_[3];
+  int nEvalChunks_[3] = {0, 0, 0};
   DevBuf<double> slotXyz_;  // 4 per member slot, for the wave evaluation
-  int nHeavyChunks_ = 0, nLightChunks_ = 0;
   std::vector<char> chunkEval_;  // per patch: evaluated in chunks
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, const double *coef, hipStream_t s,

@@ -683,42 +683,46 @@ __global__ void src_reduce_kernel(const int64_t *off, const int *ent,
   }
 }
 
-// Evaluation of one chunk of 256 * PPT (target, patch) pairs of a patch
-// whose alpha the solve stored (src/rbf_pum/solver/solver.hpp:265-273):
-// t = E alpha + V beta at every member target of the chunk, PPT pairs per
-// thread, the patch's sources staged in LDS 256 at a time (sums in source
-// order, the same for every field count). Launched with PPT = 2 for the
-// patches with more than kHeavyPairs pairs (kHeavyChunk per chunk); a wave
-// whose lanes hold no pair only helps stage.
-template <int K, int PPT>
-__global__ __launch_bounds__(256) void chunk_eval_kernel(SolveArgs a,
-                                                         const double *coef,
-                                                         const int2 *chunks) {
-  __shared__ double sxs[256], sys[256], szs[256];
-  // alpha of the staged sources with the fields fastest (zero past F): one
-  // 16-byte LDS read gives two fields
-  __shared__ double sal[256][kFMax];
-  __shared__ double scoef[kFMax][4];
-  const int tid = threadIdx.x;
-  const int patch = chunks[blockIdx.x].x, q0 = chunks[blockIdx.x].y;
+// Evaluation of one chunk of up to 64 PPL (target, patch) pairs of a patch
+// whose solve stored its alpha, by one wave (src/rbf_pum/solver/solver.hpp:
+// 265-273): lane l holds pairs q0 + l + 64 h, h < PPL (the staged reads
+// below serve all of them; from PPL = 2 on the VALU, not the LDS, sets the
+// pace), sums over the patch's sources in source order. The wave stages its patch's sources kBlk at a time in
+// its own LDS region (coalesced vector loads of the slot records: x y z 0
+// from slotXyz, 8 alpha from alphaHeavy, zero past F), loading the next
+// block while it computes this one; no workgroup barriers, so the waves of
+// many patches stay in flight. NF products per source: 8 for more than four
+// fields, else 4 (the records hold zeros past F); the phi code is the same
+// in both, so a field's result does not depend on the field count.
+constexpr int kBlk = 32;
+
+template <int K, int NF, int PPL>
+__global__ __launch_bounds__(256) void wave_eval_kernel(
+    SolveArgs a, const double *__restrict__ coef,
+    const int2 *__restrict__ chunks, int nChunks) {
+  __shared__ double sxyz[4][kBlk][4];
+  __shared__ double sal[4][kBlk * kFMax];
+  const int wv = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + wv));
+  if (w >= nChunks)
+    return;  // no workgroup barriers below
+  const int lane = threadIdx.x & 63;
+  const int2 ch = chunks[w];
+  const int patch = ch.x, q0 = ch.y;
   const int64_t s0 = a.srcOff[patch];
   const int n = (int)(a.srcOff[patch + 1] - s0);
   const int64_t p0 = a.pairOff[patch];
-  const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + 256 * PPT);
+  const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + 64 * PPL);
   const int F = a.F;
   const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
-  const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
-  const double *ah = a.alphaHeavy + a.heavyOff[patch];
-  if (tid < kFMax * 4)
-    scoef[tid >> 2][tid & 3] =
-        (tid >> 2) < F ? coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)]
-                       : 0.0;
-  double yx[PPT], yy[PPT], yz[PPT];
-  int slot[PPT];
-  bool mem[PPT];
+  const double *__restrict__ al = a.alphaHeavy + a.heavyOff[patch];
+  const double *__restrict__ xyz = a.slotXyz + 4 * s0;
+  bool mem[PPL];
+  double yx[PPL], yy[PPL], yz[PPL];
+  int slot[PPL];
 #pragma unroll
-  for (int h = 0; h < PPT; ++h) {
-    const int q = q0 + tid + 256 * h;
+  for (int h = 0; h < PPL; ++h) {
+    const int q = q0 + lane + 64 * h;
     mem[h] = false;
     yx[h] = yy[h] = yz[h] = 0.0;
     slot[h] = 0;
@@ -732,131 +736,66 @@ __global__ __launch_bounds__(256) void chunk_eval_kernel(SolveArgs a,
       slot[h] = a.pairSlot[p0 + q];
     }
   }
-  // pairs of this wave: q0 + (tid & ~63) + 256 h (the same for every lane)
-  const bool waveActive = q0 + (tid & ~63) < q1;
-  double acc[PPT][kFMax];
+  // staged block: lane l < kBlk holds source b0 + l's x y z 0, every lane 4
+  // doubles of the block's alpha records (8 per source)
+  double4 cxyz = {0.0, 0.0, 0.0, 0.0}, c4 = {0.0, 0.0, 0.0, 0.0};
+  auto load = [&](int b0) {
+    const int i = b0 + lane;
+    if (lane < kBlk && i < n)
+      cxyz = *reinterpret_cast<const double4 *>(xyz + 4 * (int64_t)i);
+    const int e = kFMax * b0 + 4 * lane;
+    if (e < kFMax * n)
+      c4 = *reinterpret_cast<const double4 *>(al + e);
+  };
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  double acc[PPL][NF];
 #pragma unroll
-  for (int h = 0; h < PPT; ++h)
+  for (int h = 0; h < PPL; ++h)
 #pragma unroll
-    for (int f = 0; f < kFMax; ++f)
+    for (int f = 0; f < NF; ++f)
       acc[h][f] = 0.0;
-  for (int j0 = 0; j0 < n; j0 += 256) {
-    const int jn = min(256, n - j0);
-    __syncthreads();
-    if (tid < jn) {
-      const int p = a.srcIdx[s0 + j0 + tid];
-      sxs[tid] = a.sx[p];
-      sys[tid] = a.sy[p];
-      szs[tid] = a.sz[p];
+  load(0);
+  for (int b0 = 0; b0 < n; b0 += kBlk) {
+    if (lane < kBlk)
+      *reinterpret_cast<double4 *>(&sxyz[wv][lane][0]) = cxyz;
+    *reinterpret_cast<double4 *>(&sal[wv][4 * lane]) = c4;
+    wave_sync();
+    if (b0 + kBlk < n)
+      load(b0 + kBlk);
+    const int bn = min(kBlk, n - b0);
+    for (int j = 0; j < bn; ++j) {
+      const double x = sxyz[wv][j][0], y = sxyz[wv][j][1],
+                   z = sxyz[wv][j][2];
 #pragma unroll
-      for (int f = 0; f < kFMax; ++f)  // zero past F in memory
-        sal[tid][f] = ah[(int64_t)(j0 + tid) * kFMax + f];
-    }
-    __syncthreads();
-    if (!waveActive)
-      continue;
-    for (int j = 0; j < jn; ++j) {
+      for (int h = 0; h < PPL; ++h) {
+        const double ph =
+            wendland<K, double, true>(dist_fast(yx[h], yy[h], yz[h], x, y, z),
+                                      dinv);
 #pragma unroll
-      for (int h = 0; h < PPT; ++h) {
-        const double ph = wendland<K, double, true>(
-            dist_fast(yx[h], yy[h], yz[h], sxs[j], sys[j], szs[j]), dinv);
-        if (F > 4) {  // every product (zeros past F): paired reads
-#pragma unroll
-          for (int f = 0; f < kFMax; ++f)
-            acc[h][f] = fma(ph, sal[j][f], acc[h][f]);
-        } else {
-#pragma unroll
-          for (int f = 0; f < kFMax; ++f)
-            if (f < F)
-              acc[h][f] = fma(ph, sal[j][f], acc[h][f]);
-        }
+        for (int f = 0; f < NF; ++f)
+          acc[h][f] = fma(ph, sal[wv][j * kFMax + f], acc[h][f]);
       }
     }
+    wave_sync();  // this block read before the next one is written
   }
+  const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
 #pragma unroll
-  for (int h = 0; h < PPT; ++h) {
+  for (int h = 0; h < PPL; ++h) {
     if (!mem[h])
       continue;
-    for (int f = 0; f < F; ++f) {
-      const double vb = scoef[f][0] + scoef[f][1] * (yx[h] - cx) +
-                        scoef[f][2] * (yy[h] - cy) + scoef[f][3] * (yz[h] - cz);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (f >= F)
+        break;
+      const double *c = coef + ((int64_t)patch * F + f) * 4;
+      const double vb = c[0] + c[1] * (yx[h] - cx) + c[2] * (yy[h] - cy) +
+                        c[3] * (yz[h] - cz);
       a.contrib[(int64_t)slot[h] * F + f] = acc[h][f] + vb;
     }
-  }
-}
-
-// Evaluation of one chunk of up to kLightChunk (= 64) (target, patch) pairs
-// of a register-resident patch by one wave (src/rbf_pum/solver/solver.hpp:
-// 265-273): lane = pair; every lane walks the patch's sources in order, so
-// each source's coordinates (slotXyz, x y z 0 per member slot) and its
-// alpha record (8 doubles, zero past F) are wave-uniform reads (scalar
-// loads). Sums in source order, as chunk_eval_kernel. No LDS and no
-// barriers: waves of many patches stay in flight and hide one another's
-// load latencies.
-template <int K>
-__global__ __launch_bounds__(256) void wave_eval_kernel(
-    SolveArgs a, const double *__restrict__ coef,
-    const int2 *__restrict__ chunks, int nChunks) {
-  const int w = __builtin_amdgcn_readfirstlane(
-      (int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-  if (w >= nChunks)
-    return;
-  const int lane = threadIdx.x & 63;
-  const int2 ch = chunks[w];
-  const int patch = ch.x, q0 = ch.y;
-  const int64_t s0 = a.srcOff[patch];
-  const int n = (int)(a.srcOff[patch + 1] - s0);
-  const int64_t p0 = a.pairOff[patch];
-  const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + kLightChunk);
-  const int F = a.F;
-  const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
-  const double *__restrict__ al = a.alphaHeavy + a.heavyOff[patch];
-  const double *__restrict__ xyz = a.slotXyz + 4 * s0;
-  const int q = q0 + lane;
-  bool mem = false;
-  double yx = 0.0, yy = 0.0, yz = 0.0;
-  int slot = 0;
-  if (q < q1) {
-    const int e = a.pairTgt[p0 + q];
-    mem = ((unsigned)e >> 31) != 0;
-    const int tp = e & 0x7fffffff;
-    yx = a.tx[tp];
-    yy = a.ty[tp];
-    yz = a.tz[tp];
-    slot = a.pairSlot[p0 + q];
-  }
-  double acc[kFMax];
-#pragma unroll
-  for (int f = 0; f < kFMax; ++f)
-    acc[f] = 0.0;
-  if (F > 4) {  // every product (zeros past F)
-    for (int j = 0; j < n; ++j) {
-      const double ph = wendland<K, double, true>(
-          dist_fast(yx, yy, yz, xyz[4 * j], xyz[4 * j + 1], xyz[4 * j + 2]),
-          dinv);
-#pragma unroll
-      for (int f = 0; f < kFMax; ++f)
-        acc[f] = fma(ph, al[j * kFMax + f], acc[f]);
-    }
-  } else {
-    for (int j = 0; j < n; ++j) {
-      const double ph = wendland<K, double, true>(
-          dist_fast(yx, yy, yz, xyz[4 * j], xyz[4 * j + 1], xyz[4 * j + 2]),
-          dinv);
-#pragma unroll
-      for (int f = 0; f < 4; ++f)
-        if (f < F)
-          acc[f] = fma(ph, al[j * kFMax + f], acc[f]);
-    }
-  }
-  if (!mem)
-    return;
-  const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
-  for (int f = 0; f < F; ++f) {
-    const double *c = coef + ((int64_t)patch * F + f) * 4;
-    const double vb =
-        c[0] + c[1] * (yx - cx) + c[2] * (yy - cy) + c[3] * (yz - cz);
-    a.contrib[(int64_t)slot * F + f] = acc[f] + vb;
   }
 }
 
@@ -886,19 +825,42 @@ __global__ void zero_i32_kernel(int *p, int64_t n) {
 int solve_fields_per_launch() { return kFMax; }
 
 void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,
-                      const int2 *chunks, int nChunks, hipStream_t s) {
+                      const int2 *chunks, int nChunks, int pairsPerLane,
+                      hipStream_t s) {
   if (nChunks <= 0)
     return;
   if (a.slotXyz == nullptr)
     throw std::runtime_error("wave evaluation: slot coordinates missing");
   const unsigned g = (unsigned)((nChunks + 3) / 4);
-  switch (kind) {
-  case 0: wave_eval_kernel<0><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
-  case 1: wave_eval_kernel<1><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
-  case 2: wave_eval_kernel<2><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
-  case 3: wave_eval_kernel<3><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
-  default: wave_eval_kernel<4><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
-  }
+  if (pairsPerLane < 1 || pairsPerLane > 3)
+    throw std::runtime_error("wave evaluation: 1 to 3 pairs per lane");
+  auto go = [&](auto nf, auto ppl) {
+    constexpr int NF = decltype(nf)::value, P = decltype(ppl)::value;
+    switch (kind) {
+    case 0: wave_eval_kernel<0, NF, P><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+    case 1: wave_eval_kernel<1, NF, P><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+    case 2: wave_eval_kernel<2, NF, P><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+    case 3: wave_eval_kernel<3, NF, P><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+    default: wave_eval_kernel<4, NF, P><<<g, 256, 0, s>>>(a, coef, chunks, nChunks); break;
+    }
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  using I8 = std::integral_constant<int, 8>;
+  auto byPpl = [&](auto nf) {
+    if (pairsPerLane == 1)
+      go(nf, I1{});
+    else if (pairsPerLane == 2)
+      go(nf, I2{});
+    else
+      go(nf, I3{});
+  };
+  if (a.F > 4)
+    byPpl(I8{});
+  else
+    byPpl(I4{});
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
@@ -912,22 +874,6 @@ void launch_slot_xyz(const int *srcIdx, int64_t total, const double *sx,
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
-void launch_chunk_eval(int kind, const SolveArgs &a, const double *coef,
-                       const int2 *chunks, int nChunks, int pairsPerThread,
-                       hipStream_t s) {
-  if (nChunks <= 0)
-    return;
-  if (pairsPerThread != 2)
-    throw std::runtime_error("chunk evaluation: 2 pairs per thread");
-  switch (kind) {
-  case 0: chunk_eval_kernel<0, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  case 1: chunk_eval_kernel<1, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  case 2: chunk_eval_kernel<2, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  case 3: chunk_eval_kernel<3, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  default: chunk_eval_kernel<4, 2><<<nChunks, 256, 0, s>>>(a, coef, chunks); break;
-  }
-  PACMAN_HIP_CHECK(hipGetLastError());
-}
 
 void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s) {
   if (K <= 0)
