@@ -210,8 +210,10 @@ int pacman_rbf_plan_patch_info(pacman_rbf_plan plan, int *polyColumns,
  * 7 blend, 8 apply total, 9 polynomial fit of the first field chunk,
  * 10 first-pass workspace launches (patches above 264 sources) of the first
  * field chunk (0 when the plan has none), 11 the register-resident solve
- * launches of the first field chunk, 12 the chunked evaluation launches of
- * the first field chunk (the patches whose solve stored alpha). */
+ * launches of the first field chunk, 12 from the end of those to the end of
+ * the chunked evaluation of the first field chunk (the patches whose solve
+ * stored alpha; it runs beside the workspace launches, so 10 lies inside
+ * 12). */
 int pacman_rbf_plan_phase_times(pacman_rbf_plan plan, double *ms, int n);
 
 /* Work counters (up to n entries; returns the count written): 0 patches K,

@@ -85,6 +85,8 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   for (auto &e : ev_)
     PACMAN_HIP_CHECK(hipEventCreate(&e));
   PACMAN_HIP_CHECK(hipEventCreateWithFlags(&bfork_, hipEventDisableTiming));
+  PACMAN_HIP_CHECK(hipEventCreateWithFlags(&efork_, hipEventDisableTiming));
+  PACMAN_HIP_CHECK(hipEventCreateWithFlags(&ejoin_, hipEventDisableTiming));
   for (int k = 0; k + 1 < kBinStreams; ++k) {
     PACMAN_HIP_CHECK(
         hipStreamCreateWithFlags(&bstream_[k], hipStreamNonBlocking));
@@ -312,6 +314,10 @@ RbfPumPlan::~RbfPumPlan() {
       (void)hipEventDestroy(e);
   if (bfork_)
     (void)hipEventDestroy(bfork_);
+  if (efork_)
+    (void)hipEventDestroy(efork_);
+  if (ejoin_)
+    (void)hipEventDestroy(ejoin_);
   for (int k = 0; k + 1 < kBinStreams; ++k) {
     if (bjoin_[k])
       (void)hipEventDestroy(bjoin_[k]);
@@ -860,17 +866,32 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
     });
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[7], s));
+    if (f0 == 0 && events)
+      PACMAN_HIP_CHECK(hipEventRecord(ev_[8], s));
+    // the register-resident patches' evaluation on a second stream, beside
+    // the workspace kernel (one workgroup per CU leaves room for its waves)
+    const bool side = !wsHost_.empty();
+    hipStream_t es = side ? bstream_[0] : s;
+    if (side) {
+      PACMAN_HIP_CHECK(hipEventRecord(efork_, s));
+      PACMAN_HIP_CHECK(hipStreamWaitEvent(es, efork_, 0));
+    }
+    for (int i = 2; i >= 0; --i)
+      launch_wave_eval(prm_.kind, a, bu.coef, evalChunks_[0][i].get(),
+                       nEvalChunks_[0][i], i + 1, es);
     if (!wsHost_.empty()) {
       if (f0 > 0)
         launch_ws_reset(counters_.get(), false, s);
       launch_workspace(a, bu.coef, s, events && f0 == 0,
                        workspace_fp32(nf));
     }
-    if (f0 == 0 && events)
-      PACMAN_HIP_CHECK(hipEventRecord(ev_[8], s));
     for (int i = 2; i >= 0; --i)
-      launch_wave_eval(prm_.kind, a, bu.coef, evalChunks_[i].get(),
-                       nEvalChunks_[i], i + 1, s);
+      launch_wave_eval(prm_.kind, a, bu.coef, evalChunks_[1][i].get(),
+                       nEvalChunks_[1][i], i + 1, s);
+    if (side) {
+      PACMAN_HIP_CHECK(hipEventRecord(ejoin_, es));
+      PACMAN_HIP_CHECK(hipStreamWaitEvent(s, ejoin_, 0));
+    }
     if (f0 == 0 && events)
       PACMAN_HIP_CHECK(hipEventRecord(ev_[9], s));
     if (F <= perLaunch) {
@@ -1082,15 +1103,19 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
 //    would keep one workgroup evaluating for milliseconds.
 // They are evaluated apart, one wave per chunk of u units of 64 pairs
 // (u pairs per lane): a patch of one unit in one chunk, the others in
-// chunks of two units and, for an odd count, one last chunk of three.
+// chunks of two units and, for an odd count, one last chunk of three. The
+// chunks of the register-resident patches run beside the workspace kernel
+// (their alpha is ready once the size bins are done), those of workspace
+// patches after it.
 void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
-  for (int i = 0; i < 3; ++i)
-    nEvalChunks_[i] = 0;
+  for (int g = 0; g < 2; ++g)
+    for (int i = 0; i < 3; ++i)
+      nEvalChunks_[g][i] = 0;
   chunkEval_.assign(K_, 0);
   if (K_ == 0)
     return;
   std::vector<int64_t> hoff(K_, -1);
-  std::vector<int2> lists[3];
+  std::vector<int2> lists[2][3];
   int64_t total = 0;
   for (int64_t k = 0; k < K_; ++k) {
     const int64_t m = poff[k + 1] - poff[k];
@@ -1099,16 +1124,17 @@ void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
     chunkEval_[k] = 1;
     hoff[k] = total;
     total += (int64_t)patchN_[k] * solve_fields_per_launch();
+    auto &L = lists[patchN_[k] > tileMaxN_ ? 1 : 0];
     const int64_t u = (m + 63) / 64;
     if (u <= 1) {
       if (m > 0)
-        lists[0].push_back(make_int2((int)k, 0));
+        L[0].push_back(make_int2((int)k, 0));
       continue;
     }
     int64_t q = 0;
     for (int64_t left = u; left > 0;) {
       const int take = left == 3 ? 3 : 2;
-      lists[take - 1].push_back(make_int2((int)k, (int)q));
+      L[take - 1].push_back(make_int2((int)k, (int)q));
       q += 64 * take;
       left -= take;
     }
@@ -1121,19 +1147,21 @@ void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
   alphaHeavy_.alloc(total + solve_fields_per_launch());
   PACMAN_HIP_CHECK(hipMemcpyAsync(heavyOff_.get(), hoff.data(), 8 * K_,
                                   hipMemcpyHostToDevice, s));
-  for (int i = 0; i < 3; ++i) {
-    evalChunks_[i].alloc(std::max<size_t>(lists[i].size(), 1));
-    if (!lists[i].empty())
-      PACMAN_HIP_CHECK(hipMemcpyAsync(evalChunks_[i].get(), lists[i].data(),
-                                      sizeof(int2) * lists[i].size(),
-                                      hipMemcpyHostToDevice, s));
-  }
+  for (int g = 0; g < 2; ++g)
+    for (int i = 0; i < 3; ++i) {
+      evalChunks_[g][i].alloc(std::max<size_t>(lists[g][i].size(), 1));
+      if (!lists[g][i].empty())
+        PACMAN_HIP_CHECK(hipMemcpyAsync(
+            evalChunks_[g][i].get(), lists[g][i].data(),
+            sizeof(int2) * lists[g][i].size(), hipMemcpyHostToDevice, s));
+    }
   slotXyz_.alloc((size_t)4 * (totalSrc_ + 1));  // one spare record
   launch_slot_xyz(srcIdx_.get(), totalSrc_, gs_.view.x, gs_.view.y,
                   gs_.view.z, slotXyz_.get(), s);
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
-  for (int i = 0; i < 3; ++i)
-    nEvalChunks_[i] = (int)lists[i].size();
+  for (int g = 0; g < 2; ++g)
+    for (int i = 0; i < 3; ++i)
+      nEvalChunks_[g][i] = (int)lists[g][i].size();
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,

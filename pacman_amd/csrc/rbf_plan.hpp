@@ -123,8 +123,10 @@ public:
   // stream: 5 value gather, 6 per-patch solve, 7 blend, 8 apply total,
   // 9 polynomial fit of the first field chunk, 10 first-pass workspace
   // launches (patches above tileMaxN_) of the first field chunk, 11 the
-  // register-resident solve launches of the first field chunk, 12 the
-  // chunked evaluation launches of the first field chunk.
+  // register-resident solve launches of the first field chunk, 12 from the
+  // end of those to the end of the chunked evaluation of the first field
+  // chunk (the evaluation of the register-resident patches runs beside the
+  // workspace launches, so 10 lies inside 12).
   static constexpr int kPhases = 13;
   void phaseTimes(double *ms) const;
   // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
@@ -187,9 +189,11 @@ private:
   void plan_eval(const std::vector<int64_t> &poff, hipStream_t s);
   DevBuf<int64_t> heavyOff_;
   DevBuf<double> alphaHeavy_;
-  // chunks of the wave evaluation with 1, 2 and 3 pairs per lane
-  DevBuf<int2> evalChunks_[3];
-  int nEvalChunks_[3] = {0, 0, 0};
+  // chunks of the wave evaluation with 1, 2 and 3 pairs per lane, [0] of
+  // the patches of the register-resident solve, [1] of the workspace ones
+  DevBuf<int2> evalChunks_[2][3];
+  int nEvalChunks_[2][3] = {};
+  hipEvent_t efork_ = nullptr, ejoin_ = nullptr;
   DevBuf<double> slotXyz_;  // 4 per member slot, for the wave evaluation
   std::vector<char> chunkEval_;  // per patch: evaluated in chunks
   void require_register_patches(const char *what) const;
