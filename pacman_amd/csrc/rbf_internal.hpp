@@ -241,6 +241,11 @@ struct SolveArgs {
   const double *slotXyz;
 };
 constexpr int kHeavyPairs = 4096;
+// Doubles per source in the alpha records of the patches evaluated apart
+// (zero past F): 1 for one field, 4 for up to four, 8 above.
+__host__ __device__ inline int alpha_stride(int F) {
+  return F == 1 ? 1 : (F <= 4 ? 4 : 8);
+}
 
 // ---- workspace solve (rbf_solve_ws.hip), patches of any size -------------
 constexpr int kWsBlock = 32;   // block edge of the workspace factorisation
@@ -284,7 +289,8 @@ void launch_ws_reset(int *counters, bool all, hipStream_t s,
 void launch_ws_solve(bool fp32, int kind, const WsArgs &w, int grid,
                      hipStream_t s);
 // Evaluation of the patches whose alpha the solve stored (alpha of source
-// t at a.alphaHeavy + a.heavyOff[k] + 8 t + f, zero past F; polynomial from
+// t at a.alphaHeavy + a.heavyOff[k] + alpha_stride(F) t + f, zero past F
+// within the record; polynomial from
 // coef; source coordinates from a.slotXyz): chunks[c] = (patch, first pair)
 // of up to 64 pairsPerLane (1 to 3) pairs, one wave each.
 void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,

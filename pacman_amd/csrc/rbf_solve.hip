@@ -687,13 +687,14 @@ __global__ void src_reduce_kernel(const int64_t *off, const int *ent,
 // whose solve stored its alpha, by one wave (src/rbf_pum/solver/solver.hpp:
 // 265-273): lane l holds pairs q0 + l + 64 h, h < PPL (the staged reads
 // below serve all of them; from PPL = 2 on the VALU, not the LDS, sets the
-// pace), sums over the patch's sources in source order. The wave stages its patch's sources kBlk at a time in
-// its own LDS region (coalesced vector loads of the slot records: x y z 0
-// from slotXyz, 8 alpha from alphaHeavy, zero past F), loading the next
-// block while it computes this one; no workgroup barriers, so the waves of
-// many patches stay in flight. NF products per source: 8 for more than four
-// fields, else 4 (the records hold zeros past F); the phi code is the same
-// in both, so a field's result does not depend on the field count.
+// pace), sums over the patch's sources in source order. The wave stages
+// its patch's sources kBlk at a time in its own LDS region (coalesced
+// vector loads of the slot records: x y z 0 from slotXyz, NF alpha from
+// alphaHeavy, zero past F), loading the next block while it computes this
+// one; no workgroup barriers, so the waves of many patches stay in flight.
+// NF = alpha_stride(F) products per source (1, 4 or 8); the phi code is
+// the same for every NF, so a field's result does not depend on the field
+// count.
 constexpr int kBlk = 32;
 
 template <int K, int NF, int PPL>
@@ -701,7 +702,10 @@ __global__ __launch_bounds__(256) void wave_eval_kernel(
     SolveArgs a, const double *__restrict__ coef,
     const int2 *__restrict__ chunks, int nChunks) {
   __shared__ double sxyz[4][kBlk][4];
-  __shared__ double sal[4][kBlk * kFMax];
+  __shared__ double sal[4][kBlk * NF];
+  // alpha doubles per lane and block (lanes below kBlk * NF / VW load)
+  constexpr int VW = kBlk * NF >= 64 ? kBlk * NF / 64 : 1;
+  constexpr int VL = kBlk * NF / VW;
   const int wv = threadIdx.x >> 6;
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + wv));
   if (w >= nChunks)
@@ -736,16 +740,33 @@ __global__ __launch_bounds__(256) void wave_eval_kernel(
       slot[h] = a.pairSlot[p0 + q];
     }
   }
-  // staged block: lane l < kBlk holds source b0 + l's x y z 0, every lane 4
-  // doubles of the block's alpha records (8 per source)
-  double4 cxyz = {0.0, 0.0, 0.0, 0.0}, c4 = {0.0, 0.0, 0.0, 0.0};
+  // staged block: lane l < kBlk holds source b0 + l's x y z 0, lane l < VL
+  // VW doubles of the block's alpha records (NF per source)
+  double4 cxyz = {0.0, 0.0, 0.0, 0.0};
+  double cv[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v)
+    cv[v] = 0.0;
   auto load = [&](int b0) {
     const int i = b0 + lane;
     if (lane < kBlk && i < n)
       cxyz = *reinterpret_cast<const double4 *>(xyz + 4 * (int64_t)i);
-    const int e = kFMax * b0 + 4 * lane;
-    if (e < kFMax * n)
-      c4 = *reinterpret_cast<const double4 *>(al + e);
+    const int e = NF * b0 + VW * lane;
+    if (lane < VL && e < NF * n) {
+      if constexpr (VW == 4) {
+        const double4 t = *reinterpret_cast<const double4 *>(al + e);
+        cv[0] = t.x;
+        cv[1] = t.y;
+        cv[2] = t.z;
+        cv[3] = t.w;
+      } else if constexpr (VW == 2) {
+        const double2 t = *reinterpret_cast<const double2 *>(al + e);
+        cv[0] = t.x;
+        cv[1] = t.y;
+      } else {
+        cv[0] = al[e];
+      }
+    }
   };
   auto wave_sync = [] {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -762,7 +783,10 @@ __global__ __launch_bounds__(256) void wave_eval_kernel(
   for (int b0 = 0; b0 < n; b0 += kBlk) {
     if (lane < kBlk)
       *reinterpret_cast<double4 *>(&sxyz[wv][lane][0]) = cxyz;
-    *reinterpret_cast<double4 *>(&sal[wv][4 * lane]) = c4;
+    if (lane < VL)
+#pragma unroll
+      for (int v = 0; v < VW; ++v)
+        sal[wv][VW * lane + v] = cv[v];
     wave_sync();
     if (b0 + kBlk < n)
       load(b0 + kBlk);
@@ -777,7 +801,7 @@ __global__ __launch_bounds__(256) void wave_eval_kernel(
                                       dinv);
 #pragma unroll
         for (int f = 0; f < NF; ++f)
-          acc[h][f] = fma(ph, sal[wv][j * kFMax + f], acc[h][f]);
+          acc[h][f] = fma(ph, sal[wv][j * NF + f], acc[h][f]);
       }
     }
     wave_sync();  // this block read before the next one is written
@@ -857,10 +881,13 @@ void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,
     else
       go(nf, I3{});
   };
-  if (a.F > 4)
+  const int S = alpha_stride(a.F);
+  if (S == 8)
     byPpl(I8{});
-  else
+  else if (S == 4)
     byPpl(I4{});
+  else
+    byPpl(std::integral_constant<int, 1>{});
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 

@@ -726,13 +726,15 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
         for (int f = 0; f < F; ++f)
           a.alphaOut[(s0 + t) * F + f] = rhs[f][t];
     if (a.heavyOff && a.heavyOff[patch] >= 0) {
-      // evaluated in chunks across many workgroups; alpha of source t at
-      // ah[t * kFMax + f], zero past F (one 64-byte record per source)
+      // evaluated in chunks across many waves; alpha of source t at
+      // ah[t * S + f], zero past F (one record of S doubles per source)
       double *ah = a.alphaHeavy + a.heavyOff[patch];
+      const int S = alpha_stride(F);
       for (int t = tid; t < n; t += NT)
 #pragma unroll
         for (int f = 0; f < kFMax; ++f)
-          ah[t * kFMax + f] = f < F ? rhs[f][t] : 0.0;
+          if (f < S)
+            ah[t * S + f] = f < F ? rhs[f][t] : 0.0;
       return;
     }
   }

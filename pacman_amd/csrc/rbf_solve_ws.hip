@@ -591,12 +591,13 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
         a.alphaOut[(s0 + c) * F + f] = alp[f * ast + c];
       }
     if (a.heavyOff && a.heavyOff[patch] >= 0) {
-      // evaluated in chunks across many workgroups; alpha of source c at
-      // ah[c * kFMax + f], zero past F
+      // evaluated in chunks across many waves; alpha of source c at
+      // ah[c * S + f], zero past F
       double *ah = a.alphaHeavy + a.heavyOff[patch];
-      for (int t = tid; t < kFMax * n; t += NT) {
+      const int S = alpha_stride(F);
+      for (int t = tid; t < S * n; t += NT) {
         const int f = t / n, c = t % n;
-        ah[c * kFMax + f] = f < F ? alp[f * ast + c] : 0.0;
+        ah[c * S + f] = f < F ? alp[f * ast + c] : 0.0;
       }
       if (tid == 0 && a.status)
         a.status[patch] = status;
