@@ -1035,6 +1035,7 @@ void RbfPumPlan::patchInfo(int *pv, int *status, hipStream_t s) const {
 // (PACMAN_AMD_WS_BUDGET_MB, default 8192). Launches are stream-ordered, so
 // the queues share one buffer.
 void RbfPumPlan::plan_workspace(hipStream_t s) {
+  TraceRange tr("pacman::plan workspace queues");
   wsHost_.clear();
   for (auto &q : wsQ_) {
     q.host.clear();
@@ -1108,14 +1109,20 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
 // (their alpha is ready once the size bins are done), those of workspace
 // patches after it.
 void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
+  TraceRange tr("pacman::plan evaluation chunks");
   for (int g = 0; g < 2; ++g)
     for (int i = 0; i < 3; ++i)
       nEvalChunks_[g][i] = 0;
   chunkEval_.assign(K_, 0);
   if (K_ == 0)
     return;
-  std::vector<int64_t> hoff(K_, -1);
-  std::vector<int2> lists[2][3];
+  // host copies kept until the constructor's closing synchronisation
+  std::vector<int64_t> &hoff = evalHostOff_;
+  hoff.assign(K_, -1);
+  auto &lists = evalHostChunks_;
+  for (auto &g : lists)
+    for (auto &l : g)
+      l.clear();
   int64_t total = 0;
   for (int64_t k = 0; k < K_; ++k) {
     const int64_t m = poff[k + 1] - poff[k];
@@ -1158,7 +1165,6 @@ void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
   slotXyz_.alloc((size_t)4 * (totalSrc_ + 1));  // one spare record
   launch_slot_xyz(srcIdx_.get(), totalSrc_, gs_.view.x, gs_.view.y,
                   gs_.view.z, slotXyz_.get(), s);
-  PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   for (int g = 0; g < 2; ++g)
     for (int i = 0; i < 3; ++i)
       nEvalChunks_[g][i] = (int)lists[g][i].size();

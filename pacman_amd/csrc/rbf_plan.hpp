@@ -196,6 +196,8 @@ private:
   hipEvent_t efork_ = nullptr, ejoin_ = nullptr;
   DevBuf<double> slotXyz_;  // 4 per member slot, for the wave evaluation
   std::vector<char> chunkEval_;  // per patch: evaluated in chunks
+  std::vector<int64_t> evalHostOff_;               // sources of the copies
+  std::vector<int2> evalHostChunks_[2][3];         // to the device above
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, const double *coef, hipStream_t s,
                         bool timed, bool fp32);
