@@ -171,17 +171,20 @@ def main() -> int:
     out0 = out[0] if out.ndim == 2 else out
     rel_l2 = float(torch.linalg.norm(out0 - exact) / torch.linalg.norm(exact))
 
-    # Dominant kernel: the fused per-patch solve, one launch per size bin
-    # (the polynomial fit runs before it in its own kernel and is timed
-    # apart). Algorithmic flops per patch of n sources and m member targets,
-    # counted as for the reference's chain: unpivoted LU (2/3) n^3, forward +
-    # back substitution 2 n^2, evaluation 2 n m; kernel-value evaluations are
-    # not counted. The symmetric LDL^T executed here needs n^3 / 3 for the
+    # Dominant kernel: the per-patch solve, one launch per size bin (the
+    # polynomial fit runs before it and the evaluation after it, each in
+    # kernels of their own, timed apart). Algorithmic flops per patch of n
+    # sources and m member targets it evaluates itself, counted as for the
+    # reference's chain: unpivoted LU (2/3) n^3, forward + back substitution
+    # 2 n^2 per field, evaluation 2 n m per field (none: every register-
+    # resident patch is evaluated apart); kernel-value evaluations are not
+    # counted. The symmetric LDL^T executed here needs n^3 / 3 for the
     # factorisation; that rate is reported as executed_frac.
-    f_alg = (2.0 / 3.0) * work["sum_n3"] + 2 * work["sum_n2"] + \
-        2 * work["sum_nm"]
-    f_exec = (1.0 / 3.0) * work["sum_n3"] + 2 * work["sum_n2"] + \
-        2 * work["sum_nm"]
+    Fk = args.fields
+    f_alg = (2.0 / 3.0) * work["tile_sum_n3"] + Fk * (
+        2 * work["tile_sum_n2"] + 2 * work["tile_sum_nm"])
+    f_exec = (1.0 / 3.0) * work["tile_sum_n3"] + Fk * (
+        2 * work["tile_sum_n2"] + 2 * work["tile_sum_nm"])
     kern_ms = ph["tiles"]
     achieved = f_alg / (kern_ms / 1e3) / 1e12
     traffic = None
