@@ -928,6 +928,9 @@ void launch_solve_f64(int R, int waves, int kind, const SolveArgs &a,
     throw std::runtime_error("solve: unknown Wendland kind");
   if (!solve_tile_variant(R, waves))
     throw std::runtime_error("solve: unsupported size bin");
+  if (a.heavyOff == nullptr || a.alphaHeavy == nullptr)
+    throw std::runtime_error("solve: alpha records (the evaluation runs "
+                             "apart) missing");
   if (waves == 16) {
     switch (R) {
     case 11: launch_solve_tile<11, 16>(kind, a, coef, s); break;

@@ -38,8 +38,9 @@
 // block: the control wave computes alpha_B = Linv_B^T x_B as matrix-vector
 // products (with the sub-diagonal tile staged in LDS), the tile waves add
 // L_IJ^T alpha_I to the sums of the blocks further up as soon as alpha_I is
-// known. Evaluation at the member targets then follows
-// src/rbf_pum/solver/solver.hpp:265-273.
+// known. The forward kernel then stores alpha, one record per source, for
+// the evaluation at the member targets (src/rbf_pum/solver/solver.hpp:
+// 265-273), which runs apart in wave_eval_kernel (rbf_solve.hip).
 #pragma once
 
 #include "rbf_solve_common.hpp"
@@ -174,10 +175,12 @@ template <int R, int NW_ = 8> struct TileShape {
   static_assert(TPW <= kMaxTilesPerWave, "tiles per wave");
   static constexpr int LDW = NMAX + 4;      // panel column stride (doubles)
   static constexpr int PANEL = 2 * 16 * LDW;
-  static constexpr int EVAL = NW * kFMax * 64;
   static constexpr int BACK = (R - 1) * 16 * 17 + R * kFMax * 16;
-  static constexpr int SBUF0 = PANEL > EVAL ? PANEL : EVAL;
-  static constexpr int SBUF = SBUF0 > BACK ? SBUF0 : BACK;
+  // the transpose chain stages its right-hand side in the same buffer:
+  // target coordinates [3][64], h [kFMax][64], second halves [kFMax][256]
+  static constexpr int ADJRHS = (3 + kFMax) * 64 + kFMax * 256;
+  static constexpr int SBUF0 = PANEL > BACK ? PANEL : BACK;
+  static constexpr int SBUF = SBUF0 > ADJRHS ? SBUF0 : ADJRHS;
 };
 
 // ADJ: the transpose chain. The right-hand side is E^T h with h = w g at
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
 
   __shared__ double px[NMAX], py[NMAX], pz[NMAX];
   __shared__ double rhs[kFMax][NMAX];
-  __shared__ double sbuf[TS::SBUF];          // panels, later eval partials
+  __shared__ double sbuf[TS::SBUF];  // panels, back substitution, rhs (ADJ)
   __shared__ double linv[2][16][17];         // Linv of the diagonal block
   __shared__ double linvAll[R][120];         // strictly-lower Linv, packed
   __shared__ double dtile[16][17];           // diagonal tile hand-over
@@ -334,6 +337,7 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
     // right-hand side E^T h, h = w g at the member targets: tiles of 64
     // pairs staged in LDS; thread (j, half) sums half of each tile for
     // source j; the halves are added in a fixed order.
+    static_assert(TS::SBUF >= TS::ADJRHS, "rhs staging fits the buffer");
     double *sty = sbuf;                        // [3][64] target coordinates
     double *sth = sbuf + 3 * 64;               // [kFMax][64] h
     double *sred = sbuf + (3 + kFMax) * 64;    // [kFMax][256] second halves
@@ -725,169 +729,15 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
       for (int t = tid; t < n; t += NT)
         for (int f = 0; f < F; ++f)
           a.alphaOut[(s0 + t) * F + f] = rhs[f][t];
-    if (a.heavyOff && a.heavyOff[patch] >= 0) {
-      // evaluated in chunks across many waves; alpha of source t at
-      // ah[t * S + f], zero past F (one record of S doubles per source)
-      double *ah = a.alphaHeavy + a.heavyOff[patch];
-      const int S = alpha_stride(F);
-      for (int t = tid; t < n; t += NT)
+    // the evaluation runs apart (wave_eval_kernel): alpha of source t at
+    // ah[t * S + f], zero past F (one record of S doubles per source)
+    double *ah = a.alphaHeavy + a.heavyOff[patch];
+    const int S = alpha_stride(F);
+    for (int t = tid; t < n; t += NT)
 #pragma unroll
-        for (int f = 0; f < kFMax; ++f)
-          if (f < S)
-            ah[t * S + f] = f < F ? rhs[f][t] : 0.0;
-      return;
-    }
-  }
-
-  // evaluation at the member targets: thread = (pair, slice) over chunks of
-  // up to NT pairs. With c pairs in a chunk, S = NT / c slices of
-  // ceil(n / S) sources each (four at a time into four partial sums); the
-  // slices' sums are added in slice order. S depends only on n and the pair
-  // count, so the result does not depend on the number of fields.
-  {
-    // With more than four fields, alpha with the fields fastest (alT[j][f],
-    // zero past F) in place of rhs: one 16-byte LDS read gives two fields of
-    // a source, and all kFMax products are formed (the same sums for the
-    // first F). With up to four, rhs as it is.
-    const bool wide = F > 4;
-    double *alT = &rhs[0][0];
-    if (wide) {
-      double v[(kFMax * NMAX + NT - 1) / NT];
-#pragma unroll
-      for (int u = 0; u < (kFMax * NMAX + NT - 1) / NT; ++u) {
-        const int idx = tid + u * NT;  // (j, f) of alT
-        const int j = idx / kFMax, f = idx % kFMax;
-        v[u] = (idx < kFMax * NMAX && f < F && j < n) ? rhs[f][j] : 0.0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < (kFMax * NMAX + NT - 1) / NT; ++u) {
-        const int idx = tid + u * NT;
-        if (idx < kFMax * NMAX)
-          alT[idx] = v[u];
-      }
-      __syncthreads();
-    }
-    double *epart = sbuf;  // [S][F][c], S * c <= NT
-    const int64_t p0 = a.pairOff[patch], p1 = a.pairOff[patch + 1];
-    const int mall = (int)(p1 - p0);
-    for (int c0 = 0; c0 < mall; c0 += NT) {
-      const int c = min(NT, mall - c0);
-      const int S = NT / c;
-      const int pi = tid % c, sl = tid / c;
-      const bool act = sl < S;
-      const int qn = (n + S - 1) / S;
-      const int l0 = min(n, sl * qn), l1 = min(n, l0 + qn);
-      int e = 0;
-      double qx = 0.0, qy = 0.0, qz = 0.0;
-      if (act) {
-        e = a.pairTgt[p0 + c0 + pi];
-        const int tp = e & 0x7fffffff;
-        qx = a.tx[tp];
-        qy = a.ty[tp];
-        qz = a.tz[tp];
-      }
-      const bool mem = act && ((unsigned)e >> 31) != 0;
-      // four partial sums per field: chain k takes the sources l0 + 4i + k
-      // of the full groups of four, chain 0 then the remaining ones, and
-      // the slice's sum is (chain 0 + chain 1) + (chain 2 + chain 3)
-      double acc0[kFMax], acc1[kFMax], acc2[kFMax], acc3[kFMax];
-#pragma unroll
-      for (int f = 0; f < kFMax; ++f) {
-        acc0[f] = 0.0;
-        acc1[f] = 0.0;
-        acc2[f] = 0.0;
-        acc3[f] = 0.0;
-      }
-      if (mem) {
-        with_kind([&](auto kc) __attribute__((always_inline)) {
-          constexpr int K = decltype(kc)::value;
-          auto phi = [&](int l) __attribute__((always_inline)) {
-            return wendland<K, double, true>(
-                dist_fast(qx, qy, qz, px[l], py[l], pz[l]), dinv);
-          };
-          if (wide) {
-            // one chain after the other: 8 fields x at most three chains
-            // live, where four interleaved chains exceed the registers
-            const int nq = (l1 - l0) >> 2;
-            auto chain = [&](int l, int end, int step, double *acc)
-                __attribute__((always_inline)) {
-              for (; l < end; l += step) {
-                const double w = phi(l);
-#pragma unroll
-                for (int f = 0; f < kFMax; ++f)
-                  acc[f] = fma(w, alT[l * kFMax + f], acc[f]);
-              }
-            };
-            chain(l0, l0 + 4 * nq, 4, acc0);
-            chain(l0 + 4 * nq, l1, 1, acc0);
-            chain(l0 + 1, l0 + 4 * nq, 4, acc1);
-#pragma unroll
-            for (int f = 0; f < kFMax; ++f) {
-              acc0[f] = acc0[f] + acc1[f];
-              acc1[f] = 0.0;
-            }
-            chain(l0 + 2, l0 + 4 * nq, 4, acc1);
-            chain(l0 + 3, l0 + 4 * nq, 4, acc2);
-#pragma unroll
-            for (int f = 0; f < kFMax; ++f)
-              acc0[f] = acc0[f] + (acc1[f] + acc2[f]);
-            return;
-          }
-          int l = l0;
-          for (; l + 3 < l1; l += 4) {
-            const double f0 = phi(l), f1 = phi(l + 1), f2 = phi(l + 2),
-                         f3 = phi(l + 3);
-#pragma unroll
-            for (int f = 0; f < kFMax; ++f)
-              if (f < F) {
-                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
-                acc1[f] = fma(f1, rhs[f][l + 1], acc1[f]);
-                acc2[f] = fma(f2, rhs[f][l + 2], acc2[f]);
-                acc3[f] = fma(f3, rhs[f][l + 3], acc3[f]);
-              }
-          }
-          for (; l < l1; ++l) {
-            const double f0 = phi(l);
-#pragma unroll
-            for (int f = 0; f < kFMax; ++f)
-              if (f < F)
-                acc0[f] = fma(f0, rhs[f][l], acc0[f]);
-          }
-#pragma unroll
-          for (int f = 0; f < kFMax; ++f)
-            acc0[f] = (acc0[f] + acc1[f]) + (acc2[f] + acc3[f]);
-        });
-      }
-      if (act) {
-#pragma unroll
-        for (int f = 0; f < kFMax; ++f)
-          if (f < F)
-            epart[(sl * F + f) * c + pi] = acc0[f];
-      }
-      __syncthreads();
-      if (tid < c) {
-        const int ee = a.pairTgt[p0 + c0 + tid];
-        if (((unsigned)ee >> 31) != 0) {
-          const int tp = ee & 0x7fffffff;
-          const double yx = a.tx[tp], yy = a.ty[tp], yz = a.tz[tp];
-          const int slot = a.pairSlot[p0 + c0 + tid];
-#pragma unroll
-          for (int f = 0; f < kFMax; ++f)
-            if (f < F) {
-              double t = epart[f * c + tid];
-              for (int q = 1; q < S; ++q)
-                t += epart[(q * F + f) * c + tid];
-              // V beta (solver.hpp:272), around the centre
-              const double vb = scoef[f][0] + scoef[f][1] * (yx - cx) +
-                                scoef[f][2] * (yy - cy) +
-                                scoef[f][3] * (yz - cz);
-              a.contrib[(int64_t)slot * F + f] = t + vb;
-            }
-        }
-      }
-      __syncthreads();
-    }
+      for (int f = 0; f < kFMax; ++f)
+        if (f < S)
+          ah[t * S + f] = f < F ? rhs[f][t] : 0.0;
   }
 }
 
