@@ -22,15 +22,20 @@ def cloud(seed, n, m, dim=3):
     return rng.uniform(0, 1, (n, dim)), rng.uniform(0, 1, (m, dim))
 
 
-@pytest.mark.parametrize("rbf,dim", [(0x11, 3), (0x13, 3), (0x11, 2)])
-def test_transpose_equals_oracle_operator_transpose(oracle_mod, rbf, dim):
+# npc = 8: patches of about ten sources (one or two 16-row tile blocks), where
+# the transpose chain's right-hand-side staging is the largest user of the
+# solve kernel's scratch buffer
+@pytest.mark.parametrize("rbf,dim,npc", [(0x11, 3, 40), (0x13, 3, 40),
+                                         (0x11, 2, 40), (0x11, 3, 8)])
+def test_transpose_equals_oracle_operator_transpose(oracle_mod, rbf, dim,
+                                                    npc):
     S, T = cloud(3 + dim, 400, 300, dim)
-    ora = oracle_mod.Setup(dim, rbf, S, T, 40, 0.15, -1.0)
+    ora = oracle_mod.Setup(dim, rbf, S, T, npc, 0.15, -1.0)
     Op = ora.apply(np.eye(len(S)))            # (N, M): row j = Op e_j
     rng = np.random.default_rng(11)
     g = rng.normal(size=(2, len(T)))
     ref = g @ Op.T                            # Op^T g per field, (2, N)
-    plan = pacman.rbf.Plan(dim, rbf, S, T, 40, 0.15, -1.0)
+    plan = pacman.rbf.Plan(dim, rbf, S, T, npc, 0.15, -1.0)
     got = plan.apply_transpose(g)
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert err < 1e-10, err
