@@ -220,10 +220,10 @@ int pacman_rbf_plan_phase_times(pacman_rbf_plan plan, double *ms, int n);
  * 1 sum n_k, 2 sum n_k^2, 3 sum n_k^3, 4 sum n_k m_k (member targets),
  * 5 target/patch pairs within r, 6 largest n_k; over the patches above 264
  * sources: 7 count, 8 sum n_k^3, 9 sum n_k^2, 10 sum n_k m_k of those the
- * workspace kernel evaluates; 11 sum n_k m_k of those evaluated in chunks
- * (every patch of up to 264 sources and every one with more than 4096
- * target/patch pairs); over the patches of up to 264 sources: 12 sum
- * n_k^3, 13 sum n_k^2, 14 sum n_k m_k of those their solve evaluates. */
+ * workspace kernel evaluates (none: every patch is evaluated apart from its
+ * solve); 11 sum n_k m_k of those evaluated in chunks (all); over the
+ * patches of up to 264 sources: 12 sum n_k^3, 13 sum n_k^2, 14 sum n_k m_k
+ * of those their solve evaluates (none). */
 int pacman_rbf_plan_work(pacman_rbf_plan plan, double *work, int n);
 
 const char *pacman_rbf_last_error(void);

@@ -229,11 +229,11 @@ struct SolveArgs {
   // patch; each one found adds 1 to *singular (when set).
   double phi0;
   int *singular;
-  // Patches evaluated apart from their solve (every register-resident one,
-  // and those with more member-target pairs than kHeavyPairs): heavyOff[k]
-  // >= 0 is where their alpha goes (alphaHeavy + heavyOff[k] + e * F + f);
-  // the solve kernels store it and skip the evaluation, which
-  // launch_wave_eval spreads over many waves.
+  // Patches evaluated apart from their solve (every patch of the plans
+  // built here): heavyOff[k] >= 0 is where their alpha records go
+  // (alphaHeavy + heavyOff[k] + alpha_stride(F) e + f); the solve kernels
+  // store them and skip the evaluation, which launch_wave_eval spreads
+  // over many waves.
   const int64_t *heavyOff;
   double *alphaHeavy;
   // x y z 0 of the source in each member slot (4 per slot, slot order);

@@ -1093,15 +1093,17 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));  // queues copied
 }
 
-// The solve kernels store alpha (alphaHeavy_, n x 8 doubles per patch) and
-// skip the evaluation for:
-//  * every patch of the register-resident solve: inside that kernel the
-//    evaluation ran at the solve's occupancy (two workgroups per CU, 8 or
-//    16 waves each), where its dependent pair, target and slot loads left
-//    it latency-bound (about a quarter of the solve's time);
-//  * every patch with more than kHeavyPairs (target, patch) pairs (large
-//    cells of sparse regions next to dense ones: up to ~1e5 pairs), which
-//    would keep one workgroup evaluating for milliseconds.
+// The solve kernels store alpha (alphaHeavy_, n x 8 doubles reserved per
+// patch) and leave the evaluation of every patch to the wave evaluation:
+//  * inside the register-resident solve the evaluation ran at the solve's
+//    occupancy (two workgroups per CU, 8 or 16 waves each), where its
+//    dependent pair, target and slot loads left it latency-bound (about a
+//    quarter of the solve's time);
+//  * inside the workspace kernel (one workgroup per CU) it lengthened the
+//    kernel by half (3.5 -> 2.1 ms without it, uneven cloud);
+//  * patches with more than kHeavyPairs (target, patch) pairs (large cells
+//    of sparse regions next to dense ones: up to ~1e5 pairs) would keep one
+//    workgroup evaluating for milliseconds.
 // They are evaluated apart, one wave per chunk of u units of 64 pairs
 // (u pairs per lane): a patch of one unit in one chunk, the others in
 // chunks of two units and, for an odd count, one last chunk of three. The
@@ -1126,8 +1128,6 @@ void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
   int64_t total = 0;
   for (int64_t k = 0; k < K_; ++k) {
     const int64_t m = poff[k + 1] - poff[k];
-    if (m <= kHeavyPairs && patchN_[k] > tileMaxN_)
-      continue;  // evaluated by the workspace kernel
     chunkEval_[k] = 1;
     hoff[k] = total;
     total += (int64_t)patchN_[k] * solve_fields_per_launch();

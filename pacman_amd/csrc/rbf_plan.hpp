@@ -132,11 +132,10 @@ public:
   // Work counters: 0 K, 1 sum n, 2 sum n^2, 3 sum n^3, 4 sum n m (member
   // targets), 5 pairs (targets within r of a centre), 6 max n; over the
   // patches above tileMaxN_: 7 count, 8 sum n^3, 9 sum n^2, 10 sum n m of
-  // those evaluated by the workspace kernel, 11 sum n m of the patches
-  // evaluated in chunks (every register-resident patch and every one with
-  // more than kHeavyPairs pairs); over the patches of the register-resident
-  // solve: 12 sum n^3, 13 sum n^2, 14 sum n m of those it evaluates itself
-  // (none: they are all evaluated in chunks).
+  // those evaluated by the workspace kernel (none), 11 sum n m of the
+  // patches evaluated in chunks (all of them); over the patches of the
+  // register-resident solve: 12 sum n^3, 13 sum n^2, 14 sum n m of those it
+  // evaluates itself (none).
   static constexpr int kWork = 15;
   void work(double *w) const;
   ~RbfPumPlan();
@@ -183,9 +182,8 @@ private:
                        std::chrono::steady_clock::time_point t0);
   void setup_adaptive(const double *dSrc, const double *dTgt, hipStream_t s);
   DevBuf<double> patchR_, patchRinv_, patchDinv_;  // adaptive plans only
-  // Patches evaluated apart from their solve, in chunks of pairs (one wave
-  // each) by their own launches: every register-resident patch and every
-  // patch with more than kHeavyPairs pairs.
+  // Every patch is evaluated apart from its solve, in chunks of pairs (one
+  // wave each) by launches of their own.
   void plan_eval(const std::vector<int64_t> &poff, hipStream_t s);
   DevBuf<int64_t> heavyOff_;
   DevBuf<double> alphaHeavy_;
