@@ -229,12 +229,11 @@ struct SolveArgs {
   // patch; each one found adds 1 to *singular (when set).
   double phi0;
   int *singular;
-  // Patches evaluated apart from their solve (every patch of the plans
-  // built here): heavyOff[k] >= 0 is where their alpha records go
-  // (alphaHeavy + heavyOff[k] + alpha_stride(F) e + f); the solve kernels
-  // store them and skip the evaluation, which launch_wave_eval spreads
-  // over many waves.
-  const int64_t *heavyOff;
+  // Alpha records of the patches evaluated apart from their solve (every
+  // patch of the plans built here): patch k's at alphaHeavy + 8 srcOff[k]
+  // + alpha_stride(F) e + f (8 doubles reserved per member slot); the solve
+  // kernels store them and skip the evaluation, which launch_wave_eval
+  // spreads over many waves. Null: the workspace kernel evaluates itself.
   double *alphaHeavy;
   // x y z 0 of the source in each member slot (4 per slot, slot order);
   // read by the one-wave-per-chunk evaluation

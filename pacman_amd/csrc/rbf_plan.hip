@@ -60,6 +60,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
 
   // Size bins by register tile; per-patch source and pair counts on the
   // host (one synchronisation for both).
+  TraceRange trb("pacman::plan bookkeeping");
   std::vector<int64_t> off(K_ + 1), poff(K_ + 1);
   PACMAN_HIP_CHECK(hipMemcpyAsync(off.data(), srcOff_.get(), 8 * (K_ + 1),
                                   hipMemcpyDeviceToHost, s));
@@ -255,6 +256,7 @@ const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
   Bins &B = bins_[F - 1];
   if (B.built)
     return B;
+  TraceRange tr("pacman::size bins");
   std::vector<std::vector<int>> bins(solve_bin_rows(tileMaxN_, F) + 1);
   for (int64_t kk = 0; kk < K_; ++kk)
     if (patchN_[kk] <= tileMaxN_)  // larger ones: workspace path
@@ -379,7 +381,7 @@ void RbfPumPlan::work(double *w) const {
     sn2 += n * n;
     sn3 += n * n * n;
     snm += n * (double)m;
-    const bool heavy = chunkEval_.size() == (size_t)K_ && chunkEval_[k];
+    const bool heavy = true;  // every patch is evaluated in chunks
     if (heavy)
       hnm += n * (double)m;
     if (n > tileMaxN_) {
@@ -417,6 +419,7 @@ void RbfPumPlan::work(double *w) const {
 void RbfPumPlan::prepare(int F, hipStream_t s) {
   if (F < 1)
     throw std::invalid_argument("apply needs at least one field");
+  TraceRange tr("pacman::apply buffers and size bins");
   const int64_t P = std::max<int64_t>(pairs_.P, 1);
   grow(valsSorted_, (size_t)F * N_);
   const int perSolve = solve_fields_per_launch();
@@ -604,7 +607,6 @@ SolveArgs RbfPumPlan::solveArgs(int F, const double *vals) const {
   a.singular = counters_.get() + kWsSingular;
   a.patchDinv = patchDinv_.n ? patchDinv_.get() : nullptr;
   a.patchRinv = patchRinv_.n ? patchRinv_.get() : nullptr;
-  a.heavyOff = heavyOff_.n ? heavyOff_.get() : nullptr;
   a.alphaHeavy = alphaHeavy_.get();
   a.slotXyz = slotXyz_.n ? slotXyz_.get() : nullptr;
   return a;
@@ -877,7 +879,8 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
       PACMAN_HIP_CHECK(hipStreamWaitEvent(es, efork_, 0));
     }
     for (int i = 2; i >= 0; --i)
-      launch_wave_eval(prm_.kind, a, bu.coef, evalChunks_[0][i].get(),
+      launch_wave_eval(prm_.kind, a, bu.coef,
+                       evalChunks_.get() + evalFirst_[0][i],
                        nEvalChunks_[0][i], i + 1, es);
     if (!wsHost_.empty()) {
       if (f0 > 0)
@@ -886,7 +889,8 @@ void RbfPumPlan::enqueue(const double *dVals, int F, double *dOut,
                        workspace_fp32(nf));
     }
     for (int i = 2; i >= 0; --i)
-      launch_wave_eval(prm_.kind, a, bu.coef, evalChunks_[1][i].get(),
+      launch_wave_eval(prm_.kind, a, bu.coef,
+                       evalChunks_.get() + evalFirst_[1][i],
                        nEvalChunks_[1][i], i + 1, s);
     if (side) {
       PACMAN_HIP_CHECK(hipEventRecord(ejoin_, es));
@@ -1115,59 +1119,64 @@ void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
   for (int g = 0; g < 2; ++g)
     for (int i = 0; i < 3; ++i)
       nEvalChunks_[g][i] = 0;
-  chunkEval_.assign(K_, 0);
   if (K_ == 0)
     return;
-  // host copies kept until the constructor's closing synchronisation
-  std::vector<int64_t> &hoff = evalHostOff_;
-  hoff.assign(K_, -1);
-  auto &lists = evalHostChunks_;
-  for (auto &g : lists)
-    for (auto &l : g)
-      l.clear();
-  int64_t total = 0;
+  // chunks per patch: units u of 64 pairs -> one chunk of one unit, or
+  // chunks of two units and, for an odd u, a last one of three
+  auto classes = [&](int64_t k, int &g, int64_t &u) {
+    g = patchN_[k] > tileMaxN_ ? 1 : 0;
+    u = (poff[k + 1] - poff[k] + 63) / 64;
+  };
+  int64_t cnt[2][3] = {};
   for (int64_t k = 0; k < K_; ++k) {
-    const int64_t m = poff[k + 1] - poff[k];
-    chunkEval_[k] = 1;
-    hoff[k] = total;
-    total += (int64_t)patchN_[k] * solve_fields_per_launch();
-    auto &L = lists[patchN_[k] > tileMaxN_ ? 1 : 0];
-    const int64_t u = (m + 63) / 64;
-    if (u <= 1) {
-      if (m > 0)
-        L[0].push_back(make_int2((int)k, 0));
+    int g;
+    int64_t u;
+    classes(k, g, u);
+    if (u == 1)
+      ++cnt[g][0];
+    else if (u > 1) {
+      cnt[g][1] += (u & 1) ? (u - 3) / 2 : u / 2;
+      cnt[g][2] += u & 1;
+    }
+  }
+  int64_t pos[2][3], total = 0;
+  for (int g = 0; g < 2; ++g)
+    for (int i = 0; i < 3; ++i) {
+      pos[g][i] = evalFirst_[g][i] = total;
+      total += cnt[g][i];
+    }
+  // host copy kept until the constructor's closing synchronisation
+  std::vector<int2> &host = evalHostChunks_;
+  host.resize(std::max<int64_t>(total, 1));
+  for (int64_t k = 0; k < K_; ++k) {
+    int g;
+    int64_t u;
+    classes(k, g, u);
+    if (u == 1) {
+      host[pos[g][0]++] = make_int2((int)k, 0);
       continue;
     }
     int64_t q = 0;
     for (int64_t left = u; left > 0;) {
       const int take = left == 3 ? 3 : 2;
-      L[take - 1].push_back(make_int2((int)k, (int)q));
+      host[pos[g][take - 1]++] = make_int2((int)k, (int)q);
       q += 64 * take;
       left -= take;
     }
   }
-  if (total == 0)
-    return;
-  heavyOff_.alloc(K_);
-  // one spare record: the wave evaluation loads the record past a patch's
-  // last before it knows it is done
-  alphaHeavy_.alloc(total + solve_fields_per_launch());
-  PACMAN_HIP_CHECK(hipMemcpyAsync(heavyOff_.get(), hoff.data(), 8 * K_,
+  evalChunks_.alloc(host.size());
+  PACMAN_HIP_CHECK(hipMemcpyAsync(evalChunks_.get(), host.data(),
+                                  sizeof(int2) * host.size(),
                                   hipMemcpyHostToDevice, s));
-  for (int g = 0; g < 2; ++g)
-    for (int i = 0; i < 3; ++i) {
-      evalChunks_[g][i].alloc(std::max<size_t>(lists[g][i].size(), 1));
-      if (!lists[g][i].empty())
-        PACMAN_HIP_CHECK(hipMemcpyAsync(
-            evalChunks_[g][i].get(), lists[g][i].data(),
-            sizeof(int2) * lists[g][i].size(), hipMemcpyHostToDevice, s));
-    }
+  // alpha records: 8 doubles reserved per member slot (patch k's at
+  // 8 srcOff[k]), plus one spare record
+  alphaHeavy_.alloc((size_t)8 * totalSrc_ + solve_fields_per_launch());
   slotXyz_.alloc((size_t)4 * (totalSrc_ + 1));  // one spare record
   launch_slot_xyz(srcIdx_.get(), totalSrc_, gs_.view.x, gs_.view.y,
                   gs_.view.z, slotXyz_.get(), s);
   for (int g = 0; g < 2; ++g)
     for (int i = 0; i < 3; ++i)
-      nEvalChunks_[g][i] = (int)lists[g][i].size();
+      nEvalChunks_[g][i] = (int)cnt[g][i];
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,

@@ -185,17 +185,16 @@ private:
   // Every patch is evaluated apart from its solve, in chunks of pairs (one
   // wave each) by launches of their own.
   void plan_eval(const std::vector<int64_t> &poff, hipStream_t s);
-  DevBuf<int64_t> heavyOff_;
   DevBuf<double> alphaHeavy_;
   // chunks of the wave evaluation with 1, 2 and 3 pairs per lane, [0] of
-  // the patches of the register-resident solve, [1] of the workspace ones
-  DevBuf<int2> evalChunks_[2][3];
+  // the patches of the register-resident solve, [1] of the workspace ones:
+  // nEvalChunks_[g][i] chunks from evalChunks_ + evalFirst_[g][i]
+  DevBuf<int2> evalChunks_;
+  int64_t evalFirst_[2][3] = {};
   int nEvalChunks_[2][3] = {};
   hipEvent_t efork_ = nullptr, ejoin_ = nullptr;
   DevBuf<double> slotXyz_;  // 4 per member slot, for the wave evaluation
-  std::vector<char> chunkEval_;  // per patch: evaluated in chunks
-  std::vector<int64_t> evalHostOff_;               // sources of the copies
-  std::vector<int2> evalHostChunks_[2][3];         // to the device above
+  std::vector<int2> evalHostChunks_;  // source of the copy to evalChunks_
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, const double *coef, hipStream_t s,
                         bool timed, bool fp32);

@@ -719,7 +719,7 @@ __global__ __launch_bounds__(256) void wave_eval_kernel(
   const int q1 = min((int)(a.pairOff[patch + 1] - p0), q0 + 64 * PPL);
   const int F = a.F;
   const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
-  const double *__restrict__ al = a.alphaHeavy + a.heavyOff[patch];
+  const double *__restrict__ al = a.alphaHeavy + 8 * s0;
   const double *__restrict__ xyz = a.slotXyz + 4 * s0;
   bool mem[PPL];
   double yx[PPL], yy[PPL], yz[PPL];
@@ -928,7 +928,7 @@ void launch_solve_f64(int R, int waves, int kind, const SolveArgs &a,
     throw std::runtime_error("solve: unknown Wendland kind");
   if (!solve_tile_variant(R, waves))
     throw std::runtime_error("solve: unsupported size bin");
-  if (a.heavyOff == nullptr || a.alphaHeavy == nullptr)
+  if (a.alphaHeavy == nullptr)
     throw std::runtime_error("solve: alpha records (the evaluation runs "
                              "apart) missing");
   if (waves == 16) {

@@ -731,7 +731,7 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
           a.alphaOut[(s0 + t) * F + f] = rhs[f][t];
     // the evaluation runs apart (wave_eval_kernel): alpha of source t at
     // ah[t * S + f], zero past F (one record of S doubles per source)
-    double *ah = a.alphaHeavy + a.heavyOff[patch];
+    double *ah = a.alphaHeavy + 8 * s0;
     const int S = alpha_stride(F);
     for (int t = tid; t < n; t += NT)
 #pragma unroll

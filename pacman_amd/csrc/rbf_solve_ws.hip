@@ -590,10 +590,10 @@ __global__ __launch_bounds__(NT, 2) void ws_solve_kernel(WsArgs w) {
         const int f = t / n, c = t % n;
         a.alphaOut[(s0 + c) * F + f] = alp[f * ast + c];
       }
-    if (a.heavyOff && a.heavyOff[patch] >= 0) {
+    if (a.alphaHeavy) {
       // evaluated in chunks across many waves; alpha of source c at
       // ah[c * S + f], zero past F
-      double *ah = a.alphaHeavy + a.heavyOff[patch];
+      double *ah = a.alphaHeavy + 8 * s0;
       const int S = alpha_stride(F);
       for (int t = tid; t < S * n; t += NT) {
         const int f = t / n, c = t % n;
