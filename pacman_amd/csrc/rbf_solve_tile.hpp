@@ -35,8 +35,9 @@
 //      publish the next panel's tiles: the sequential elimination overlaps
 //      the trailing update.
 // Back substitution over 16-row blocks from the bottom, one barrier per
-// block: the control wave computes alpha_B = Linv_B^T x_B as matrix-vector
-// products (with the sub-diagonal tile staged in LDS), the tile waves add
+// block: the control wave computes alpha_B = Linv_B^T x_B for all fields at
+// once as 16 x 16 x 16 MFMA products (with the sub-diagonal tile staged in
+// LDS), the tile waves add
 // L_IJ^T alpha_I to the sums of the blocks further up as soon as alpha_I is
 // known. The forward kernel then stores alpha, one record per source, for
 // the evaluation at the member targets (src/rbf_pum/solver/solver.hpp:
@@ -81,23 +82,6 @@ template <int K> __device__ __forceinline__ double row_bcast_asm(double x) {
                : "=v"(r)
                : "v"(x), "i"(K));
   return r;
-}
-
-// acc += (src of lane K of this 16-lane row) * m as one FP64 FMA with a DPP
-// source; src must not have been written by the two previous instructions
-// (dpp_src_ready after writing it).
-template <int K>
-__device__ __forceinline__ void fmac_bcast_of(double &acc, double src,
-                                              double m) {
-  asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf "
-               "bank_mask:0xf"
-               : "+v"(acc)
-               : "v"(src), "v"(m), "i"(K));
-}
-// Two wait states after src was written (ordered after its producer by the
-// operand), before DPP reads of src.
-__device__ __forceinline__ void dpp_src_ready(double src) {
-  asm volatile("s_nop 1" : : "v"(src));
 }
 
 // Owners of the lower-triangle 16x16 tiles among the NWT tile waves. Tile
@@ -487,55 +471,50 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
     double(*sub)[16][17] = reinterpret_cast<double(*)[16][17]>(sbuf);
     double(*sacc)[kFMax][16] = reinterpret_cast<double(*)[kFMax][16]>(
         sbuf + (R - 1) * 16 * 17);
-    // alpha_{B+1} of field 4 g + lr at row lc, per group g of four fields
-    double alpha1[kFMax / 4] = {};
+    // Per block, two 16 x 16 x 16 FP64 matrix-core products with the fields
+    // as columns (up to 16, F used): y = L_{B+1,B}^T alpha_{B+1}, then
+    // alpha_B = Linv_B^T (x_B - S_B - y). In the accumulator layout lane l,
+    // register v holds row (l >> 4) + 4 v, field l & 15 -- exactly the B
+    // operand of step v of the next product, so alpha and x feed the matrix
+    // cores from registers. The A operands (lane l, step s: row l & 15,
+    // column 4 s + (l >> 4)) come from the staged sub-diagonal tile and the
+    // packed Linv, off the chain; rows past n and entries above the diagonal
+    // carry weight 0, the diagonal of Linv weight 1. A field's column is
+    // formed in the same order whatever the field count.
+    dbl4 aD = {0.0, 0.0, 0.0, 0.0};  // alpha_{B+1}
     for (int B = P - 1; B >= 0; --B) {
-      const bool valid = 16 * B + lc < n;
-      // Lane (lr, lc) takes field 4 g + lr, row lc of the block, for each
-      // group g of four fields. Every lane forms all four partial sums of a
-      // block sum itself (rows 4m + j, in row order, added as
-      // (p0 + p1) + (p2 + p3), so a field gets the same bits whatever the
-      // field count), reading alpha_{B+1} and x_B of the other rows of its
-      // field through DPP row_newbcast. The operands are loaded from LDS
-      // first, off the chain; rows past n and entries above the diagonal
-      // carry weight 0, the diagonal of Linv weight 1.
-      double sb[16], li[16];
+      double sA[4], cA[4];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        sb[r] = (B + 1 < P && 16 * (B + 1) + r < n) ? sub[B][r][lc] : 0.0;
-        li[r] = r > lc ? linvAll[B][r * (r - 1) / 2 + lc]
-                       : (r == lc ? 1.0 : 0.0);
+      for (int s = 0; s < 4; ++s) {
+        const int k = 4 * s + lr;
+        sA[s] = (B + 1 < P && 16 * (B + 1) + k < n) ? sub[B][k][lc] : 0.0;
+        cA[s] = k > lc ? linvAll[B][k * (k - 1) / 2 + lc]
+                       : (k == lc ? 1.0 : 0.0);
       }
-      static_for<0, kFMax / 4>([&](auto gc) __attribute__((always_inline)) {
-        constexpr int g = decltype(gc)::value;
-        if (4 * g < F) {
-          const int f = 4 * g + lr;
-          const bool fv = f < F;
-          const double rb = (valid && fv)
-                                ? rhs[f][16 * B + lc] - sacc[B][f][lc]
-                                : 0.0;
-          double yp[4] = {0.0, 0.0, 0.0, 0.0};
-          if (B + 1 < P) {
-            dpp_src_ready(alpha1[g]);
-            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-              constexpr int r = decltype(rc)::value;
-              fmac_bcast_of<r>(yp[r & 3], alpha1[g], sb[r]);
-            });
-          }
-          const double y = (yp[0] + yp[1]) + (yp[2] + yp[3]);
-          const double x = (valid && fv) ? rb - y : 0.0;
-          double pp[4] = {0.0, 0.0, 0.0, 0.0};
-          dpp_src_ready(x);
-          static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            fmac_bcast_of<r>(pp[r & 3], x, li[r]);
-          });
-          const double part = (pp[0] + pp[1]) + (pp[2] + pp[3]);
-          if (valid && fv)
-            rhs[f][16 * B + lc] = part;
-          alpha1[g] = part;
-        }
-      });
+      dbl4 y = {0.0, 0.0, 0.0, 0.0};
+      if (B + 1 < P) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          y = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[s], aD[s], y, 0, 0, 0);
+      }
+      dbl4 x;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = lr + 4 * v;
+        const bool ok = 16 * B + row < n && lc < F;
+        x[v] = ok ? (rhs[lc][16 * B + row] - sacc[B][lc][row]) - y[v] : 0.0;
+      }
+      dbl4 al = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        al = __builtin_amdgcn_mfma_f64_16x16x4f64(cA[s], x[s], al, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = lr + 4 * v;
+        if (16 * B + row < n && lc < F)
+          rhs[lc][16 * B + row] = al[v];
+      }
+      aD = al;
       __builtin_amdgcn_wave_barrier();
       __syncthreads();  // alpha_B ready
     }
