@@ -453,15 +453,20 @@ __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The scans' default stop test: never stop early.
+struct NoStop {
+  __device__ constexpr bool operator()() const { return false; }
+};
+
 // fn over the queued chunks, in queue order, one 64-lane pass per chunk
 // (lane i: position start + i, valid below count); the coordinates of
 // kChunkGroup chunks are loaded before the first of them is used, so their
-// memory latencies overlap.
-template <class Fn>
+// memory latencies overlap. Ends early once the wave-uniform stop() holds.
+template <class Fn, class Stop = NoStop>
 __device__ inline void flush_chunks(const SortedSet &S, const ScanChunk *pend,
-                                    int np, Fn &&fn) {
+                                    int np, Fn &&fn, Stop stop = Stop{}) {
   const int lane = threadIdx.x & 63;
-  for (int g = 0; g < np; g += kChunkGroup) {
+  for (int g = 0; g < np && !stop(); g += kChunkGroup) {
     double x[kChunkGroup], y[kChunkGroup], z[kChunkGroup];
     int q[kChunkGroup];
     bool v[kChunkGroup], in[kChunkGroup];
@@ -503,12 +508,14 @@ __device__ inline void flush_chunks(const SortedSet &S, const ScanChunk *pend,
 // The ranges to scan are cut into chunks of 64 positions and queued in
 // `pend` (kChunks per wave); fn sees them in the order the ranges are
 // reached, so the positions of the hits are the same as scanning each range
-// when it is reached.
-template <bool INSIDE = true, class Whole, class Fn>
+// when it is reached. A wave-uniform stop() that holds ends the scan early
+// (between ranges and between groups of chunks): fn has then seen a prefix
+// of the candidates.
+template <bool INSIDE = true, class Whole, class Fn, class Stop = NoStop>
 __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
                                  const double *c, double r, ScanNode *stk,
                                  ScanChunk *pend, Whole &&whole, Fn &&fn,
-                                 int onlyCell = -1) {
+                                 int onlyCell = -1, Stop stop = Stop{}) {
   const int lane = threadIdx.x & 63;
   const int d0 = depth_for(f, r);
   const int nr = f.D == 1 ? 3 : f.D == 2 ? 9 : 27;
@@ -552,7 +559,7 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int top = nk;
   int np = 0;
-  while (top > 0) {
+  while (top > 0 && !stop()) {
     const ScanNode nd = stk[top - 1];
     --top;
     const bool inside = INSIDE && nd.re - nd.rs > 64 &&
@@ -588,7 +595,7 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
     }
     for (int b = nd.rs; b < nd.re;) {
       if (np == kChunks) {
-        flush_chunks(S, pend, np, fn);
+        flush_chunks(S, pend, np, fn, stop);
         np = 0;
         wave_sync_lds();  // every lane has read the queue
       }
@@ -602,7 +609,7 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
     }
     wave_sync_lds();  // queue entries visible; popped entry read by all
   }
-  flush_chunks(S, pend, np, fn);
+  flush_chunks(S, pend, np, fn, stop);
   wave_sync_lds();  // the queue may be reused by the next scan
 }
 
@@ -624,11 +631,13 @@ __device__ inline int ball_count(const SortedSet &S, const Frame &f,
   return cnt;
 }
 
-// #{p : sqdiff(c, p) <= t} (candidates within sqrt(t)).
+// #{p : sqdiff(c, p) <= t} (candidates within sqrt(t)) when that is at most
+// cap, else some count above cap: the scan stops as soon as its count passes
+// cap, so a ball over a dense cluster costs a few chunks, not the cluster.
 __device__ inline int count_d2(const SortedSet &S, const Frame &f,
-                               const double *c, double t, ScanNode *stk,
-                               ScanChunk *pend) {
-  int cnt = 0;
+                               const double *c, double t, int cap,
+                               ScanNode *stk, ScanChunk *pend) {
+  int cnt = 0;  // wave-uniform: sums of range lengths and ballot counts
   ball_scan(
       S, f, c, sqrt(t), stk, pend,
       [&](int rs, int re) {
@@ -639,7 +648,7 @@ __device__ inline int count_d2(const SortedSet &S, const Frame &f,
     const bool hit =
         valid && (inside || sqdiff3(c[0], c[1], c[2], x, y, z) <= t);
     cnt += __popcll(__ballot(hit));
-  });
+  }, -1, [&] { return cnt > cap; });
   return cnt;
 }
 
@@ -659,7 +668,9 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
 // (with multiplicity). The upper end starts at r0^2 and doubles until it
 // holds kq points; while more than kKnnCap candidates lie within it, it is
 // bisected on the bits of t (non-negative doubles order as their bit
-// patterns), each count over the pruned ball of radius sqrt(t). Then the
+// patterns), each count over the pruned ball of radius sqrt(t). The counts
+// are capped at max(kq, kKnnCap): each test below (< kq, >= kq, >
+// kKnnCap) reads the same from a capped count as from the exact one. Then the
 // candidates within it are gathered into the wave's LDS buffer (`buf`,
 // kKnnCap doubles) and the kq-th smallest is taken exactly: kq rounds of
 // wave minimum, each removing one instance.
@@ -668,14 +679,15 @@ __device__ inline double knn_distance(const SortedSet &S, const Frame &f,
                                       ScanNode *stk, ScanChunk *pend,
                                       double *buf) {
   double R = r0 > 0.0 ? r0 : 1e-300;
+  const int cap = kq > kKnnCap ? kq : kKnnCap;
   int chi;
-  while ((chi = count_d2(S, f, c, R * R, stk, pend)) < kq)
+  while ((chi = count_d2(S, f, c, R * R, cap, stk, pend)) < kq)
     R *= 2.0;
   uint64_t lo = 0, hi = (uint64_t)__double_as_longlong(R * R);
   while (lo < hi && chi > kKnnCap) {
     const uint64_t mid = lo + (hi - lo) / 2;
-    const int cm =
-        count_d2(S, f, c, __longlong_as_double((long long)mid), stk, pend);
+    const int cm = count_d2(S, f, c, __longlong_as_double((long long)mid),
+                            cap, stk, pend);
     if (cm >= kq) {
       hi = mid;
       chi = cm;

@@ -41,9 +41,24 @@ def _clouds():
 CLOUDS = _clouds()
 
 
-@pytest.mark.parametrize("name", sorted(CLOUDS))
+def _cluster_edge():
+    # A tight cluster (sigma 0.01) in a sparse background: 33 of the 78
+    # cells hold fewer than nmin sources within (1 + overlap) half a
+    # diagonal, and for every one of them the doubled radius reaches the
+    # cluster (up to ~30,000 sources within it), so their nmin-th neighbour
+    # distance comes from the bisection over capped counts.
+    rng = np.random.default_rng(33)
+    S = np.concatenate([rng.normal(0.5, 0.01, (30000, 3)),
+                        rng.uniform(0, 1, (400, 3))])
+    return S, rng.uniform(0, 1, (6000, 3)), 40, 20, 600
+
+
+SETUP_CLOUDS = dict(CLOUDS, **{"3d-cluster-edge": _cluster_edge()})
+
+
+@pytest.mark.parametrize("name", sorted(SETUP_CLOUDS))
 def test_adaptive_setup_bit_exact(name, oracle_mod):
-    S, T, leaf, nmin, nmax = CLOUDS[name]
+    S, T, leaf, nmin, nmax = SETUP_CLOUDS[name]
     dim = S.shape[1]
     o = oracle_mod.AdaptiveSetup(dim, 0x11, S, T, leaf, nmin, nmax)
     plan = pacman.rbf.Plan.adaptive(dim, 0x11, S, T, leaf, nmin, nmax)
