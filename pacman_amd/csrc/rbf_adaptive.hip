@@ -613,10 +613,12 @@ __device__ inline void ball_scan(const SortedSet &S, const Frame &f,
   wave_sync_lds();  // the queue may be reused by the next scan
 }
 
+// #{p : dist(c, p) <= r} when that is at most cap, else some count above
+// cap (the scan stops once its count passes cap).
 __device__ inline int ball_count(const SortedSet &S, const Frame &f,
-                                 const double *c, double r, ScanNode *stk,
-                                 ScanChunk *pend) {
-  int cnt = 0;
+                                 const double *c, double r, int cap,
+                                 ScanNode *stk, ScanChunk *pend) {
+  int cnt = 0;  // wave-uniform: sums of range lengths and ballot counts
   ball_scan(
       S, f, c, r, stk, pend,
       [&](int rs, int re) {
@@ -627,7 +629,7 @@ __device__ inline int ball_count(const SortedSet &S, const Frame &f,
     const bool hit =
         valid && (inside || dist3(c[0], c[1], c[2], x, y, z) <= r);
     cnt += __popcll(__ballot(hit));
-  });
+  }, -1, [&] { return cnt > cap; });
   return cnt;
 }
 
@@ -783,14 +785,18 @@ __global__ __launch_bounds__(kThreads) void cell_eval_kernel(CellEvalArgs e) {
   const int d = e.cellD[i];
   double c[3], rc;
   cell_geometry(p, d, e.f, e.overlap, c, &rc);
-  int n = ball_count(e.S, e.f, c, rc, stk, pend);
+  // A cell above nmax is halved (below the deepest level) and its count is
+  // not kept, so the counts may stop past max(nmax, kq): each test below
+  // reads the same from such a capped count as from the exact one.
+  const int cap = d < e.f.DB ? max(e.nmax, e.kq) : INT_MAX;
+  int n = ball_count(e.S, e.f, c, rc, cap, stk, pend);
   double r = rc;
   if (n < e.kq) {
     r = knn_distance(e.S, e.f, c, e.kq, rc, stk, pend,
                      knnBuf[threadIdx.x >> 6]);
     if (r < rc)  // max(rc, dk) exactly
       r = rc;
-    n = ball_count(e.S, e.f, c, r, stk, pend);
+    n = ball_count(e.S, e.f, c, r, cap, stk, pend);
   }
   if (lane != 0)
     return;
