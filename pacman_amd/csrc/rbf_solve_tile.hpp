@@ -207,34 +207,49 @@ __global__ __launch_bounds__(64 * NW, 4) void solve_tile_kernel(
   const double cx = a.cx[patch], cy = a.cy[patch], cz = a.cz[patch];
   const double dinv = a.patchDinv ? a.patchDinv[patch] : a.dinv;
 
+  // gather; right-hand side = value - polynomial fit. Thread t holds source
+  // row t. The fit coefficients, its coordinates and every field's value are
+  // loaded into registers first, all in flight together; the barrier below
+  // publishes the coefficients, then the rows are formed and written to LDS.
+  static_assert(NMAX <= NT, "one source row per thread");
+  double cf = 0.0;
+  if (!ADJ && tid < kFMax * 4 && (tid >> 2) < F)
+    cf = coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)];
+  double x = 0.0, y = 0.0, z = 0.0;
+  double gv[kFMax];
+#pragma unroll
+  for (int f = 0; f < kFMax; ++f)
+    gv[f] = 0.0;
+  if (tid < n) {
+    const int p = a.srcIdx[s0 + tid];
+    x = a.sx[p];
+    y = a.sy[p];
+    z = a.sz[p];
+    if constexpr (!ADJ) {
+#pragma unroll
+      for (int f = 0; f < kFMax; ++f)
+        if (f < F)
+          gv[f] = a.vals[(int64_t)f * a.N + p];
+    }
+  }
   if (!ADJ && tid < kFMax * 4)
-    scoef[tid >> 2][tid & 3] =
-        (tid >> 2) < F ? coef[((int64_t)patch * F + (tid >> 2)) * 4 + (tid & 3)]
-                       : 0.0;
+    scoef[tid >> 2][tid & 3] = cf;
   if (tid == 0)
     spivot = 0;
-  __syncthreads();
-
-  // gather; right-hand side = value - polynomial fit
-  for (int t = tid; t < NMAX; t += NT) {
-    double x = 0.0, y = 0.0, z = 0.0;
-    int p = 0;
-    if (t < n) {
-      p = a.srcIdx[s0 + t];
-      x = a.sx[p];
-      y = a.sy[p];
-      z = a.sz[p];
-    }
-    px[t] = x;
-    py[t] = y;
-    pz[t] = z;
-    for (int f = 0; f < F; ++f) {
-      double v = 0.0;
-      if (!ADJ && t < n)
-        v = a.vals[(int64_t)f * a.N + p] -
-            (scoef[f][0] + scoef[f][1] * (x - cx) + scoef[f][2] * (y - cy) +
-             scoef[f][3] * (z - cz));
-      rhs[f][t] = v;
+  __syncthreads();  // scoef, spivot
+  if (tid < NMAX) {
+    px[tid] = x;
+    py[tid] = y;
+    pz[tid] = z;
+#pragma unroll
+    for (int f = 0; f < kFMax; ++f) {
+      if (f < F) {
+        double v = 0.0;
+        if (!ADJ && tid < n)
+          v = gv[f] - (scoef[f][0] + scoef[f][1] * (x - cx) +
+                       scoef[f][2] * (y - cy) + scoef[f][3] * (z - cz));
+        rhs[f][tid] = v;
+      }
     }
   }
   __syncthreads();
