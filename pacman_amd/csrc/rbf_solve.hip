@@ -305,9 +305,12 @@ __device__ inline void poly_basis(const SolveArgs &a, int patch, int lane,
   }
 }
 
-// One wave per patch; coef[(patch * F + f) * 4 + k], pvOut[patch].
-__global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
-                                                   double *coef) {
+// One wave per patch; coef[(patch * F + f) * 4 + k], pvOut[patch]. NF = 1
+// for one field, kFMax for 2..kFMax (the per-field sums of the wider
+// variant then fit three waves per SIMD only with the bound below).
+template <int NF>
+__global__ __launch_bounds__(256, NF > 1 ? 3 : 1) void poly_kernel(
+    SolveArgs a, int K, double *coef) {
   const int patch = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (patch >= K)
     return;
@@ -322,19 +325,40 @@ __global__ __launch_bounds__(256) void poly_kernel(SolveArgs a, int K,
   poly_basis(a, patch, lane, c, sc, pv, ca, L);
   // beta'' = G^-1 Q''^T b on the active columns (normal equations of the
   // well-conditioned scaled basis); coefficients returned around the centre.
-  for (int f = 0; f < F; ++f) {
-    double hf[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int t = lane; t < n; t += 64) {
-      const int p = a.srcIdx[s0 + t];
-      const double b = a.vals[(int64_t)f * a.N + p];
-      hf[0] += b;
-      hf[1] += (a.sx[p] - c[0]) * sc * b;
-      hf[2] += (a.sy[p] - c[1]) * sc * b;
-      hf[3] += (a.sz[p] - c[2]) * sc * b;
-    }
+  // Q''^T b of every field in one pass over the sources: each 64-source step
+  // loads the coordinates and all F values together (one dependent round
+  // trip per step, not one per field); a field's sums keep the source order.
+  double hfa[NF][4];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      hf[k] = wave_sum(hf[k]);
+      hfa[f][k] = 0.0;
+  for (int t = lane; t < n; t += 64) {
+    const int p = a.srcIdx[s0 + t];
+    double b[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      b[f] = f < F ? a.vals[(int64_t)f * a.N + p] : 0.0;
+    const double dx = (a.sx[p] - c[0]) * sc, dy = (a.sy[p] - c[1]) * sc,
+                 dz = (a.sz[p] - c[2]) * sc;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      if (f < F) {
+        hfa[f][0] += b[f];
+        hfa[f][1] += dx * b[f];
+        hfa[f][2] += dy * b[f];
+        hfa[f][3] += dz * b[f];
+      }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    if (f >= F)
+      break;
+    double hf[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      hf[k] = wave_sum(hfa[f][k]);
     double y[4], beta[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -907,7 +931,11 @@ void launch_poly(const SolveArgs &a, int64_t K, double *coef, hipStream_t s) {
     return;
   if (a.F < 1 || a.F > kFMax)
     throw std::runtime_error("poly: 1 <= fields <= 8 per launch");
-  poly_kernel<<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, coef);
+  if (a.F == 1)
+    poly_kernel<1><<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K, coef);
+  else
+    poly_kernel<kFMax><<<(unsigned)((K + 3) / 4), 256, 0, s>>>(a, (int)K,
+                                                                coef);
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
