@@ -141,10 +141,68 @@ __global__ void set_last_kernel(T *out, int64_t n, const T *src) {
   out[n] = *src;
 }
 
+// Exclusive scan of at most kSmallScan entries in one launch: one workgroup,
+// thread t scanning entries [kSmallPer t, kSmallPer (t + 1)) in registers,
+// then the thread sums across the workgroup; out[n] = total.
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallPer = 16;
+constexpr int64_t kSmallScan = (int64_t)kSmallThreads * kSmallPer;
+
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(kSmallThreads) void scan_small_kernel(
+    const Tin *in, Tout *out, int64_t n) {
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)t * kSmallPer;
+  Tout v[kSmallPer];
+  Tout sum = 0;
+#pragma unroll
+  for (int k = 0; k < kSmallPer; ++k) {
+    const int64_t idx = base + k;
+    const Tout x = idx < n ? (Tout)in[idx] : (Tout)0;
+    v[k] = sum;
+    sum += x;
+  }
+  const int lane = t % kWave, w = t / kWave;
+  Tout incl = sum;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const Tout u = __shfl_up(incl, o, kWave);
+    if (lane >= o)
+      incl += u;
+  }
+  __shared__ Tout wsum[kSmallThreads / kWave];
+  if (lane == kWave - 1)
+    wsum[w] = incl;
+  __syncthreads();
+  Tout woff = 0;
+  for (int k = 0; k < w; ++k)
+    woff += wsum[k];
+  const Tout excl = woff + incl - sum;
+#pragma unroll
+  for (int k = 0; k < kSmallPer; ++k) {
+    const int64_t idx = base + k;
+    if (idx < n)
+      out[idx] = excl + v[k];
+  }
+  if (t == kSmallThreads - 1)
+    out[n] = excl + sum;
+}
+
 // out has n + 1 entries; out[n] = total. Returns the total (synchronises).
 template <typename Tin, typename Tout>
 void scan_device(const Tin *in, Tout *out, int64_t n, hipStream_t s,
                  bool fetchTotal, Tout *totalHost) {
+  if (n <= kSmallScan) {
+    scan_small_kernel<Tin, Tout><<<1, kSmallThreads, 0, s>>>(in, out,
+                                                             std::max<int64_t>(n, 0));
+    PACMAN_HIP_CHECK(hipGetLastError());
+    if (fetchTotal) {
+      PACMAN_HIP_CHECK(hipMemcpyAsync(totalHost, out + std::max<int64_t>(n, 0),
+                                      sizeof(Tout), hipMemcpyDeviceToHost, s));
+      PACMAN_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return;
+  }
   const int64_t nb = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
   DevBuf<Tout> sums(nb + 1);
   scan_local_kernel<Tin, Tout><<<(unsigned)nb, kBlock, 0, s>>>(in, out, n,
