@@ -299,6 +299,13 @@ void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,
 void launch_slot_xyz(const int *srcIdx, int64_t total, const double *sx,
                      const double *sy, const double *sz, double *xyz,
                      hipStream_t s);
+// The evaluation's chunk list on the device: per (group g, class i) a run of
+// count[g][i] chunks from first[g][i] (group 1: patches above tileMaxN
+// sources; class i: chunks of i + 1 units of 64 pairs), patches in order.
+// Synchronises once (the class sizes).
+void plan_eval_chunks(const int64_t *srcOff, const int64_t *pairOff, int64_t K,
+                      int tileMaxN, DevBuf<int2> &chunks, int64_t first[2][3],
+                      int count[2][3], hipStream_t s);
 // Fields handled by one poly/solve launch.
 int solve_fields_per_launch();
 // Polynomial fit of every patch (one wave per patch): coef[K][F][4], pvOut.

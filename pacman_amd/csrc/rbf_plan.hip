@@ -58,14 +58,11 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
     setup_reference(dSrc, dTgt, s, tSetup);
   auto tPhase = std::chrono::steady_clock::now();
 
-  // Size bins by register tile; per-patch source and pair counts on the
-  // host (one synchronisation for both).
+  // Size bins by register tile: per-patch source counts on the host.
   TraceRange trb("pacman::plan bookkeeping");
-  std::vector<int64_t> off(K_ + 1), poff(K_ + 1);
+  std::vector<int64_t> off(K_ + 1);
   PACMAN_HIP_CHECK(hipMemcpyAsync(off.data(), srcOff_.get(), 8 * (K_ + 1),
                                   hipMemcpyDeviceToHost, s));
-  PACMAN_HIP_CHECK(hipMemcpyAsync(poff.data(), pairs_.off.get(),
-                                  8 * (K_ + 1), hipMemcpyDeviceToHost, s));
   PACMAN_HIP_CHECK(hipStreamSynchronize(s));
   maxN_ = 0;
   for (int64_t kk = 0; kk < K_; ++kk)
@@ -78,7 +75,7 @@ RbfPumPlan::RbfPumPlan(const PlanParams &p, const double *dSrc, int64_t N,
   counters_.alloc(kWsCounters);
   launch_ws_reset(counters_.get(), true, s);
   plan_workspace(s);
-  plan_eval(poff, s);
+  plan_eval(s);
   bins_.resize(solve_fields_per_launch());
   bins_for(1, s);
   pv_.alloc(std::max<int64_t>(K_, 1));
@@ -1114,69 +1111,20 @@ void RbfPumPlan::plan_workspace(hipStream_t s) {
 // chunks of the register-resident patches run beside the workspace kernel
 // (their alpha is ready once the size bins are done), those of workspace
 // patches after it.
-void RbfPumPlan::plan_eval(const std::vector<int64_t> &poff, hipStream_t s) {
+void RbfPumPlan::plan_eval(hipStream_t s) {
   TraceRange tr("pacman::plan evaluation chunks");
-  for (int g = 0; g < 2; ++g)
-    for (int i = 0; i < 3; ++i)
-      nEvalChunks_[g][i] = 0;
-  if (K_ == 0)
-    return;
   // chunks per patch: units u of 64 pairs -> one chunk of one unit, or
   // chunks of two units and, for an odd u, a last one of three
-  auto classes = [&](int64_t k, int &g, int64_t &u) {
-    g = patchN_[k] > tileMaxN_ ? 1 : 0;
-    u = (poff[k + 1] - poff[k] + 63) / 64;
-  };
-  int64_t cnt[2][3] = {};
-  for (int64_t k = 0; k < K_; ++k) {
-    int g;
-    int64_t u;
-    classes(k, g, u);
-    if (u == 1)
-      ++cnt[g][0];
-    else if (u > 1) {
-      cnt[g][1] += (u & 1) ? (u - 3) / 2 : u / 2;
-      cnt[g][2] += u & 1;
-    }
-  }
-  int64_t pos[2][3], total = 0;
-  for (int g = 0; g < 2; ++g)
-    for (int i = 0; i < 3; ++i) {
-      pos[g][i] = evalFirst_[g][i] = total;
-      total += cnt[g][i];
-    }
-  // host copy kept until the constructor's closing synchronisation
-  std::vector<int2> &host = evalHostChunks_;
-  host.resize(std::max<int64_t>(total, 1));
-  for (int64_t k = 0; k < K_; ++k) {
-    int g;
-    int64_t u;
-    classes(k, g, u);
-    if (u == 1) {
-      host[pos[g][0]++] = make_int2((int)k, 0);
-      continue;
-    }
-    int64_t q = 0;
-    for (int64_t left = u; left > 0;) {
-      const int take = left == 3 ? 3 : 2;
-      host[pos[g][take - 1]++] = make_int2((int)k, (int)q);
-      q += 64 * take;
-      left -= take;
-    }
-  }
-  evalChunks_.alloc(host.size());
-  PACMAN_HIP_CHECK(hipMemcpyAsync(evalChunks_.get(), host.data(),
-                                  sizeof(int2) * host.size(),
-                                  hipMemcpyHostToDevice, s));
+  plan_eval_chunks(srcOff_.get(), pairs_.off.get(), K_, tileMaxN_,
+                   evalChunks_, evalFirst_, nEvalChunks_, s);
+  if (K_ == 0)
+    return;
   // alpha records: 8 doubles reserved per member slot (patch k's at
   // 8 srcOff[k]), plus one spare record
   alphaHeavy_.alloc((size_t)8 * totalSrc_ + solve_fields_per_launch());
   slotXyz_.alloc((size_t)4 * (totalSrc_ + 1));  // one spare record
   launch_slot_xyz(srcIdx_.get(), totalSrc_, gs_.view.x, gs_.view.y,
                   gs_.view.z, slotXyz_.get(), s);
-  for (int g = 0; g < 2; ++g)
-    for (int i = 0; i < 3; ++i)
-      nEvalChunks_[g][i] = (int)cnt[g][i];
 }
 
 void RbfPumPlan::launch_workspace(const SolveArgs &a, const double *coef,

@@ -184,7 +184,7 @@ private:
   DevBuf<double> patchR_, patchRinv_, patchDinv_;  // adaptive plans only
   // Every patch is evaluated apart from its solve, in chunks of pairs (one
   // wave each) by launches of their own.
-  void plan_eval(const std::vector<int64_t> &poff, hipStream_t s);
+  void plan_eval(hipStream_t s);
   DevBuf<double> alphaHeavy_;
   // chunks of the wave evaluation with 1, 2 and 3 pairs per lane, [0] of
   // the patches of the register-resident solve, [1] of the workspace ones:
@@ -194,7 +194,6 @@ private:
   int nEvalChunks_[2][3] = {};
   hipEvent_t efork_ = nullptr, ejoin_ = nullptr;
   DevBuf<double> slotXyz_;  // 4 per member slot, for the wave evaluation
-  std::vector<int2> evalHostChunks_;  // source of the copy to evalChunks_
   void require_register_patches(const char *what) const;
   void launch_workspace(const SolveArgs &a, const double *coef, hipStream_t s,
                         bool timed, bool fp32);

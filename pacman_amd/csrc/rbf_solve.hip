@@ -919,6 +919,101 @@ void launch_wave_eval(int kind, const SolveArgs &a, const double *coef,
   PACMAN_HIP_CHECK(hipGetLastError());
 }
 
+// Evaluation chunks of patch k: u = ceil(pairs / 64) units of 64 pairs ->
+// one chunk of one unit (class 0), or chunks of two units (class 1) and, for
+// an odd u, a last one of three (class 2); patches above tileMaxN sources are
+// group 1, the others group 0. cnt holds the six (group, class) counts of
+// every patch, class-major: cnt[(3 g + i) K + k].
+__device__ inline void eval_chunk_shape(const int64_t *srcOff,
+                                        const int64_t *pairOff, int64_t k,
+                                        int tileMaxN, int &g, int64_t &u) {
+  g = srcOff[k + 1] - srcOff[k] > tileMaxN ? 1 : 0;
+  u = (pairOff[k + 1] - pairOff[k] + 63) / 64;
+}
+
+__global__ void eval_chunk_count_kernel(const int64_t *srcOff,
+                                        const int64_t *pairOff, int64_t K,
+                                        int tileMaxN, int *cnt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K)
+    return;
+  int g;
+  int64_t u;
+  eval_chunk_shape(srcOff, pairOff, k, tileMaxN, g, u);
+  const int c[3] = {u == 1 ? 1 : 0,
+                    u > 1 ? (int)((u & 1) ? (u - 3) / 2 : u / 2) : 0,
+                    u > 1 ? (int)(u & 1) : 0};
+  for (int gg = 0; gg < 2; ++gg)
+    for (int i = 0; i < 3; ++i)
+      cnt[(3 * gg + i) * K + k] = gg == g ? c[i] : 0;
+}
+
+// The six class boundaries of the scanned counts (pos has 6 K + 1 entries).
+__global__ void eval_chunk_bounds_kernel(const int64_t *pos, int64_t K,
+                                         int64_t *bounds) {
+  const int c = threadIdx.x;
+  if (c <= 6)
+    bounds[c] = pos[c * K];
+}
+
+// Patch k's chunks at its scanned positions, first unit first.
+__global__ void eval_chunk_fill_kernel(const int64_t *srcOff,
+                                       const int64_t *pairOff, int64_t K,
+                                       int tileMaxN, const int64_t *pos,
+                                       int2 *chunks) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K)
+    return;
+  int g;
+  int64_t u;
+  eval_chunk_shape(srcOff, pairOff, k, tileMaxN, g, u);
+  if (u == 1) {
+    chunks[pos[(3 * g) * K + k]] = make_int2((int)k, 0);
+    return;
+  }
+  int64_t p2 = pos[(3 * g + 1) * K + k], p3 = pos[(3 * g + 2) * K + k];
+  int64_t q = 0;
+  for (int64_t left = u; left > 0;) {
+    const int take = left == 3 ? 3 : 2;
+    chunks[take == 2 ? p2++ : p3++] = make_int2((int)k, (int)q);
+    q += 64 * take;
+    left -= take;
+  }
+}
+
+void plan_eval_chunks(const int64_t *srcOff, const int64_t *pairOff, int64_t K,
+                      int tileMaxN, DevBuf<int2> &chunks, int64_t first[2][3],
+                      int count[2][3], hipStream_t s) {
+  for (int g = 0; g < 2; ++g)
+    for (int i = 0; i < 3; ++i) {
+      first[g][i] = 0;
+      count[g][i] = 0;
+    }
+  if (K <= 0)
+    return;
+  DevBuf<int> cnt((size_t)6 * K);
+  DevBuf<int64_t> pos((size_t)6 * K + 1), bounds(7);
+  const unsigned g = (unsigned)((K + 255) / 256);
+  eval_chunk_count_kernel<<<g, 256, 0, s>>>(srcOff, pairOff, K, tileMaxN,
+                                            cnt.get());
+  PACMAN_HIP_CHECK(hipGetLastError());
+  exclusive_scan_i32_to_i64_async(cnt.get(), pos.get(), 6 * K, s);
+  eval_chunk_bounds_kernel<<<1, 64, 0, s>>>(pos.get(), K, bounds.get());
+  PACMAN_HIP_CHECK(hipGetLastError());
+  int64_t b[7];
+  copy_to_host(b, bounds.get(), sizeof(b), s);
+  for (int c = 0; c < 6; ++c) {
+    if (b[c + 1] - b[c] > INT_MAX)
+      throw std::runtime_error("evaluation: too many chunks in one class");
+    first[c / 3][c % 3] = b[c];
+    count[c / 3][c % 3] = (int)(b[c + 1] - b[c]);
+  }
+  chunks.alloc((size_t)std::max<int64_t>(b[6], 1));
+  eval_chunk_fill_kernel<<<g, 256, 0, s>>>(srcOff, pairOff, K, tileMaxN,
+                                           pos.get(), chunks.get());
+  PACMAN_HIP_CHECK(hipGetLastError());
+}
+
 void launch_slot_xyz(const int *srcIdx, int64_t total, const double *sx,
                      const double *sy, const double *sz, double *xyz,
                      hipStream_t s) {
