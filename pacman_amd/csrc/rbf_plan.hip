@@ -254,19 +254,28 @@ const RbfPumPlan::Bins &RbfPumPlan::bins_for(int F, hipStream_t s) {
   if (B.built)
     return B;
   TraceRange tr("pacman::size bins");
-  std::vector<std::vector<int>> bins(solve_bin_rows(tileMaxN_, F) + 1);
+  // every bin's patch list in one buffer (ascending patch within a bin, bins
+  // by R), one copy: count per bin, then place
+  const int nR = solve_bin_rows(tileMaxN_, F) + 1;
+  std::vector<int64_t> at(nR, 0);
   for (int64_t kk = 0; kk < K_; ++kk)
     if (patchN_[kk] <= tileMaxN_)  // larger ones: workspace path
-      bins[solve_bin_rows(patchN_[kk], F)].push_back((int)kk);
-  // every bin's patch list in one buffer, one copy
-  for (int R = 0; R < (int)bins.size(); ++R) {
-    if (bins[R].empty())
+      ++at[solve_bin_rows(patchN_[kk], F)];
+  int64_t total = 0;
+  for (int R = 0; R < nR; ++R) {
+    const int64_t c = at[R];
+    at[R] = total;
+    if (c == 0)
       continue;
-    B.first.push_back((int64_t)B.host.size());
-    B.host.insert(B.host.end(), bins[R].begin(), bins[R].end());
-    B.count.push_back((int)bins[R].size());
+    B.first.push_back(total);
+    B.count.push_back((int)c);
     B.R.push_back(R);
+    total += c;
   }
+  B.host.resize(total);
+  for (int64_t kk = 0; kk < K_; ++kk)
+    if (patchN_[kk] <= tileMaxN_)
+      B.host[at[solve_bin_rows(patchN_[kk], F)]++] = (int)kk;
   if (!B.host.empty()) {
     B.dev.alloc(B.host.size());
     PACMAN_HIP_CHECK(hipMemcpyAsync(B.dev.get(), B.host.data(),
